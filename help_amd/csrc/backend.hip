@@ -1,0 +1,705 @@
+// MI355X normal-equation backend: handle, schedules, renew/solve, C ABI.
+//
+// help_gpu_create   <- LinearEquation_Construct (UseCholmod.cpp:29-58)
+// help_gpu_renew    <- RenewLinearEquation      (UseCholmod.cpp:84-149)
+// help_gpu_solve    <- SolveLinearEquation      (UseCholmod.cpp:157-238)
+// help_gpu_destroy  <- LinearEquation_Destruct  (UseCholmod.cpp:60-81)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "help_mi355x.h"
+#include "symbolic.h"
+
+using namespace help;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(const std::string& s, int c) : std::runtime_error(s), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_), HELP_ERR_HIP);  \
+    } while (0)
+
+template <typename T>
+struct DBuf {   // owning device buffer
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t cnt) {
+        free();
+        n = cnt;
+        if (cnt) HIPCHK(hipMalloc((void**)&p, cnt * sizeof(T)));
+    }
+    void upload(const std::vector<T>& v, hipStream_t st) {
+        alloc(v.size());
+        if (n) HIPCHK(hipMemcpyAsync(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { free(); }
+};
+
+enum OpKind : int32_t {
+    OP_SMALL = 0, OP_EA, OP_POTRF, OP_TRSM, OP_GEMM,        // factor
+    OP_FWD, OP_FWD_GEMV, OP_BWD_GEMV, OP_BWD                 // solve
+};
+struct Op {
+    OpKind kind;
+    int32_t off, count;
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct help_gpu {
+    int32_t m = 0, n = 0;
+    int64_t nnz = 0;
+    help_opts opts{};
+    CscMatrix A;
+    CsrMatrix R;
+    AdatPattern P;
+    Supernodal S;
+    std::vector<int64_t> slot_dest, diagpos;
+    std::vector<int32_t> colof, csrcol;
+    std::vector<double> csrval;
+    std::vector<int8_t> sbig;
+    // schedules
+    std::vector<Op> fops, sops_fwd, sops_bwd;
+    std::vector<int32_t> list_small, list_fwd, list_bwd;
+    std::vector<EaTask> ea;
+    std::vector<PanelTask> potrf, trsm;
+    std::vector<GemmTask> gemm;
+    std::vector<SolveTask> fgemv, bgemv;
+    std::string err;
+    bool device = false;
+    bool factored = false;
+    bool use_fp32 = false;
+    int32_t last_tiny = 0;
+    // device state
+    hipStream_t st = nullptr;
+    hipEvent_t ev[6] = {};
+    DBuf<int64_t> d_colptr, d_rowptr, d_pairptr, d_dest, d_diagpos, d_loff, d_uoff, d_rowoff, d_reloff;
+    DBuf<int32_t> d_rowind, d_colind, d_pair_p, d_pair_q, d_colof, d_iperm, d_sfirst, d_snrow,
+        d_rowidx, d_relmap, d_childptr, d_child, d_list_small, d_list_fwd, d_list_bwd, d_err;
+    DBuf<double> d_val, d_csrval, d_d, d_dinv, d_b1, d_b2, d_x1, d_x2, d_rp, d_W, d_t, d_tn, d_r,
+        d_c, d_slotvals;
+    DBuf<double> d_L64, d_U64;
+    DBuf<float> d_L32, d_U32;
+    DBuf<int8_t> d_big;
+    DBuf<EaTask> d_ea;
+    DBuf<PanelTask> d_potrf, d_trsm;
+    DBuf<GemmTask> d_gemm;
+    DBuf<SolveTask> d_fgemv, d_bgemv;
+    DevSym sym{};
+
+    ~help_gpu() {
+        if (st) { (void)hipStreamSynchronize(st); }
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+// Position of slot (i,k) (original indices) inside the supernodal L storage.
+void build_assembly_maps(help_gpu* h) {
+    const Supernodal& S = h->S;
+    int64_t nup = (int64_t)h->P.colind.size();
+    h->slot_dest.resize(nup);
+    h->diagpos.resize(h->m);
+    for (int32_t i = 0; i < h->m; ++i)
+        for (int64_t s = h->P.rowptr[i]; s < h->P.rowptr[i + 1]; ++s) {
+            int32_t k = h->P.colind[s];
+            int32_t a = S.iperm[i], b = S.iperm[k];
+            int32_t col = std::min(a, b), row = std::max(a, b);
+            int32_t sn = S.col2sn[col];
+            const int32_t* rb = S.rowidx.data() + S.rowoff[sn];
+            const int32_t* re = S.rowidx.data() + S.rowoff[sn + 1];
+            const int32_t* it = std::lower_bound(rb, re, row);
+            if (it == re || *it != row) throw std::runtime_error("assembly map: row not in structure");
+            int64_t pos = S.loff[sn] + (int64_t)(col - S.sfirst[sn]) * S.snrow[sn] + (it - rb);
+            h->slot_dest[s] = pos;
+            if (k == i) h->diagpos[i] = pos;
+        }
+    h->colof.resize(h->nnz);
+    for (int32_t j = 0; j < h->n; ++j)
+        for (int64_t p = h->A.colptr[j]; p < h->A.colptr[j + 1]; ++p) h->colof[p] = j;
+    h->csrcol = h->R.colind;
+    h->csrval.resize(h->nnz);
+    for (int64_t t = 0; t < h->nnz; ++t) h->csrval[t] = h->A.val[h->R.cscpos[t]];
+}
+
+bool is_small_front(int32_t c, int32_t ncol) {
+    return c <= kSmallRows && (int64_t)c * ncol <= kSmallElems;
+}
+
+bool is_big_solve(int32_t c, int32_t ncol) {
+    return (c - ncol) > 128 && (int64_t)ncol * (c - ncol) > 32768;
+}
+
+void build_schedules(help_gpu* h) {
+    const Supernodal& S = h->S;
+    h->sbig.assign(S.nsuper, 0);
+    for (int32_t l = 0; l < S.nlevels; ++l) {
+        std::vector<int32_t> large;
+        int32_t off = (int32_t)h->list_small.size();
+        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
+            int32_t s = S.levelsn[t];
+            int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
+            if (is_small_front(c, nc)) h->list_small.push_back(s);
+            else large.push_back(s);
+        }
+        if ((int32_t)h->list_small.size() > off)
+            h->fops.push_back({OP_SMALL, off, (int32_t)h->list_small.size() - off});
+        if (large.empty()) continue;
+        // extend-add by 32-column chunks of each front
+        off = (int32_t)h->ea.size();
+        for (int32_t s : large) {
+            int32_t c = S.snrow[s];
+            if (S.childptr[s + 1] == S.childptr[s]) continue;
+            for (int32_t c0 = 0; c0 < c; c0 += 32) h->ea.push_back({s, c0, std::min(c, c0 + 32)});
+        }
+        if ((int32_t)h->ea.size() > off) h->fops.push_back({OP_EA, off, (int32_t)h->ea.size() - off});
+        int32_t maxn = 0;
+        for (int32_t s : large) maxn = std::max(maxn, S.sfirst[s + 1] - S.sfirst[s]);
+        for (int32_t b = 0; b < maxn; b += kTile) {
+            int32_t po = (int32_t)h->potrf.size(), to = (int32_t)h->trsm.size(),
+                    go = (int32_t)h->gemm.size();
+            for (int32_t s : large) {
+                int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
+                if (b >= nc) continue;
+                int32_t nb = std::min(kTile, nc - b);
+                h->potrf.push_back({s, b, 0});
+                for (int32_t r0 = b + nb, rb = 1; r0 < c; r0 += kTile, ++rb) h->trsm.push_back({s, b, rb});
+                for (int32_t c0 = b + nb; c0 < nc; c0 += kTile)
+                    for (int32_t r0 = c0; r0 < c; r0 += kTile)
+                        h->gemm.push_back({s, r0, c0, b, b + nb, nc});
+            }
+            h->fops.push_back({OP_POTRF, po, (int32_t)h->potrf.size() - po});
+            if ((int32_t)h->trsm.size() > to) h->fops.push_back({OP_TRSM, to, (int32_t)h->trsm.size() - to});
+            if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go});
+        }
+        // U -= L21 L21^T with the whole panel as inner dimension
+        int32_t go = (int32_t)h->gemm.size();
+        for (int32_t s : large) {
+            int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
+            for (int32_t c0 = nc; c0 < c; c0 += kTile)
+                for (int32_t r0 = c0; r0 < c; r0 += kTile) h->gemm.push_back({s, r0, c0, 0, nc, c});
+        }
+        if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go});
+    }
+    // triangular solves
+    for (int32_t l = 0; l < S.nlevels; ++l) {
+        int32_t off = (int32_t)h->list_fwd.size(), to = (int32_t)h->fgemv.size();
+        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
+            int32_t s = S.levelsn[t];
+            int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
+            h->list_fwd.push_back(s);
+            if (is_big_solve(c, nc)) {
+                h->sbig[s] = 1;
+                for (int32_t b = 0; nc + b * 64 < c; ++b) h->fgemv.push_back({s, b});
+            }
+        }
+        h->sops_fwd.push_back({OP_FWD, off, (int32_t)h->list_fwd.size() - off});
+        if ((int32_t)h->fgemv.size() > to) h->sops_fwd.push_back({OP_FWD_GEMV, to, (int32_t)h->fgemv.size() - to});
+    }
+    for (int32_t l = S.nlevels - 1; l >= 0; --l) {
+        int32_t off = (int32_t)h->list_bwd.size(), to = (int32_t)h->bgemv.size();
+        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
+            int32_t s = S.levelsn[t];
+            int32_t nc = S.sfirst[s + 1] - S.sfirst[s];
+            h->list_bwd.push_back(s);
+            if (h->sbig[s])
+                for (int32_t b = 0; b * 64 < nc; ++b) h->bgemv.push_back({s, b});
+        }
+        if ((int32_t)h->bgemv.size() > to) h->sops_bwd.push_back({OP_BWD_GEMV, to, (int32_t)h->bgemv.size() - to});
+        h->sops_bwd.push_back({OP_BWD, off, (int32_t)h->list_bwd.size() - off});
+    }
+}
+
+void setup_device(help_gpu* h) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        throw HipError("no HIP device visible", HELP_ERR_NOGPU);
+    HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    for (auto& e : h->ev) HIPCHK(hipEventCreate(&e));
+    hipStream_t st = h->st;
+    const Supernodal& S = h->S;
+    h->d_colptr.upload(h->A.colptr, st);
+    h->d_rowind.upload(h->A.rowind, st);
+    h->d_val.upload(h->A.val, st);
+    h->d_rowptr.upload(h->R.rowptr, st);
+    h->d_colind.upload(h->csrcol, st);
+    h->d_csrval.upload(h->csrval, st);
+    h->d_pairptr.upload(h->P.pairptr, st);
+    h->d_pair_p.upload(h->P.pair_p, st);
+    h->d_pair_q.upload(h->P.pair_q, st);
+    h->d_colof.upload(h->colof, st);
+    h->d_dest.upload(h->slot_dest, st);
+    h->d_diagpos.upload(h->diagpos, st);
+    h->d_iperm.upload(S.iperm, st);
+    h->d_sfirst.upload(S.sfirst, st);
+    h->d_snrow.upload(S.snrow, st);
+    h->d_loff.upload(S.loff, st);
+    h->d_uoff.upload(S.uoff, st);
+    h->d_rowoff.upload(S.rowoff, st);
+    h->d_rowidx.upload(S.rowidx, st);
+    h->d_reloff.upload(S.reloff, st);
+    h->d_relmap.upload(S.relmap, st);
+    h->d_childptr.upload(S.childptr, st);
+    h->d_child.upload(S.child, st);
+    h->d_list_small.upload(h->list_small, st);
+    h->d_list_fwd.upload(h->list_fwd, st);
+    h->d_list_bwd.upload(h->list_bwd, st);
+    h->d_big.upload(h->sbig, st);
+    h->d_ea.upload(h->ea, st);
+    h->d_potrf.upload(h->potrf, st);
+    h->d_trsm.upload(h->trsm, st);
+    h->d_gemm.upload(h->gemm, st);
+    h->d_fgemv.upload(h->fgemv, st);
+    h->d_bgemv.upload(h->bgemv, st);
+    h->d_err.alloc(1);
+    const int32_t m = h->m, n = h->n;
+    h->d_d.alloc(n); h->d_dinv.alloc(n); h->d_b1.alloc(n); h->d_x1.alloc(n); h->d_tn.alloc(n);
+    h->d_b2.alloc(m); h->d_x2.alloc(m); h->d_rp.alloc(m); h->d_t.alloc(m); h->d_r.alloc(m); h->d_c.alloc(m);
+    h->d_W.alloc(std::max<int64_t>(1, S.reloff[S.nsuper]));
+    int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
+    if (h->use_fp32) { h->d_L32.alloc(std::max<int64_t>(1, nl)); h->d_U32.alloc(std::max<int64_t>(1, nu)); }
+    else { h->d_L64.alloc(std::max<int64_t>(1, nl)); h->d_U64.alloc(std::max<int64_t>(1, nu)); }
+    DevSym& y = h->sym;
+    y.nsuper = S.nsuper;
+    y.sfirst = h->d_sfirst.p; y.snrow = h->d_snrow.p; y.loff = h->d_loff.p; y.uoff = h->d_uoff.p;
+    y.rowoff = h->d_rowoff.p; y.rowidx = h->d_rowidx.p; y.reloff = h->d_reloff.p;
+    y.relmap = h->d_relmap.p; y.childptr = h->d_childptr.p; y.child = h->d_child.p;
+    HIPCHK(hipStreamSynchronize(st));
+    h->device = true;
+}
+
+template <typename T>
+void run_factor(help_gpu* h, T* L, T* U) {
+    hipStream_t st = h->st;
+    for (const Op& o : h->fops) {
+        switch (o.kind) {
+            case OP_SMALL: launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, h->d_err.p, st); break;
+            case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->sym, L, U, st); break;
+            case OP_POTRF: launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, h->d_err.p, st); break;
+            case OP_TRSM: launch_trsm_rows<T>(h->d_trsm.p + o.off, o.count, h->sym, L, st); break;
+            case OP_GEMM: launch_gemm<T>(h->d_gemm.p + o.off, o.count, h->sym, L, U, st); break;
+            default: break;
+        }
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// x (permuted numbering, FP64) <- (L L^T)^-1 x
+template <typename T>
+void run_trisolve(help_gpu* h, const T* L, double* x) {
+    hipStream_t st = h->st;
+    for (const Op& o : h->sops_fwd) {
+        if (o.kind == OP_FWD) launch_fwd<T>(h->d_list_fwd.p + o.off, o.count, h->sym, h->d_big.p, L, x, h->d_W.p, st);
+        else launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, x, h->d_W.p, st);
+    }
+    for (const Op& o : h->sops_bwd) {
+        if (o.kind == OP_BWD) launch_bwd<T>(h->d_list_bwd.p + o.off, o.count, h->sym, h->d_big.p, L, x, h->d_t.p, st);
+        else launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, st);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+float elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+int fail(help_gpu* h, const std::exception& e) {
+    h->err = e.what();
+    if (auto* he = dynamic_cast<const HipError*>(&e)) return he->code;
+    return HELP_ERR_ARG;
+}
+
+}  // namespace
+
+extern "C" {
+
+void help_opts_default(help_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->mode = HELP_MODE_DIRECT_FP64;
+    o->beta = 1e-5;
+    o->refine_max = 20;
+    o->refine_tol = 1e-12;
+    o->pcg_k = 500;
+    o->pcg_max_iter = 5000;
+    o->pcg_tol = 1e-10;
+    o->pcg_fp32_precond = 1;
+}
+
+int help_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char* help_last_create_error(void) { return g_create_error.c_str(); }
+
+int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
+                    const int32_t* rowind, const double* val, const help_opts* opts,
+                    help_gpu** out) {
+    g_create_error.clear();
+    if (!out || m <= 0 || n <= 0 || nnz < 0 || !colptr || (nnz > 0 && (!rowind || !val))) {
+        g_create_error = "help_gpu_create: bad arguments";
+        return HELP_ERR_ARG;
+    }
+    *out = nullptr;
+    help_gpu* h = new (std::nothrow) help_gpu();
+    if (!h) return HELP_ERR_NOMEM;
+    try {
+        if (opts) h->opts = *opts; else help_opts_default(&h->opts);
+        if (colptr[0] != 0 || colptr[n] != nnz) throw std::runtime_error("colptr inconsistent with nnz");
+        for (int32_t j = 0; j < n; ++j) {
+            if (colptr[j + 1] < colptr[j]) throw std::runtime_error("colptr not monotone");
+            for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) {
+                if (rowind[p] < 0 || rowind[p] >= m) throw std::runtime_error("row index out of range");
+                if (p > colptr[j] && rowind[p] <= rowind[p - 1])
+                    throw std::runtime_error("row indices not strictly ascending within a column");
+            }
+        }
+        if (nnz > INT32_MAX) throw std::runtime_error("nnz exceeds the int32 index range");
+        h->m = m; h->n = n; h->nnz = nnz;
+        h->use_fp32 = h->opts.mode == HELP_MODE_DIRECT_FP32;
+        if (h->opts.mode == HELP_MODE_PCG || h->opts.mode == HELP_MODE_PCG_LEARNED)
+            throw std::runtime_error("PCG modes are not available in this build");
+        h->A.m = m; h->A.n = n;
+        h->A.colptr.assign(colptr, colptr + n + 1);
+        h->A.rowind.assign(rowind, rowind + nnz);
+        h->A.val.assign(val, val + nnz);
+        h->R = csc_to_csr(h->A);
+        h->P = adat_pattern(h->A, h->R, true);
+        std::vector<int64_t> adjptr;
+        std::vector<int32_t> adj;
+        adat_adjacency(m, h->P, adjptr, adj);
+        RelaxParams rp;
+        h->S = analyze(m, adjptr, adj, {}, rp);
+        build_assembly_maps(h);
+        build_schedules(h);
+        if (!h->opts.symbolic_only) setup_device(h);
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        int code = HELP_ERR_ARG;
+        if (auto* he = dynamic_cast<const HipError*>(&e)) code = he->code;
+        if (auto* ba = dynamic_cast<const std::bad_alloc*>(&e)) { (void)ba; code = HELP_ERR_NOMEM; }
+        delete h;
+        return code;
+    }
+    *out = h;
+    return HELP_OK;
+}
+
+int help_gpu_destroy(help_gpu* h) {
+    delete h;
+    return HELP_OK;
+}
+
+const char* help_gpu_last_error(const help_gpu* h) { return h ? h->err.c_str() : ""; }
+
+int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* rs) {
+    if (!h || !d) return HELP_ERR_ARG;
+    if (!h->device) { h->err = "renew: handle has no device state"; return HELP_ERR_STATE; }
+    try {
+        hipStream_t st = h->st;
+        const int32_t m = h->m, n = h->n;
+        HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(h->ev[0], st));
+        launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
+        HIPCHK(hipMemsetAsync(h->d_err.p, 0, sizeof(int32_t), st));
+        const Supernodal& S = h->S;
+        int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
+        int64_t nslot = (int64_t)h->P.colind.size();
+        if (h->use_fp32) {
+            HIPCHK(hipMemsetAsync(h->d_L32.p, 0, sizeof(float) * nl, st));
+            HIPCHK(hipMemsetAsync(h->d_U32.p, 0, sizeof(float) * std::max<int64_t>(nu, 1), st));
+            launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
+                                 h->d_dinv.p, nslot, h->d_dest.p, nullptr, h->d_L32.p, st);
+            launch_add_beta(h->d_diagpos.p, m, h->opts.beta, nullptr, h->d_L32.p, st);
+        } else {
+            HIPCHK(hipMemsetAsync(h->d_L64.p, 0, sizeof(double) * nl, st));
+            HIPCHK(hipMemsetAsync(h->d_U64.p, 0, sizeof(double) * std::max<int64_t>(nu, 1), st));
+            launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
+                                 h->d_dinv.p, nslot, h->d_dest.p, h->d_L64.p, nullptr, st);
+            launch_add_beta(h->d_diagpos.p, m, h->opts.beta, h->d_L64.p, nullptr, st);
+        }
+        HIPCHK(hipEventRecord(h->ev[1], st));
+        if (h->use_fp32) run_factor<float>(h, h->d_L32.p, h->d_U32.p);
+        else run_factor<double>(h, h->d_L64.p, h->d_U64.p);
+        HIPCHK(hipEventRecord(h->ev[2], st));
+        int32_t nerr = 0;
+        HIPCHK(hipMemcpyAsync(&nerr, h->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        h->last_tiny = nerr;
+        h->factored = true;
+        if (rs) {
+            std::memset(rs, 0, sizeof(*rs));
+            rs->ms_assemble = elapsed_ms(h->ev[0], h->ev[1]);
+            rs->ms_factor = elapsed_ms(h->ev[1], h->ev[2]);
+            rs->n_tiny_pivots = nerr;
+        }
+        return HELP_OK;
+    } catch (const std::exception& e) {
+        return fail(h, e);
+    }
+}
+
+int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double* b2, double* x1,
+                   double* x2, help_solve_stats* ss) {
+    if (!h || !d || !b1 || !b2 || !x1 || !x2) return HELP_ERR_ARG;
+    if (!h->device || !h->factored) { h->err = "solve: call renew first"; return HELP_ERR_STATE; }
+    try {
+        hipStream_t st = h->st;
+        const int32_t m = h->m, n = h->n;
+        help_solve_stats loc{};
+        HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(h->d_b1.p, b1, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(h->d_b2.p, b2, sizeof(double) * m, hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(h->ev[0], st));
+        // r = A (b1 ./ d) - b2, permuted
+        launch_rhs(m, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p, h->d_d.p, h->d_b2.p,
+                   h->d_iperm.p, h->d_rp.p, st);
+        HIPCHK(hipEventRecord(h->ev[1], st));
+        if (!h->use_fp32) {
+            run_trisolve<double>(h, h->d_L64.p, h->d_rp.p);
+            launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, h->d_x2.p, st);
+        } else {
+            // FP32 factor + FP64 iterative refinement on (A D^-1 A^T + beta I)
+            launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, h->d_r.p, st);   // rhs, original order
+            std::vector<double> rhs(m), res(m), x(m);
+            HIPCHK(hipMemcpyAsync(rhs.data(), h->d_r.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+            run_trisolve<float>(h, h->d_L32.p, h->d_rp.p);
+            launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, h->d_x2.p, st);
+            launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
+            HIPCHK(hipStreamSynchronize(st));
+            double rn0 = 0;
+            for (double v : rhs) rn0 += v * v;
+            rn0 = std::sqrt(rn0);
+            int it = 0;
+            double rel = 0;
+            for (;; ++it) {
+                launch_normal_matvec(m, n, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_colptr.p,
+                                     h->d_rowind.p, h->d_val.p, h->d_dinv.p, h->opts.beta, h->d_x2.p,
+                                     h->d_tn.p, h->d_c.p, st);
+                HIPCHK(hipMemcpyAsync(res.data(), h->d_c.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                double rn = 0;
+                for (int32_t i = 0; i < m; ++i) { res[i] = rhs[i] - res[i]; rn += res[i] * res[i]; }
+                rel = rn0 > 0 ? std::sqrt(rn) / rn0 : 0.0;
+                if (rel <= h->opts.refine_tol || it >= h->opts.refine_max) break;
+                HIPCHK(hipMemcpyAsync(h->d_r.p, res.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
+                launch_scatter_perm(m, h->d_r.p, h->d_iperm.p, h->d_rp.p, st);
+                run_trisolve<float>(h, h->d_L32.p, h->d_rp.p);
+                launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, h->d_c.p, st);
+                HIPCHK(hipMemcpyAsync(x.data(), h->d_x2.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(res.data(), h->d_c.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                for (int32_t i = 0; i < m; ++i) x[i] += res[i];
+                HIPCHK(hipMemcpyAsync(h->d_x2.p, x.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
+            }
+            loc.refine_steps = it;
+            loc.rel_residual = rel;
+        }
+        HIPCHK(hipEventRecord(h->ev[2], st));
+        launch_backsub(n, h->d_colptr.p, h->d_rowind.p, h->d_val.p, h->d_b1.p, h->d_d.p, h->d_x2.p,
+                       h->d_x1.p, st);
+        HIPCHK(hipEventRecord(h->ev[3], st));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(x1, h->d_x1.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(x2, h->d_x2.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        loc.ms_rhs = elapsed_ms(h->ev[0], h->ev[1]);
+        loc.ms_solve = elapsed_ms(h->ev[1], h->ev[2]);
+        loc.ms_back = elapsed_ms(h->ev[2], h->ev[3]);
+        if (ss) *ss = loc;
+        return HELP_OK;
+    } catch (const std::exception& e) {
+        return fail(h, e);
+    }
+}
+
+int help_gpu_symbolic_info(const help_gpu* h, help_symbolic_info* info) {
+    if (!h || !info) return HELP_ERR_ARG;
+    const Supernodal& S = h->S;
+    std::memset(info, 0, sizeof(*info));
+    info->nnz_adat_upper = (int64_t)h->P.colind.size();
+    info->nnz_l = S.loff[S.nsuper];
+    info->nnz_l_exact = S.nnz_l_exact;
+    info->n_supernodes = S.nsuper;
+    info->n_levels = S.nlevels;
+    int32_t mx = 0;
+    for (int32_t s = 0; s < S.nsuper; ++s) mx = std::max(mx, S.snrow[s]);
+    info->max_front = mx;
+    info->factor_flops = S.flops;
+    info->n_pairs = (int64_t)h->P.pair_p.size();
+    return HELP_OK;
+}
+
+int help_gpu_symbolic_arrays(const help_gpu* h, int32_t* perm, int32_t* parent, int32_t* colcount) {
+    if (!h) return HELP_ERR_ARG;
+    if (perm) std::memcpy(perm, h->S.perm.data(), sizeof(int32_t) * h->m);
+    if (parent) std::memcpy(parent, h->S.parent.data(), sizeof(int32_t) * h->m);
+    if (colcount) std::memcpy(colcount, h->S.colcount.data(), sizeof(int32_t) * h->m);
+    return HELP_OK;
+}
+
+int help_gpu_adat_pattern(const help_gpu* h, int64_t* rowptr, int32_t* colind) {
+    if (!h) return HELP_ERR_ARG;
+    if (rowptr) std::memcpy(rowptr, h->P.rowptr.data(), sizeof(int64_t) * (h->m + 1));
+    if (colind) std::memcpy(colind, h->P.colind.data(), sizeof(int32_t) * h->P.colind.size());
+    return HELP_OK;
+}
+
+int help_gpu_adat_values(help_gpu* h, const double* d, double* vals) {
+    if (!h || !d || !vals) return HELP_ERR_ARG;
+    if (!h->device) { h->err = "adat_values: no device state"; return HELP_ERR_STATE; }
+    try {
+        hipStream_t st = h->st;
+        int64_t nslot = (int64_t)h->P.colind.size();
+        h->d_slotvals.alloc(std::max<int64_t>(1, nslot));
+        HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * h->n, hipMemcpyHostToDevice, st));
+        launch_dinv(h->d_d.p, h->d_dinv.p, h->n, st);
+        launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
+                             h->d_dinv.p, nslot, nullptr, h->d_slotvals.p, nullptr, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(vals, h->d_slotvals.p, sizeof(double) * nslot, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        h->d_slotvals.free();
+        return HELP_OK;
+    } catch (const std::exception& e) {
+        return fail(h, e);
+    }
+}
+
+int help_gpu_spmv(help_gpu* h, int32_t trans, const double* x, double* y) {
+    if (!h || !x || !y) return HELP_ERR_ARG;
+    if (!h->device) { h->err = "spmv: no device state"; return HELP_ERR_STATE; }
+    try {
+        hipStream_t st = h->st;
+        const int32_t m = h->m, n = h->n;
+        if (!trans) {
+            HIPCHK(hipMemcpyAsync(h->d_b1.p, x, sizeof(double) * n, hipMemcpyHostToDevice, st));
+            launch_spmv_csr(m, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p, h->d_b2.p, st);
+            HIPCHK(hipMemcpyAsync(y, h->d_b2.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        } else {
+            HIPCHK(hipMemcpyAsync(h->d_b2.p, x, sizeof(double) * m, hipMemcpyHostToDevice, st));
+            launch_spmv_csc_t(n, h->d_colptr.p, h->d_rowind.p, h->d_val.p, h->d_b2.p, h->d_b1.p, st);
+            HIPCHK(hipMemcpyAsync(y, h->d_b1.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        return HELP_OK;
+    } catch (const std::exception& e) {
+        return fail(h, e);
+    }
+}
+
+int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows, double* vals, int32_t* nr) {
+    if (!h || !rows || !vals || !nr || j < 0 || j >= h->m) return HELP_ERR_ARG;
+    if (!h->device || !h->factored) { h->err = "factor_column: no factor"; return HELP_ERR_STATE; }
+    try {
+        const Supernodal& S = h->S;
+        int32_t s = S.col2sn[j];
+        int32_t c = S.snrow[s], jj = j - S.sfirst[s];
+        int64_t off = S.loff[s] + (int64_t)jj * c;
+        int32_t cnt = c - jj;
+        if (h->use_fp32) {
+            std::vector<float> tmp(cnt);
+            HIPCHK(hipMemcpy(tmp.data(), h->d_L32.p + off + jj, sizeof(float) * cnt, hipMemcpyDeviceToHost));
+            for (int32_t t = 0; t < cnt; ++t) vals[t] = tmp[t];
+        } else {
+            HIPCHK(hipMemcpy(vals, h->d_L64.p + off + jj, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+        }
+        for (int32_t t = 0; t < cnt; ++t) rows[t] = S.rowidx[S.rowoff[s] + jj + t];
+        *nr = cnt;
+        return HELP_OK;
+    } catch (const std::exception& e) {
+        return fail(h, e);
+    }
+}
+
+// ------------------------------------------------------------------ backend
+struct GpuBackendCtx {
+    help_opts opts;
+    help_gpu* h = nullptr;
+    int32_t last_status = 0;
+};
+
+static int gb_construct(void* ctx, const help_lp* lp) {
+    auto* c = (GpuBackendCtx*)ctx;
+    if (c->h) { help_gpu_destroy(c->h); c->h = nullptr; }
+    return help_gpu_create(lp->m, lp->n, lp->nnz, lp->colptr, lp->rowind, lp->val, &c->opts, &c->h);
+}
+static int gb_destruct(void* ctx) {
+    auto* c = (GpuBackendCtx*)ctx;
+    help_gpu_destroy(c->h);
+    c->h = nullptr;
+    return 0;
+}
+static int gb_renew(void* ctx, const double* d) {
+    auto* c = (GpuBackendCtx*)ctx;
+    return help_gpu_renew(c->h, d, nullptr);
+}
+static int gb_solve(void* ctx, const double* d, const double* b1, const double* b2, double* x1,
+                    double* x2) {
+    auto* c = (GpuBackendCtx*)ctx;
+    return help_gpu_solve(c->h, d, b1, b2, x1, x2, nullptr);
+}
+
+int help_gpu_backend_new(const help_opts* opts, help_backend* out) {
+    if (!out) return HELP_ERR_ARG;
+    auto* c = new (std::nothrow) GpuBackendCtx();
+    if (!c) return HELP_ERR_NOMEM;
+    if (opts) c->opts = *opts; else help_opts_default(&c->opts);
+    c->opts.symbolic_only = 0;
+    out->ctx = c;
+    out->construct = gb_construct;
+    out->destruct = gb_destruct;
+    out->renew = gb_renew;
+    out->solve = gb_solve;
+    return HELP_OK;
+}
+
+void help_gpu_backend_free(help_backend* be) {
+    if (!be || !be->ctx) return;
+    auto* c = (GpuBackendCtx*)be->ctx;
+    help_gpu_destroy(c->h);
+    delete c;
+    be->ctx = nullptr;
+}
+
+help_gpu* help_gpu_backend_handle(const help_backend* be) {
+    if (!be || !be->ctx) return nullptr;
+    return ((GpuBackendCtx*)be->ctx)->h;
+}
+
+}  // extern "C"

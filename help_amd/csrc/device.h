@@ -1,0 +1,122 @@
+// Internal declarations shared by the backend's HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace help {
+
+// Supernodal structure as uploaded to the device (all arrays device-resident).
+struct DevSym {
+    int32_t nsuper;
+    const int32_t* sfirst;    // nsuper+1
+    const int32_t* snrow;     // nsuper
+    const int64_t* loff;      // nsuper+1
+    const int64_t* uoff;      // nsuper+1
+    const int64_t* rowoff;    // nsuper+1
+    const int32_t* rowidx;
+    const int64_t* reloff;    // nsuper+1
+    const int32_t* relmap;
+    const int32_t* childptr;  // nsuper+1
+    const int32_t* child;
+};
+
+// C -= A B^T task on one 64x64 tile of a front.  Front coordinates: rows and
+// columns 0..nrow-1 of supernode s; columns < ncol live in the L panel, the
+// rest in the update matrix U.
+struct GemmTask {
+    int32_t s;
+    int32_t row0, col0;   // tile origin (front coordinates)
+    int32_t k0, k1;       // inner range (panel columns)
+    int32_t col_end;      // columns >= col_end are not updated
+};
+
+// Panel task: factor the diagonal block [b, b+nb) of front s (every task of
+// the step does it redundantly in LDS) and solve the rows of block rb below it.
+struct PanelTask {
+    int32_t s;
+    int32_t b;            // first panel column
+    int32_t rb;           // 0: write back the diagonal block; >=1: row block rb-1
+};
+
+// Extend-add task: add every child's update matrix into the columns
+// [c0, c1) of the front of supernode s.
+struct EaTask {
+    int32_t s;
+    int32_t c0, c1;
+};
+
+constexpr int kTile = 64;      // GEMM tile and panel width
+constexpr int kSmallRows = 256;
+constexpr int kSmallElems = 7680;  // panel elements resident in LDS for a small front (60 KB FP64)
+
+// ---- sparse kernels (kernels_sparse.hip) ----
+void launch_dinv(const double* d, double* dinv, int32_t n, hipStream_t st);
+void launch_adat_assemble(const int64_t* pairptr, const int32_t* pair_p, const int32_t* pair_q,
+                          const int32_t* colof, const double* val, const double* dinv,
+                          int64_t nslots, const int64_t* dest, double* out_d, float* out_f,
+                          hipStream_t st);
+void launch_add_beta(const int64_t* diagpos, int32_t m, double beta, double* Ld, float* Lf,
+                     hipStream_t st);
+void launch_spmv_csr(int32_t m, const int64_t* rowptr, const int32_t* colind, const double* val,
+                     const double* x, double* y, hipStream_t st);
+void launch_spmv_csc_t(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
+                       const double* x, double* y, hipStream_t st);
+void launch_rhs(int32_t m, const int64_t* rowptr, const int32_t* colind, const double* csrval,
+                const double* b1, const double* d, const double* b2, const int32_t* iperm,
+                double* rp, hipStream_t st);
+void launch_backsub(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
+                    const double* b1, const double* d, const double* x2, double* x1,
+                    hipStream_t st);
+void launch_gather_perm(int32_t m, const double* xp, const int32_t* iperm, double* x,
+                        hipStream_t st);
+void launch_scatter_perm(int32_t m, const double* x, const int32_t* iperm, double* xp,
+                         hipStream_t st);
+// y = (A D^-1 A^T + beta I) x in original numbering (refinement operator)
+void launch_normal_matvec(int32_t m, int32_t n, const int64_t* rowptr, const int32_t* colind,
+                          const double* csrval, const int64_t* colptr, const int32_t* rowind,
+                          const double* cscval, const double* dinv, double beta, const double* x,
+                          double* tmp_n, double* y, hipStream_t st);
+
+// ---- supernodal factorization (kernels_factor.hip) ----
+template <typename T>
+void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U,
+                         int32_t* err, hipStream_t st);
+template <typename T>
+void launch_extend_add(const EaTask* tasks, int32_t count, DevSym sym, T* L, T* U,
+                       hipStream_t st);
+template <typename T>
+void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, int32_t* err,
+                       hipStream_t st);
+template <typename T>
+void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, hipStream_t st);
+template <typename T>
+void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st);
+
+// ---- supernodal triangular solves (kernels_solve.hip) ----
+constexpr int kMaxSnCols = 2048;   // supernodes are split to at most this many columns
+
+// Row/column block task of a big supernode in the triangular solves.
+struct SolveTask {
+    int32_t s;
+    int32_t blk;          // 64-row block of L21 (forward) / 64-column block (backward)
+};
+// Forward: for every supernode of the level gather the children's update
+// vectors and solve with L11; small supernodes also apply L21 (big[s] == 0).
+template <typename T>
+void launch_fwd(const int32_t* list, int32_t count, DevSym sym, const int8_t* big, const T* L,
+                double* x, double* W, hipStream_t st);
+// Forward, big supernodes: W_s += L21 x_s by 64-row blocks.
+template <typename T>
+void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
+                     const double* x, double* W, hipStream_t st);
+// Backward, big supernodes: t[f+j] = sum_{i>=n} L21(i,j) x[row_i] by 64-column blocks.
+template <typename T>
+void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
+                     const double* x, double* t, hipStream_t st);
+// Backward: x_s = L11^-T (x_s - L21^T x_rows) (big supernodes read t).
+template <typename T>
+void launch_bwd(const int32_t* list, int32_t count, DevSym sym, const int8_t* big, const T* L,
+                double* x, const double* t, hipStream_t st);
+
+}  // namespace help

@@ -1,0 +1,347 @@
+// Supernodal multifrontal Cholesky on CDNA4.
+//
+// Storage per supernode s (ncol n, nrow c, structure rows sorted):
+//   L panel  c x n, column-major (lda = c), rows = the structure of s;
+//   U        (c-n) x (c-n), column-major lower, the update ("contribution")
+//            matrix passed to the parent by extend-add.
+// The symbolic schedule processes the elimination tree level by level;
+// within a level every front is independent.  A parent pulls its children's
+// update matrices one child after the other, so the summation order is fixed
+// and no atomics touch the factor.
+//
+// Small fronts (c <= 256, c*n <= 8192): one workgroup per front does the
+// extend-add, the dense partial factorization in LDS and the update SYRK.
+// Large fronts: extend-add by column chunks, then for each 64-column panel a
+// diagonal-block Cholesky, a row-block triangular solve and an MFMA GEMM on
+// the trailing panel columns, and finally one MFMA SYRK for U with the full
+// inner dimension n.
+#include "device.h"
+
+namespace help {
+
+namespace {
+
+template <typename T> struct Huge;
+template <> struct Huge<double> { static constexpr double v = 1e128; };
+template <> struct Huge<float> { static constexpr float v = 1e30f; };
+
+// pivot policy: a non-positive (or NaN) pivot is replaced by a huge value, so
+// the column of L becomes ~0 (that row is dropped from the solve), and counted.
+template <typename T>
+__device__ inline T safe_sqrt_pivot(T d, int32_t* err) {
+    if (!(d > T(0))) {
+        atomicAdd(err, 1);
+        d = Huge<T>::v;
+    }
+    return sqrt(d);
+}
+
+// ------------------------------------------------------------------ small
+template <typename T>
+__global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict__ list, DevSym sym,
+                                                      T* __restrict__ L, T* __restrict__ U,
+                                                      int32_t* __restrict__ err) {
+    __shared__ T F[kSmallElems];
+    __shared__ T piv;
+    const int32_t s = list[blockIdx.x];
+    const int32_t f = sym.sfirst[s];
+    const int32_t n = sym.sfirst[s + 1] - f;
+    const int32_t c = sym.snrow[s];
+    const int32_t cu = c - n;
+    T* Lp = L + sym.loff[s];
+    T* Up = U + sym.uoff[s];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int32_t cn = c * n;
+    for (int32_t idx = tid; idx < cn; idx += nt) F[idx] = Lp[idx];
+    __syncthreads();
+    // extend-add the children, one after the other
+    for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
+        const int32_t ch = sym.child[t];
+        const int32_t chn = sym.sfirst[ch + 1] - sym.sfirst[ch];
+        const int32_t chc = sym.snrow[ch] - chn;
+        const int32_t* rel = sym.relmap + sym.reloff[ch];
+        const T* Uc = U + sym.uoff[ch];
+        const int64_t tot = (int64_t)chc * chc;
+        for (int64_t idx = tid; idx < tot; idx += nt) {
+            int32_t jj = (int32_t)(idx / chc), ii = (int32_t)(idx - (int64_t)jj * chc);
+            if (ii < jj) continue;
+            int32_t r = rel[ii], cc = rel[jj];
+            T v = Uc[idx];
+            if (cc < n) F[cc * c + r] += v;
+            else Up[(int64_t)(cc - n) * cu + (r - n)] += v;
+        }
+        __syncthreads();
+    }
+    // dense partial Cholesky of the panel, right-looking by columns
+    for (int32_t j = 0; j < n; ++j) {
+        if (tid == 0) {
+            T d = safe_sqrt_pivot<T>(F[j * c + j], err);
+            F[j * c + j] = d;
+            piv = d;
+        }
+        __syncthreads();
+        const T p = piv;
+        for (int32_t i = j + 1 + tid; i < c; i += nt) F[j * c + i] /= p;
+        __syncthreads();
+        const int32_t rem = n - j - 1;
+        const int32_t tot = rem * c;
+        for (int32_t idx = tid; idx < tot; idx += nt) {
+            int32_t k = j + 1 + idx / c, i = idx % c;
+            if (i >= k) F[k * c + i] -= F[j * c + i] * F[j * c + k];
+        }
+        __syncthreads();
+    }
+    for (int32_t idx = tid; idx < cn; idx += nt) Lp[idx] = F[idx];
+    // U -= L21 L21^T
+    const int32_t tot = cu * cu;
+    for (int32_t idx = tid; idx < tot; idx += nt) {
+        int32_t jj = idx / cu, ii = idx - jj * cu;
+        if (ii < jj) continue;
+        T acc = 0;
+        for (int32_t j = 0; j < n; ++j) acc += F[j * c + n + ii] * F[j * c + n + jj];
+        Up[idx] -= acc;
+    }
+}
+
+// ------------------------------------------------------------ extend-add
+template <typename T>
+__global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks, DevSym sym,
+                                                    T* __restrict__ L, T* __restrict__ U) {
+    const EaTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s;
+    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
+    const int32_t c = sym.snrow[s];
+    const int32_t cu = c - n;
+    T* Lp = L + sym.loff[s];
+    T* Up = U + sym.uoff[s];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
+        const int32_t ch = sym.child[t];
+        const int32_t chn = sym.sfirst[ch + 1] - sym.sfirst[ch];
+        const int32_t chc = sym.snrow[ch] - chn;
+        const int32_t* rel = sym.relmap + sym.reloff[ch];
+        const T* Uc = U + sym.uoff[ch];
+        // rel is ascending: find the child columns that land in [c0, c1)
+        int32_t lo = 0, hi = chc;
+        while (lo < hi) { int32_t mid = (lo + hi) >> 1; if (rel[mid] < tk.c0) lo = mid + 1; else hi = mid; }
+        for (int32_t jj = lo; jj < chc; ++jj) {
+            const int32_t cc = rel[jj];
+            if (cc >= tk.c1) break;
+            const T* col = Uc + (int64_t)jj * chc;
+            if (cc < n) {
+                T* dst = Lp + (int64_t)cc * c;
+                for (int32_t ii = jj + tid; ii < chc; ii += nt) dst[rel[ii]] += col[ii];
+            } else {
+                T* dst = Up + (int64_t)(cc - n) * cu - n;
+                for (int32_t ii = jj + tid; ii < chc; ii += nt) dst[rel[ii]] += col[ii];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------- diagonal block Cholesky
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_diag(const PanelTask* __restrict__ tasks, DevSym sym,
+                                                    T* __restrict__ L, int32_t* __restrict__ err) {
+    __shared__ T D[kTile * (kTile + 1)];
+    __shared__ T piv;
+    const PanelTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s, b = tk.b;
+    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
+    const int32_t c = sym.snrow[s];
+    const int32_t nb = min(kTile, n - b);
+    T* Lp = L + sym.loff[s];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    constexpr int ld = kTile + 1;
+    for (int32_t idx = tid; idx < nb * nb; idx += nt) {
+        int32_t j = idx / nb, i = idx - j * nb;
+        D[j * ld + i] = Lp[(int64_t)(b + j) * c + b + i];
+    }
+    __syncthreads();
+    for (int32_t j = 0; j < nb; ++j) {
+        if (tid == 0) {
+            T d = safe_sqrt_pivot<T>(D[j * ld + j], err);
+            D[j * ld + j] = d;
+            piv = d;
+        }
+        __syncthreads();
+        const T p = piv;
+        for (int32_t i = j + 1 + tid; i < nb; i += nt) D[j * ld + i] /= p;
+        __syncthreads();
+        const int32_t rem = nb - j - 1;
+        for (int32_t idx = tid; idx < rem * nb; idx += nt) {
+            int32_t k = j + 1 + idx / nb, i = idx % nb;
+            if (i >= k) D[k * ld + i] -= D[j * ld + i] * D[j * ld + k];
+        }
+        __syncthreads();
+    }
+    for (int32_t idx = tid; idx < nb * nb; idx += nt) {
+        int32_t j = idx / nb, i = idx - j * nb;
+        if (i >= j) Lp[(int64_t)(b + j) * c + b + i] = D[j * ld + i];
+    }
+}
+
+// ----------------------------------------- row-block triangular solve
+// X := X L_bb^-T for the 64 rows of block rb-1 below the diagonal block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_rows(const PanelTask* __restrict__ tasks, DevSym sym,
+                                                   T* __restrict__ L) {
+    __shared__ T D[kTile * (kTile + 1)];
+    __shared__ T X[kTile * (kTile + 1)];
+    const PanelTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s, b = tk.b;
+    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
+    const int32_t c = sym.snrow[s];
+    const int32_t nb = min(kTile, n - b);
+    const int32_t r0 = b + nb + (tk.rb - 1) * kTile;
+    const int32_t nr = min(kTile, c - r0);
+    T* Lp = L + sym.loff[s];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    constexpr int ld = kTile + 1;
+    for (int32_t idx = tid; idx < nb * nb; idx += nt) {
+        int32_t j = idx / nb, i = idx - j * nb;
+        D[j * ld + i] = i >= j ? Lp[(int64_t)(b + j) * c + b + i] : T(0);
+    }
+    for (int32_t idx = tid; idx < nb * nr; idx += nt) {
+        int32_t j = idx / nr, i = idx - j * nr;
+        X[j * ld + i] = Lp[(int64_t)(b + j) * c + r0 + i];
+    }
+    __syncthreads();
+    if (tid < nr) {
+        const int32_t i = tid;
+        for (int32_t j = 0; j < nb; ++j) {
+            T x = X[j * ld + i];
+            for (int32_t k = 0; k < j; ++k) x -= X[k * ld + i] * D[k * ld + j];
+            X[j * ld + i] = x / D[j * ld + j];
+        }
+    }
+    __syncthreads();
+    for (int32_t idx = tid; idx < nb * nr; idx += nt) {
+        int32_t j = idx / nr, i = idx - j * nr;
+        Lp[(int64_t)(b + j) * c + r0 + i] = X[j * ld + i];
+    }
+}
+
+// ------------------------------------------------------------- MFMA GEMM
+// C[row0:+64, col0:+64] -= P[rows, k0:k1] * P[cols, k0:k1]^T, lower part only,
+// with P the L panel of the front.  4 waves, each a 32x32 quarter built from
+// 2x2 MFMA 16x16x4 blocks.  Lane layout of V_MFMA_*_16X16X4 (one block):
+// A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane
+// j + 16(i/4), element i%4.
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double4_t acc_t;
+    static __device__ inline acc_t fma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef float4_t acc_t;
+    static __device__ inline acc_t fma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int kKc = 16;   // inner-dimension chunk staged in LDS
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, DevSym sym,
+                                              T* __restrict__ L, T* __restrict__ U) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T As[kKc][kTile];
+    __shared__ T Bs[kKc][kTile];
+    const GemmTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s;
+    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
+    const int32_t c = sym.snrow[s];
+    const int32_t cu = c - n;
+    const T* Lp = L + sym.loff[s];
+    T* Lw = L + sym.loff[s];
+    T* Up = U + sym.uoff[s];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    acc_t acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    const int li = tid & 63, lk = tid >> 6;
+    for (int32_t kb = tk.k0; kb < tk.k1; kb += kKc) {
+#pragma unroll
+        for (int r = 0; r < kKc / 4; ++r) {
+            int32_t k = lk + 4 * r;
+            int32_t kg = kb + k;
+            int32_t ia = tk.row0 + li, ib = tk.col0 + li;
+            bool kin = kg < tk.k1;
+            As[k][li] = (kin && ia < c) ? Lp[(int64_t)kg * c + ia] : T(0);
+            Bs[k][li] = (kin && ib < c) ? Lp[(int64_t)kg * c + ib] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 4) {
+            const int kr = kk + (lane >> 4);
+            T a0 = As[kr][wr + (lane & 15)];
+            T a1 = As[kr][wr + 16 + (lane & 15)];
+            T b0 = Bs[kr][wc + (lane & 15)];
+            T b1 = Bs[kr][wc + 16 + (lane & 15)];
+            acc[0][0] = Mfma<T>::fma(a0, b0, acc[0][0]);
+            acc[0][1] = Mfma<T>::fma(a0, b1, acc[0][1]);
+            acc[1][0] = Mfma<T>::fma(a1, b0, acc[1][0]);
+            acc[1][1] = Mfma<T>::fma(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int r = 0; r < 4; ++r) {
+                int32_t i = tk.row0 + wr + a * 16 + 4 * (lane >> 4) + r;
+                int32_t j = tk.col0 + wc + b * 16 + (lane & 15);
+                if (i >= c || j >= tk.col_end || i < j) continue;
+                T v = acc[a][b][r];
+                if (j < n) Lw[(int64_t)j * c + i] -= v;
+                else Up[(int64_t)(j - n) * cu + (i - n)] -= v;
+            }
+}
+
+}  // namespace
+
+template <typename T>
+void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, int32_t* err,
+                         hipStream_t st) {
+    if (count > 0) k_small_fronts<T><<<count, 256, 0, st>>>(list, sym, L, U, err);
+}
+template <typename T>
+void launch_extend_add(const EaTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st) {
+    if (count > 0) k_extend_add<T><<<count, 256, 0, st>>>(tasks, sym, L, U);
+}
+template <typename T>
+void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st) {
+    if (count > 0) k_gemm<T><<<count, 256, 0, st>>>(tasks, sym, L, U);
+}
+
+template <typename T>
+void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, int32_t* err,
+                       hipStream_t st) {
+    if (count > 0) k_potrf_diag<T><<<count, 256, 0, st>>>(tasks, sym, L, err);
+}
+template <typename T>
+void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, hipStream_t st) {
+    if (count > 0) k_trsm_rows<T><<<count, 256, 0, st>>>(tasks, sym, L);
+}
+
+#define HELP_INST(T)                                                                            \
+    template void launch_small_fronts<T>(const int32_t*, int32_t, DevSym, T*, T*, int32_t*,      \
+                                         hipStream_t);                                          \
+    template void launch_extend_add<T>(const EaTask*, int32_t, DevSym, T*, T*, hipStream_t);   \
+    template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, hipStream_t);       \
+    template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, int32_t*,        \
+                                       hipStream_t);                                            \
+    template void launch_trsm_rows<T>(const PanelTask*, int32_t, DevSym, T*, hipStream_t);
+HELP_INST(double)
+HELP_INST(float)
+#undef HELP_INST
+
+}  // namespace help

@@ -1,0 +1,202 @@
+// HBM-bound sparse kernels: ADA^T numeric assembly, SpMV gathers, the
+// right-hand side and back-substitution of the KKT solve.
+//
+// Summation order is fixed and sequential per output, and multiply/add are
+// never contracted into an FMA, so every kernel here is bit-identical to the
+// CPU restatement of the same HELP loop.
+#pragma clang fp contract(off)
+
+#include "device.h"
+
+namespace help {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+__global__ void k_dinv(const double* __restrict__ d, double* __restrict__ dinv, int32_t n) {
+    int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) dinv[j] = 1.0 / d[j];   // UsePardiso.cpp:172-173
+}
+
+// One slot (i,k) of the upper ADA^T per thread: sum over the columns j it
+// shares with, in ascending j, of (A_ij * dinv_j) * A_kj starting from zero
+// (Helper.cpp:358-373).  The result goes to dest[s] (a position in the
+// supernodal factor storage) or to slot s when dest is null.
+__global__ void k_adat(const int64_t* __restrict__ pairptr, const int32_t* __restrict__ pp,
+                       const int32_t* __restrict__ pq, const int32_t* __restrict__ colof,
+                       const double* __restrict__ val, const double* __restrict__ dinv,
+                       int64_t nslots, const int64_t* __restrict__ dest, double* __restrict__ outd,
+                       float* __restrict__ outf) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    double acc = 0.0;
+    int64_t t1 = pairptr[s + 1];
+    for (int64_t t = pairptr[s]; t < t1; ++t) {
+        int32_t p = pp[t], q = pq[t];
+        double vpdj = val[p] * dinv[colof[p]];
+        acc = acc + vpdj * val[q];
+    }
+    int64_t o = dest ? dest[s] : s;
+    if (outd) outd[o] = acc;
+    else outf[o] = (float)acc;
+}
+
+__global__ void k_add_beta(const int64_t* __restrict__ diagpos, int32_t m, double beta,
+                           double* __restrict__ Ld, float* __restrict__ Lf) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int64_t o = diagpos[i];
+    if (Ld) Ld[o] = Ld[o] + beta;
+    else Lf[o] = (float)((double)Lf[o] + beta);
+}
+
+__global__ void k_spmv_csr(int32_t m, const int64_t* __restrict__ rowptr,
+                           const int32_t* __restrict__ colind, const double* __restrict__ val,
+                           const double* __restrict__ x, double* __restrict__ y) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double acc = 0.0;
+    for (int64_t t = rowptr[i]; t < rowptr[i + 1]; ++t) acc = acc + val[t] * x[colind[t]];
+    y[i] = acc;
+}
+
+__global__ void k_spmv_csc_t(int32_t n, const int64_t* __restrict__ colptr,
+                             const int32_t* __restrict__ rowind, const double* __restrict__ val,
+                             const double* __restrict__ x, double* __restrict__ y) {
+    int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double acc = 0.0;
+    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) acc = acc + val[p] * x[rowind[p]];
+    y[j] = acc;
+}
+
+// r_i = -b2_i + sum_j A_ij (b1_j / d_j), columns ascending (cholmod_sdmult
+// with alpha = beta = 1 on Y = -b2, UseCholmod.cpp:173-179), written to the
+// permuted position iperm[i] (UseCholmod.cpp:180-182).
+__global__ void k_rhs(int32_t m, const int64_t* __restrict__ rowptr,
+                      const int32_t* __restrict__ colind, const double* __restrict__ csrval,
+                      const double* __restrict__ b1, const double* __restrict__ d,
+                      const double* __restrict__ b2, const int32_t* __restrict__ iperm,
+                      double* __restrict__ rp) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double acc = -b2[i];
+    for (int64_t t = rowptr[i]; t < rowptr[i + 1]; ++t) {
+        int32_t j = colind[t];
+        double tj = b1[j] / d[j];
+        acc = acc + csrval[t] * tj;
+    }
+    rp[iperm[i]] = acc;
+}
+
+// x1_j = (b1_j - sum_i A_ij x2_i) / d_j, subtracting each term from b1_j over
+// ascending rows (Helper.cpp:145-154 order; UseCholmod.cpp:226-231).
+__global__ void k_backsub(int32_t n, const int64_t* __restrict__ colptr,
+                          const int32_t* __restrict__ rowind, const double* __restrict__ val,
+                          const double* __restrict__ b1, const double* __restrict__ d,
+                          const double* __restrict__ x2, double* __restrict__ x1) {
+    int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double acc = b1[j];
+    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) acc = acc - val[p] * x2[rowind[p]];
+    x1[j] = acc / d[j];
+}
+
+__global__ void k_gather_perm(int32_t m, const double* __restrict__ xp,
+                              const int32_t* __restrict__ iperm, double* __restrict__ x) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) x[i] = xp[iperm[i]];
+}
+
+__global__ void k_scatter_perm(int32_t m, const double* __restrict__ x,
+                               const int32_t* __restrict__ iperm, double* __restrict__ xp) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) xp[iperm[i]] = x[i];
+}
+
+__global__ void k_csc_t_scaled(int32_t n, const int64_t* __restrict__ colptr,
+                               const int32_t* __restrict__ rowind, const double* __restrict__ val,
+                               const double* __restrict__ dinv, const double* __restrict__ x,
+                               double* __restrict__ y) {
+    int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double acc = 0.0;
+    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) acc = acc + val[p] * x[rowind[p]];
+    y[j] = acc * dinv[j];
+}
+
+__global__ void k_csr_plus_shift(int32_t m, const int64_t* __restrict__ rowptr,
+                                 const int32_t* __restrict__ colind, const double* __restrict__ val,
+                                 const double* __restrict__ t, double beta,
+                                 const double* __restrict__ x, double* __restrict__ y) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double acc = 0.0;
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) acc = acc + val[k] * t[colind[k]];
+    y[i] = acc + beta * x[i];
+}
+
+}  // namespace
+
+void launch_dinv(const double* d, double* dinv, int32_t n, hipStream_t st) {
+    if (n > 0) k_dinv<<<grid_for(n), kBlock, 0, st>>>(d, dinv, n);
+}
+
+void launch_adat_assemble(const int64_t* pairptr, const int32_t* pair_p, const int32_t* pair_q,
+                          const int32_t* colof, const double* val, const double* dinv,
+                          int64_t nslots, const int64_t* dest, double* out_d, float* out_f,
+                          hipStream_t st) {
+    if (nslots > 0)
+        k_adat<<<grid_for(nslots), kBlock, 0, st>>>(pairptr, pair_p, pair_q, colof, val, dinv,
+                                                    nslots, dest, out_d, out_f);
+}
+
+void launch_add_beta(const int64_t* diagpos, int32_t m, double beta, double* Ld, float* Lf,
+                     hipStream_t st) {
+    if (m > 0) k_add_beta<<<grid_for(m), kBlock, 0, st>>>(diagpos, m, beta, Ld, Lf);
+}
+
+void launch_spmv_csr(int32_t m, const int64_t* rowptr, const int32_t* colind, const double* val,
+                     const double* x, double* y, hipStream_t st) {
+    if (m > 0) k_spmv_csr<<<grid_for(m), kBlock, 0, st>>>(m, rowptr, colind, val, x, y);
+}
+
+void launch_spmv_csc_t(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
+                       const double* x, double* y, hipStream_t st) {
+    if (n > 0) k_spmv_csc_t<<<grid_for(n), kBlock, 0, st>>>(n, colptr, rowind, val, x, y);
+}
+
+void launch_rhs(int32_t m, const int64_t* rowptr, const int32_t* colind, const double* csrval,
+                const double* b1, const double* d, const double* b2, const int32_t* iperm,
+                double* rp, hipStream_t st) {
+    if (m > 0) k_rhs<<<grid_for(m), kBlock, 0, st>>>(m, rowptr, colind, csrval, b1, d, b2, iperm, rp);
+}
+
+void launch_backsub(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
+                    const double* b1, const double* d, const double* x2, double* x1,
+                    hipStream_t st) {
+    if (n > 0) k_backsub<<<grid_for(n), kBlock, 0, st>>>(n, colptr, rowind, val, b1, d, x2, x1);
+}
+
+void launch_gather_perm(int32_t m, const double* xp, const int32_t* iperm, double* x,
+                        hipStream_t st) {
+    if (m > 0) k_gather_perm<<<grid_for(m), kBlock, 0, st>>>(m, xp, iperm, x);
+}
+
+void launch_scatter_perm(int32_t m, const double* x, const int32_t* iperm, double* xp,
+                         hipStream_t st) {
+    if (m > 0) k_scatter_perm<<<grid_for(m), kBlock, 0, st>>>(m, x, iperm, xp);
+}
+
+void launch_normal_matvec(int32_t m, int32_t n, const int64_t* rowptr, const int32_t* colind,
+                          const double* csrval, const int64_t* colptr, const int32_t* rowind,
+                          const double* cscval, const double* dinv, double beta, const double* x,
+                          double* tmp_n, double* y, hipStream_t st) {
+    if (n > 0) k_csc_t_scaled<<<grid_for(n), kBlock, 0, st>>>(n, colptr, rowind, cscval, dinv, x, tmp_n);
+    if (m > 0) k_csr_plus_shift<<<grid_for(m), kBlock, 0, st>>>(m, rowptr, colind, csrval, tmp_n, beta, x, y);
+}
+
+}  // namespace help

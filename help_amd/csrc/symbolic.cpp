@@ -1,0 +1,534 @@
+// Host symbolic analysis: ADA^T pattern, AMD ordering, etree, postorder,
+// column counts, relaxed supernodes, relative maps, level schedule.
+#include "symbolic.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <stdexcept>
+
+namespace help {
+
+CsrMatrix csc_to_csr(const CscMatrix& A) {
+    CsrMatrix R;
+    R.m = A.m; R.n = A.n;
+    int64_t nnz = A.colptr[A.n];
+    R.rowptr.assign(A.m + 1, 0);
+    for (int64_t p = 0; p < nnz; ++p) R.rowptr[A.rowind[p] + 1]++;
+    for (int32_t i = 0; i < A.m; ++i) R.rowptr[i + 1] += R.rowptr[i];
+    R.colind.resize(nnz);
+    R.cscpos.resize(nnz);
+    std::vector<int64_t> fill(R.rowptr.begin(), R.rowptr.end() - 1);
+    for (int32_t j = 0; j < A.n; ++j)          // ascending j -> columns sorted per row
+        for (int64_t p = A.colptr[j]; p < A.colptr[j + 1]; ++p) {
+            int64_t t = fill[A.rowind[p]]++;
+            R.colind[t] = j;
+            R.cscpos[t] = p;
+        }
+    return R;
+}
+
+// HELP Helper.cpp:273-329 (ADAt_Allocate, linker form) gives the slot order;
+// Helper.cpp:350-377 (ADAt_Calc) gives the order of the contributions.
+AdatPattern adat_pattern(const CscMatrix& A, const CsrMatrix& R, bool with_pairs) {
+    AdatPattern P;
+    const int32_t m = A.m;
+    P.rowptr.assign(m + 1, 0);
+    std::vector<int32_t> flag(m, -1), slot(m, 0);
+    // pass 1: slots per row (and pair counts)
+    std::vector<int64_t> npair;
+    for (int32_t i = 0; i < m; ++i) {
+        int64_t cnt = 0;
+        for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
+            int32_t j = R.colind[t];
+            for (int64_t q = R.cscpos[t]; q < A.colptr[j + 1]; ++q) {
+                int32_t k = A.rowind[q];
+                if (flag[k] != i) { flag[k] = i; ++cnt; }
+            }
+        }
+        P.rowptr[i + 1] = P.rowptr[i] + cnt;
+    }
+    int64_t nup = P.rowptr[m];
+    P.colind.resize(nup);
+    if (with_pairs) npair.assign(nup, 0);
+    std::fill(flag.begin(), flag.end(), -1);
+    for (int32_t i = 0; i < m; ++i) {
+        int64_t s = P.rowptr[i];
+        for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
+            int32_t j = R.colind[t];
+            for (int64_t q = R.cscpos[t]; q < A.colptr[j + 1]; ++q) {
+                int32_t k = A.rowind[q];
+                if (flag[k] != i) { flag[k] = i; slot[k] = (int32_t)(s - P.rowptr[i]); P.colind[s++] = k; }
+                if (with_pairs) npair[P.rowptr[i] + slot[k]]++;
+            }
+        }
+    }
+    if (!with_pairs) return P;
+    P.pairptr.assign(nup + 1, 0);
+    for (int64_t s = 0; s < nup; ++s) P.pairptr[s + 1] = P.pairptr[s] + npair[s];
+    int64_t np = P.pairptr[nup];
+    if (np > INT32_MAX) throw std::runtime_error("too many ADA^T contribution pairs");
+    P.pair_p.resize(np);
+    P.pair_q.resize(np);
+    std::vector<int64_t> fillp(P.pairptr.begin(), P.pairptr.end() - 1);
+    std::fill(flag.begin(), flag.end(), -1);
+    for (int32_t i = 0; i < m; ++i) {
+        int64_t base = P.rowptr[i];
+        for (int64_t s = base; s < P.rowptr[i + 1]; ++s) slot[P.colind[s]] = (int32_t)(s - base);
+        for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
+            int32_t j = R.colind[t];
+            int64_t p = R.cscpos[t];
+            for (int64_t q = p; q < A.colptr[j + 1]; ++q) {
+                int64_t f = fillp[base + slot[A.rowind[q]]]++;
+                P.pair_p[f] = (int32_t)p;
+                P.pair_q[f] = (int32_t)q;
+            }
+        }
+    }
+    return P;
+}
+
+void adat_adjacency(int32_t m, const AdatPattern& P, std::vector<int64_t>& adjptr,
+                    std::vector<int32_t>& adj) {
+    adjptr.assign(m + 1, 0);
+    for (int32_t i = 0; i < m; ++i)
+        for (int64_t s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
+            int32_t k = P.colind[s];
+            if (k == i) continue;
+            adjptr[i + 1]++;
+            adjptr[k + 1]++;
+        }
+    for (int32_t i = 0; i < m; ++i) adjptr[i + 1] += adjptr[i];
+    adj.resize(adjptr[m]);
+    std::vector<int64_t> fill(adjptr.begin(), adjptr.end() - 1);
+    for (int32_t i = 0; i < m; ++i)
+        for (int64_t s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
+            int32_t k = P.colind[s];
+            if (k == i) continue;
+            adj[fill[i]++] = k;
+            adj[fill[k]++] = i;
+        }
+    for (int32_t i = 0; i < m; ++i) std::sort(adj.begin() + adjptr[i], adj.begin() + adjptr[i + 1]);
+}
+
+// ---------------------------------------------------------------------------
+// Approximate minimum degree on the quotient graph (Amestoy, Davis & Duff,
+// SIAM J. Matrix Anal. Appl. 17(4), 1996): element absorption, aggressive
+// absorption, approximate external degrees, indistinguishable-variable
+// (supervariable) detection by hashing, mass elimination and postponement
+// of dense rows (degree > max(16, alpha*sqrt(n))).
+// ---------------------------------------------------------------------------
+std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
+                               const std::vector<int32_t>& adj, double dense_alpha) {
+    enum : int8_t { VAR = 0, ELEM = 1, DEAD = 2 };
+    std::vector<int32_t> order;
+    order.reserve(n);
+    if (n == 0) return order;
+    std::vector<int32_t> nv(n, 1), sv_next(n, -1), sv_tail(n);
+    std::vector<int8_t> status(n, VAR);
+    std::vector<int64_t> deg(n, 0), edeg(n, 0), extdeg(n, 0), w(n, 0);
+    std::vector<uint64_t> hsh(n, 0);
+    std::vector<std::vector<int32_t>> E(n), Av(n), Le(n);
+    std::iota(sv_tail.begin(), sv_tail.end(), 0);
+
+    int64_t dense = std::max<int64_t>(16, (int64_t)(dense_alpha * std::sqrt((double)n)));
+    std::vector<int32_t> dense_nodes;
+    for (int32_t i = 0; i < n; ++i)
+        if (adjptr[i + 1] - adjptr[i] > dense) { status[i] = DEAD; dense_nodes.push_back(i); }
+    int64_t nlive = n - (int64_t)dense_nodes.size();
+
+    std::vector<int32_t> head(n + 1, -1), nxt(n, -1), prv(n, -1);
+    auto bucket_insert = [&](int32_t i, int64_t d) {
+        nxt[i] = head[d]; prv[i] = -1;
+        if (head[d] >= 0) prv[head[d]] = i;
+        head[d] = i;
+    };
+    auto bucket_remove = [&](int32_t i) {
+        int64_t d = deg[i];
+        if (prv[i] >= 0) nxt[prv[i]] = nxt[i]; else head[d] = nxt[i];
+        if (nxt[i] >= 0) prv[nxt[i]] = prv[i];
+        nxt[i] = prv[i] = -1;
+    };
+    for (int32_t i = 0; i < n; ++i) {
+        if (status[i] != VAR) continue;
+        for (int64_t t = adjptr[i]; t < adjptr[i + 1]; ++t)
+            if (status[adj[t]] == VAR) Av[i].push_back(adj[t]);
+        deg[i] = (int64_t)Av[i].size();
+    }
+    for (int32_t i = n - 1; i >= 0; --i)
+        if (status[i] == VAR) bucket_insert(i, deg[i]);
+
+    std::vector<int32_t> mark(n, 0);
+    int32_t stamp = 0;
+    int64_t wflg = 1;
+    int64_t nel = 0, mindeg = 0;
+    auto emit = [&](int32_t v) {   // order a whole supervariable
+        for (int32_t j = v; j >= 0; j = sv_next[j]) order.push_back(j);
+    };
+    auto next_stamp = [&]() {
+        if (stamp == INT32_MAX) { std::fill(mark.begin(), mark.end(), 0); stamp = 0; }
+        return ++stamp;
+    };
+    std::vector<int32_t> Lp_live;
+    std::vector<std::pair<uint64_t, int32_t>> hl;
+
+    while (nel < nlive) {
+        while (head[mindeg] < 0) ++mindeg;
+        int32_t p = head[mindeg];
+        bucket_remove(p);
+        // --- new element p: L_p = (A_p U L_e for e in E_p) \ {p} ---
+        int32_t st = next_stamp();
+        mark[p] = st;
+        std::vector<int32_t>& Lp = Le[p];
+        Lp.clear();
+        int64_t degme = 0;
+        for (int32_t j : Av[p])
+            if (status[j] == VAR && nv[j] > 0 && mark[j] != st) { mark[j] = st; Lp.push_back(j); degme += nv[j]; }
+        for (int32_t e : E[p]) {
+            if (status[e] != ELEM) continue;
+            for (int32_t j : Le[e])
+                if (status[j] == VAR && nv[j] > 0 && mark[j] != st) { mark[j] = st; Lp.push_back(j); degme += nv[j]; }
+            status[e] = DEAD;
+            std::vector<int32_t>().swap(Le[e]);
+        }
+        std::vector<int32_t>().swap(Av[p]);
+        std::vector<int32_t>().swap(E[p]);
+        status[p] = ELEM;
+        nel += nv[p];
+        emit(p);
+        for (int32_t i : Lp) bucket_remove(i);
+        // --- |L_e \ L_p| for every element adjacent to L_p ---
+        wflg += (int64_t)n + 2;
+        for (int32_t i : Lp)
+            for (int32_t e : E[i]) {
+                if (status[e] != ELEM) continue;
+                if (w[e] < wflg) w[e] = wflg + edeg[e];
+                w[e] -= nv[i];
+            }
+        // --- prune lists, external degrees, mass elimination, hashes ---
+        Lp_live.clear();
+        for (int32_t i : Lp) {
+            int64_t ext = 0;
+            uint64_t h = 0;
+            size_t k = 0;
+            for (int32_t e : E[i]) {
+                if (status[e] != ELEM) continue;
+                int64_t we = w[e] - wflg;
+                if (we <= 0) {   // aggressive absorption: L_e is inside L_p
+                    status[e] = DEAD;
+                    std::vector<int32_t>().swap(Le[e]);
+                    continue;
+                }
+                ext += we;
+                h += (uint64_t)e;
+                E[i][k++] = e;
+            }
+            E[i].resize(k);
+            k = 0;
+            for (int32_t j : Av[i]) {
+                if (status[j] != VAR || nv[j] == 0 || mark[j] == st) continue;
+                ext += nv[j];
+                h += (uint64_t)j;
+                Av[i][k++] = j;
+            }
+            Av[i].resize(k);
+            if (E[i].empty() && Av[i].empty()) {   // mass elimination with p
+                status[i] = DEAD;
+                nel += nv[i];
+                degme -= nv[i];
+                emit(i);
+                std::vector<int32_t>().swap(E[i]);
+                std::vector<int32_t>().swap(Av[i]);
+                continue;
+            }
+            E[i].push_back(p);
+            h += (uint64_t)p;
+            extdeg[i] = ext;
+            hsh[i] = h;
+            Lp_live.push_back(i);
+        }
+        // --- supervariable detection among L_p ---
+        hl.clear();
+        for (int32_t i : Lp_live) hl.emplace_back(hsh[i], i);
+        std::sort(hl.begin(), hl.end());
+        for (size_t a = 0; a < hl.size();) {
+            size_t b = a;
+            while (b < hl.size() && hl[b].first == hl[a].first) ++b;
+            for (size_t x = a; x < b; ++x) {
+                int32_t i = hl[x].second;
+                if (nv[i] == 0) continue;
+                bool marked = false;
+                int32_t st2 = 0;
+                for (size_t y = x + 1; y < b; ++y) {
+                    int32_t j = hl[y].second;
+                    if (nv[j] == 0) continue;
+                    if (E[i].size() != E[j].size() || Av[i].size() != Av[j].size()) continue;
+                    if (!marked) {
+                        st2 = next_stamp();
+                        for (int32_t e : E[i]) mark[e] = st2;
+                        for (int32_t v : Av[i]) mark[v] = st2;
+                        marked = true;
+                    }
+                    bool same = true;
+                    for (int32_t e : E[j]) if (mark[e] != st2) { same = false; break; }
+                    if (same) for (int32_t v : Av[j]) if (mark[v] != st2) { same = false; break; }
+                    if (!same) continue;
+                    nv[i] += nv[j];
+                    nv[j] = 0;
+                    status[j] = DEAD;
+                    sv_next[sv_tail[i]] = j;
+                    sv_tail[i] = sv_tail[j];
+                    std::vector<int32_t>().swap(E[j]);
+                    std::vector<int32_t>().swap(Av[j]);
+                }
+            }
+            a = b;
+        }
+        // --- finalize degrees and L_p ---
+        size_t k = 0;
+        for (int32_t i : Lp_live) {
+            if (nv[i] == 0) continue;
+            int64_t d = std::min(deg[i] + degme - nv[i], extdeg[i] + degme - nv[i]);
+            d = std::min(d, nlive - nel - nv[i]);
+            if (d < 0) d = 0;
+            deg[i] = d;
+            bucket_insert(i, d);
+            if (d < mindeg) mindeg = d;
+            Lp[k++] = i;
+        }
+        Lp.resize(k);
+        edeg[p] = degme;
+    }
+    for (int32_t i : dense_nodes) order.push_back(i);
+    if ((int32_t)order.size() != n) throw std::runtime_error("amd_order: incomplete ordering");
+    return order;
+}
+
+std::vector<int32_t> etree(int32_t m, const std::vector<int64_t>& adjptr,
+                           const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
+                           const std::vector<int32_t>& iperm) {
+    std::vector<int32_t> parent(m, -1), anc(m, -1);
+    for (int32_t k = 0; k < m; ++k) {
+        int32_t old = perm[k];
+        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
+            int32_t i = iperm[adj[t]];
+            while (i != -1 && i < k) {
+                int32_t inext = anc[i];
+                anc[i] = k;
+                if (inext == -1) parent[i] = k;
+                i = inext;
+            }
+        }
+    }
+    return parent;
+}
+
+std::vector<int32_t> postorder(const std::vector<int32_t>& parent) {
+    int32_t m = (int32_t)parent.size();
+    std::vector<int32_t> head(m, -1), next(m, -1), post;
+    post.reserve(m);
+    for (int32_t j = m - 1; j >= 0; --j)
+        if (parent[j] >= 0) { next[j] = head[parent[j]]; head[parent[j]] = j; }
+    std::vector<int32_t> stack;
+    for (int32_t r = 0; r < m; ++r) {
+        if (parent[r] != -1) continue;
+        stack.push_back(r);
+        while (!stack.empty()) {
+            int32_t v = stack.back();
+            int32_t c = head[v];
+            if (c == -1) { stack.pop_back(); post.push_back(v); }
+            else { head[v] = next[c]; stack.push_back(c); }
+        }
+    }
+    return post;
+}
+
+std::vector<int32_t> colcounts(int32_t m, const std::vector<int64_t>& adjptr,
+                               const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
+                               const std::vector<int32_t>& iperm, const std::vector<int32_t>& parent) {
+    std::vector<int32_t> cc(m, 1), mark(m, -1);
+    for (int32_t i = 0; i < m; ++i) {   // row subtree of row i of L
+        mark[i] = i;
+        int32_t old = perm[i];
+        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
+            int32_t k = iperm[adj[t]];
+            if (k >= i) continue;
+            while (mark[k] != i) { cc[k]++; mark[k] = i; k = parent[k]; }
+        }
+    }
+    return cc;
+}
+
+Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vector<int32_t>& adj,
+                   const std::vector<int32_t>& perm_in, const RelaxParams& rp) {
+    Supernodal S;
+    S.m = m;
+    std::vector<int32_t> perm = perm_in.empty() ? amd_order(m, adjptr, adj) : perm_in;
+    std::vector<int32_t> iperm(m);
+    for (int32_t k = 0; k < m; ++k) iperm[perm[k]] = k;
+    // etree, then postorder it (does not change fill) and recompute
+    std::vector<int32_t> par = etree(m, adjptr, adj, perm, iperm);
+    std::vector<int32_t> post = postorder(par);
+    std::vector<int32_t> perm2(m);
+    for (int32_t k = 0; k < m; ++k) perm2[k] = perm[post[k]];
+    perm.swap(perm2);
+    for (int32_t k = 0; k < m; ++k) iperm[perm[k]] = k;
+    par = etree(m, adjptr, adj, perm, iperm);
+    std::vector<int32_t> cc = colcounts(m, adjptr, adj, perm, iperm, par);
+    S.perm = perm; S.iperm = iperm; S.parent = par; S.colcount = cc;
+    for (int32_t j = 0; j < m; ++j) {
+        S.nnz_l_exact += cc[j];
+        S.flops += (double)cc[j] * (double)cc[j];
+    }
+    // fundamental supernodes
+    std::vector<int32_t> nchild(m, 0);
+    for (int32_t j = 0; j < m; ++j) if (par[j] >= 0) nchild[par[j]]++;
+    std::vector<int32_t> fs{0};
+    for (int32_t j = 1; j < m; ++j)
+        if (!(par[j - 1] == j && cc[j - 1] == cc[j] + 1 && nchild[j] == 1)) fs.push_back(j);
+    fs.push_back(m);
+    int32_t nf = (int32_t)fs.size() - 1;
+    // relaxed amalgamation, bottom-up: a group may absorb into its parent
+    // group when the parent starts right after it (it is the last child).
+    std::vector<int32_t> fcol2sn(m);
+    for (int32_t s = 0; s < nf; ++s)
+        for (int32_t j = fs[s]; j < fs[s + 1]; ++j) fcol2sn[j] = s;
+    std::vector<int32_t> gfirst(nf), gncol(nf), gnrow(nf);
+    std::vector<int64_t> gactual(nf);
+    std::vector<int32_t> merged_into(nf, -1);
+    for (int32_t s = 0; s < nf; ++s) {
+        gfirst[s] = fs[s];
+        gncol[s] = fs[s + 1] - fs[s];
+        gnrow[s] = cc[fs[s]];
+        int64_t a = 0;
+        for (int32_t j = fs[s]; j < fs[s + 1]; ++j) a += cc[j];
+        gactual[s] = a;
+    }
+    for (int32_t s = 0; s < nf; ++s) {
+        int32_t last = fs[s + 1] - 1;
+        int32_t pc = par[last];
+        if (pc < 0) continue;
+        int32_t p = fcol2sn[pc];
+        if (p != s + 1) continue;                 // not contiguous: cannot merge
+        int64_t nc = (int64_t)gncol[s] + gncol[p];
+        if (nc > rp.max_cols) continue;
+        int64_t nr = (int64_t)gncol[s] + gnrow[p];
+        int64_t stored = nc * nr - nc * (nc - 1) / 2;
+        int64_t zeros = stored - (gactual[s] + gactual[p]);
+        double frac = stored > 0 ? (double)zeros / (double)stored : 0.0;
+        bool ok = nc <= rp.nrelax[0] ||
+                  (nc <= rp.nrelax[1] && frac < rp.zrelax[0]) ||
+                  (nc <= rp.nrelax[2] && frac < rp.zrelax[1]) ||
+                  frac < rp.zrelax[2];
+        if (!ok) continue;
+        gfirst[p] = gfirst[s];
+        gncol[p] = (int32_t)nc;
+        gnrow[p] = (int32_t)nr;
+        gactual[p] += gactual[s];
+        merged_into[s] = p;
+    }
+    S.sfirst.clear();
+    for (int32_t s = 0; s < nf; ++s) {
+        if (merged_into[s] >= 0) continue;
+        // a supernode split into consecutive pieces stays a valid partition:
+        // each piece's parent is the next piece
+        int32_t f = gfirst[s], l = fs[s + 1];
+        for (int32_t k = f; k < l; k += rp.max_sn_cols) S.sfirst.push_back(k);
+    }
+    S.sfirst.push_back(m);
+    S.nsuper = (int32_t)S.sfirst.size() - 1;
+    const int32_t ns = S.nsuper;
+    S.col2sn.resize(m);
+    for (int32_t s = 0; s < ns; ++s)
+        for (int32_t j = S.sfirst[s]; j < S.sfirst[s + 1]; ++j) S.col2sn[j] = s;
+    S.sparent.assign(ns, -1);
+    for (int32_t s = 0; s < ns; ++s) {
+        int32_t pc = par[S.sfirst[s + 1] - 1];
+        S.sparent[s] = pc < 0 ? -1 : S.col2sn[pc];
+    }
+    S.childptr.assign(ns + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) if (S.sparent[s] >= 0) S.childptr[S.sparent[s] + 1]++;
+    for (int32_t s = 0; s < ns; ++s) S.childptr[s + 1] += S.childptr[s];
+    S.child.resize(S.childptr[ns]);
+    {
+        std::vector<int32_t> f(S.childptr.begin(), S.childptr.end() - 1);
+        for (int32_t s = 0; s < ns; ++s) if (S.sparent[s] >= 0) S.child[f[S.sparent[s]]++] = s;
+    }
+    // row structures: own columns U lower entries of M U children's update rows
+    S.rowoff.assign(ns + 1, 0);
+    S.snrow.assign(ns, 0);
+    std::vector<int32_t> mk(m, -1), rows;
+    std::vector<std::vector<int32_t>> tmp(ns);
+    for (int32_t s = 0; s < ns; ++s) {
+        int32_t f = S.sfirst[s], l = S.sfirst[s + 1];
+        rows.clear();
+        for (int32_t j = f; j < l; ++j) { mk[j] = s; rows.push_back(j); }
+        for (int32_t j = f; j < l; ++j) {
+            int32_t old = perm[j];
+            for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
+                int32_t i = iperm[adj[t]];
+                if (i >= l && mk[i] != s) { mk[i] = s; rows.push_back(i); }
+            }
+        }
+        for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
+            int32_t c = S.child[t];
+            int32_t cn = S.sfirst[c + 1] - S.sfirst[c];
+            const std::vector<int32_t>& cr = tmp[c];
+            for (size_t u = cn; u < cr.size(); ++u)
+                if (mk[cr[u]] != s) { mk[cr[u]] = s; rows.push_back(cr[u]); }
+        }
+        std::sort(rows.begin(), rows.end());
+        tmp[s] = rows;
+        S.snrow[s] = (int32_t)rows.size();
+        S.rowoff[s + 1] = S.rowoff[s] + (int64_t)rows.size();
+    }
+    S.rowidx.resize(S.rowoff[ns]);
+    for (int32_t s = 0; s < ns; ++s) {
+        std::copy(tmp[s].begin(), tmp[s].end(), S.rowidx.begin() + S.rowoff[s]);
+        std::vector<int32_t>().swap(tmp[s]);
+    }
+    S.loff.assign(ns + 1, 0);
+    S.uoff.assign(ns + 1, 0);
+    S.reloff.assign(ns + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) {
+        int64_t nc = S.sfirst[s + 1] - S.sfirst[s], nr = S.snrow[s];
+        S.loff[s + 1] = S.loff[s] + nc * nr;
+        S.uoff[s + 1] = S.uoff[s] + (nr - nc) * (nr - nc);
+        S.reloff[s + 1] = S.reloff[s] + (nr - nc);
+    }
+    S.relmap.assign(S.reloff[ns], -1);
+    std::vector<int32_t> pos(m, -1);
+    for (int32_t p = 0; p < ns; ++p) {
+        for (int64_t t = S.rowoff[p]; t < S.rowoff[p + 1]; ++t) pos[S.rowidx[t]] = (int32_t)(t - S.rowoff[p]);
+        for (int32_t t = S.childptr[p]; t < S.childptr[p + 1]; ++t) {
+            int32_t c = S.child[t];
+            int32_t cn = S.sfirst[c + 1] - S.sfirst[c];
+            for (int32_t u = cn; u < S.snrow[c]; ++u) {
+                int32_t r = S.rowidx[S.rowoff[c] + u];
+                S.relmap[S.reloff[c] + u - cn] = pos[r];
+            }
+        }
+    }
+    // level schedule: leaves at level 0, parent after all children
+    S.level.assign(ns, 0);
+    int32_t maxl = 0;
+    for (int32_t s = 0; s < ns; ++s) {
+        int32_t lv = 0;
+        for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) lv = std::max(lv, S.level[S.child[t]] + 1);
+        S.level[s] = lv;
+        maxl = std::max(maxl, lv);
+    }
+    S.nlevels = ns ? maxl + 1 : 0;
+    S.levelptr.assign(S.nlevels + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) S.levelptr[S.level[s] + 1]++;
+    for (int32_t l = 0; l < S.nlevels; ++l) S.levelptr[l + 1] += S.levelptr[l];
+    S.levelsn.resize(ns);
+    {
+        std::vector<int32_t> f(S.levelptr.begin(), S.levelptr.end() - 1);
+        for (int32_t s = 0; s < ns; ++s) S.levelsn[f[S.level[s]]++] = s;
+    }
+    return S;
+}
+
+}  // namespace help

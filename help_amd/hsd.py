@@ -1,0 +1,49 @@
+"""HSD interior-point driver (C++ restatement of HELP HSDSolver.cpp) over a backend.
+
+``solve_lp(inst, backend)`` runs the driver; ``backend`` is a ``help_backend``
+table: the MI355X backend (``gpu_backend(mode)``) or any other C table with
+the same four callbacks (construct/destruct/renew/solve).
+"""
+import ctypes as C
+
+from . import _lib as L
+from .backend import HelpError
+
+
+class GpuBackend:
+    """The MI355X backend as a help_backend callback table."""
+
+    def __init__(self, mode=L.MODE_DIRECT_FP64, **opts):
+        self.opts = L.default_opts(mode=mode, **opts)
+        self.table = L.Backend()
+        st = L.lib().help_gpu_backend_new(C.byref(self.opts), C.byref(self.table))
+        if st != L.HELP_OK:
+            raise HelpError(st, "help_gpu_backend_new failed")
+
+    def close(self):
+        if self.table.ctx:
+            L.lib().help_gpu_backend_free(C.byref(self.table))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_lp(inst, table, max_iter=0, log=None):
+    """Run the HSD driver. ``table`` is an L.Backend. Returns (result, log_rows)."""
+    rows = []
+
+    def _cb(user, it, pobj, dobj, gap, pinf, dinf):
+        rows.append((it, pobj, dobj, gap, pinf, dinf))
+        if log is not None:
+            log(it, pobj, dobj, gap, pinf, dinf)
+
+    cb = L.LOG_FN(_cb)
+    res = L.HsdResult()
+    s = inst.as_struct()
+    st = L.lib().help_hsd_solve(C.byref(s), C.byref(table), int(max_iter), cb, None, C.byref(res))
+    if st != L.HELP_OK:
+        raise HelpError(st, f"help_hsd_solve failed (backend error {res.backend_error})")
+    return res, rows

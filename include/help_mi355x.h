@@ -1,0 +1,245 @@
+/*
+ * help_mi355x.h — C ABI of the MI355X normal-equation backend for HELP.
+ *
+ * The backend solves the per-iteration KKT system of HELP's homogeneous
+ * self-dual interior-point driver,
+ *
+ *     [ D  A^T ] [x1]   [b1]        (A D^-1 A^T + beta I) x2 = A D^-1 b1 - b2
+ *     [ A   0  ] [x2] = [b2]   <=>  x1 = D^-1 (b1 - A^T x2)
+ *
+ * with beta = 1e-5 on the factor only.  Every entry point is a blocking,
+ * host-synchronous call on one HIP stream owned by the handle; all status
+ * codes are ints (0 = success) and the library never calls exit().
+ *
+ * Reference interface replaced (HELP, LP.h:157-160, one backend linked per
+ * binary, Makefile:18-27):
+ *   int  LinearEquation_Construct();                       -> help_gpu_create
+ *   int  LinearEquation_Destruct();                        -> help_gpu_destroy
+ *   void RenewLinearEquation(double* d);                   -> help_gpu_renew
+ *   int  SolveLinearEquation(d, b_1, b_2, x_1, x_2);       -> help_gpu_solve
+ * The adapter translation unit help_amd/csrc/UseMI355X.cpp implements those
+ * four C++ symbols over this ABI.
+ */
+#ifndef HELP_MI355X_H
+#define HELP_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status codes                                                        */
+/* ------------------------------------------------------------------ */
+#define HELP_OK                 0
+#define HELP_ERR_ARG            1   /* bad argument / size               */
+#define HELP_ERR_HIP            2   /* HIP runtime error                 */
+#define HELP_ERR_NOT_PD         3   /* non-positive pivot in the factor  */
+#define HELP_ERR_STATE          4   /* call out of order (e.g. solve before renew) */
+#define HELP_ERR_NOMEM          5
+#define HELP_ERR_NOCONV         6   /* refinement / PCG did not converge */
+#define HELP_ERR_NOGPU          7   /* no HIP device visible              */
+
+/* ------------------------------------------------------------------ */
+/* solve modes                                                         */
+/* ------------------------------------------------------------------ */
+#define HELP_MODE_DIRECT_FP64    0  /* FP64 supernodal factor, CHOLMOD semantics  */
+#define HELP_MODE_DIRECT_FP32    1  /* FP32 factor + FP64 iterative refinement    */
+#define HELP_MODE_PCG            2  /* matrix-free PCG, partial-Cholesky precond. */
+#define HELP_MODE_PCG_LEARNED    3  /* PCG with message-passing network x0        */
+
+/* fault-injection flags (host-forced; every kernel still exits normally) */
+#define HELP_FAULT_FP32_BREAKDOWN  1  /* treat the FP32 factor as broken down     */
+#define HELP_FAULT_REFINE_NOCONV   2  /* treat refinement as non-convergent       */
+#define HELP_FAULT_NET_NONFINITE   4  /* treat the network guess as non-finite    */
+
+typedef struct help_opts {
+    int32_t mode;            /* HELP_MODE_*                                     */
+    double  beta;            /* diagonal shift of the factor (HELP: 1e-5)       */
+    int32_t refine_max;      /* max FP64 refinement steps (FP32 mode)           */
+    double  refine_tol;      /* stop when ||r||/||rhs|| <= refine_tol           */
+    int32_t pcg_k;           /* partial-Cholesky columns (HELP PCG.h:20 uses 10)*/
+    int32_t pcg_max_iter;    /* PCG iteration cap                               */
+    double  pcg_tol;         /* PCG stop: ||r|| <= pcg_tol * ||r0||             */
+    int32_t pcg_fp32_precond;/* 1: apply the partial-Cholesky factor in FP32    */
+    int32_t fault_flags;     /* HELP_FAULT_*                                    */
+    int32_t verbose;
+    int32_t symbolic_only;   /* 1: analysis only, no device state (host tests)  */
+} help_opts;
+
+/* Defaults: FP64 direct, beta=1e-5, refine 20 / 1e-12, pcg_k 500,
+ * pcg_max_iter 5000, pcg_tol 1e-10, FP32 preconditioner. */
+void help_opts_default(help_opts* o);
+
+typedef struct help_solve_stats {
+    int32_t refine_steps;    /* FP64 refinement steps taken                     */
+    int32_t pcg_iters;       /* PCG iterations taken                            */
+    int32_t fallback;        /* 0 none, 1 FP32->FP64 factor, 2 refine->PCG,
+                                3 network x0 -> zero x0                         */
+    double  rel_residual;    /* FP64 ||rhs - M x2|| / ||rhs|| at exit           */
+    double  ms_rhs, ms_solve, ms_back; /* phase times (HIP events)              */
+} help_solve_stats;
+
+typedef struct help_renew_stats {
+    double  ms_assemble;     /* ADA^T numeric assembly                          */
+    double  ms_factor;       /* numeric factorization                           */
+    double  ms_precond;      /* partial-Cholesky build (PCG modes)              */
+    int32_t n_tiny_pivots;   /* pivots replaced in the FP32 factor              */
+} help_renew_stats;
+
+typedef struct help_gpu help_gpu;   /* opaque handle */
+
+/* A in CSC: colptr[n+1], rowind[nnz] (ascending within each column, as
+ * HELP's presolve leaves them: Presolve.cpp:60-75,111-112), val[nnz].
+ * Replaces LinearEquation_Construct (UseCholmod.cpp:29-58). Runs the
+ * symbolic analysis (ADA^T pattern, ordering, etree, supernodes) once. */
+int help_gpu_create(int32_t m, int32_t n, int64_t nnz,
+                    const int64_t* colptr, const int32_t* rowind,
+                    const double* val, const help_opts* opts,
+                    help_gpu** out);
+
+/* Replaces LinearEquation_Destruct (UseCholmod.cpp:60-81). Safe on NULL. */
+int help_gpu_destroy(help_gpu* h);
+
+/* Replaces RenewLinearEquation (UseCholmod.cpp:84-149): assemble
+ * A D^-1 A^T + beta I from d[n] (> 0) and factor it. */
+int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* st);
+
+/* Replaces SolveLinearEquation (UseCholmod.cpp:157-238). Host arrays:
+ * d[n] (same values as the last renew), b1[n], b2[m] in; x1[n], x2[m] out. */
+int help_gpu_solve(help_gpu* h, const double* d, const double* b1,
+                   const double* b2, double* x1, double* x2,
+                   help_solve_stats* st);
+
+/* Text of the last error on this handle ("" when none). */
+const char* help_gpu_last_error(const help_gpu* h);
+/* Text of the last error of help_gpu_create when it returned no handle. */
+const char* help_last_create_error(void);
+
+/* Number of visible HIP devices (0 on a host without a GPU). */
+int help_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* Introspection used by the parity tests (host arrays in and out).    */
+/* ------------------------------------------------------------------ */
+
+/* Sizes of the symbolic analysis. */
+typedef struct help_symbolic_info {
+    int64_t nnz_adat_upper;  /* upper-triangle nnz of A A^T incl. diagonal     */
+    int64_t nnz_l;           /* nnz of L (supernodal, incl. relaxed zeros)     */
+    int64_t nnz_l_exact;     /* nnz of L from the exact column counts          */
+    int64_t n_supernodes;
+    int64_t n_levels;        /* etree levels of the supernodal schedule        */
+    int64_t max_front;       /* largest supernode row count                     */
+    double  factor_flops;    /* sum over columns of colcount^2                  */
+    int64_t n_pairs;         /* ADA^T contribution pairs sum_j c_j(c_j+1)/2     */
+} help_symbolic_info;
+int help_gpu_symbolic_info(const help_gpu* h, help_symbolic_info* info);
+
+/* perm[m] (new -> old), parent[m] (etree of the permuted matrix, -1 root),
+ * colcount[m] (exact column counts of L incl. diagonal). Any may be NULL. */
+int help_gpu_symbolic_arrays(const help_gpu* h, int32_t* perm,
+                             int32_t* parent, int32_t* colcount);
+
+/* ADA^T upper pattern in HELP's row order (diagonal first, then discovery
+ * order; Helper.cpp:273-329): rowptr[m+1], colind[nnz_adat_upper]. */
+int help_gpu_adat_pattern(const help_gpu* h, int64_t* rowptr, int32_t* colind);
+
+/* Run only the numeric assembly on the GPU and return the values of the
+ * upper pattern above (no beta shift), in that slot order. */
+int help_gpu_adat_values(help_gpu* h, const double* d, double* vals);
+
+/* y = A x (trans=0; x[n], y[m]) or y = A^T x (trans=1; x[m], y[n]) on the
+ * GPU, summing each output in ascending index order from zero. */
+int help_gpu_spmv(help_gpu* h, int32_t trans, const double* x, double* y);
+
+/* Dense dump of column j (permuted numbering) of L after renew:
+ * rows[nr], vals[nr]; returns nr via *nr (<= colcount bound). */
+int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows,
+                           double* vals, int32_t* nr);
+
+/* ------------------------------------------------------------------ */
+/* Instance generators (host; deterministic splitmix64/xoshiro256**).  */
+/* ------------------------------------------------------------------ */
+typedef struct help_lp {
+    int32_t m, n;            /* rows, columns                                   */
+    int64_t nnz;
+    int32_t n_ub, n_lb, n_fr;/* HELP column layout: [UB | LB-only | free]        */
+    int64_t* colptr;         /* n+1                                             */
+    int32_t* rowind;         /* nnz, ascending per column                       */
+    double*  val;            /* nnz                                             */
+    double*  b;              /* m                                               */
+    double*  c;              /* n                                               */
+    double*  ub;             /* n_ub (may be NULL when n_ub = 0)                */
+    double   c_intercept;
+} help_lp;
+
+#define HELP_GEN_RANDOM        0   /* A=[B|I], B m x nb with k nnz per column   */
+#define HELP_GEN_MCF           1   /* multicommodity flow on a grid              */
+#define HELP_GEN_STAIRCASE     2   /* staircase, T periods                        */
+#define HELP_GEN_ZIPF          3   /* A=[B|I], Zipf(s) row popularity            */
+
+typedef struct help_gen_params {
+    int32_t kind;            /* HELP_GEN_*                                      */
+    uint64_t seed;
+    int32_t m, nb, k;        /* RANDOM / ZIPF: rows, B columns, nnz per column  */
+    double  zipf_s;          /* ZIPF exponent                                   */
+    int32_t side, commodities, pairs; /* MCF                                    */
+    int32_t periods, block_rows, block_cols, block_k; /* STAIRCASE              */
+} help_gen_params;
+
+/* Allocates the arrays of *lp (free with help_lp_free). */
+int  help_generate(const help_gen_params* p, help_lp* lp);
+void help_lp_free(help_lp* lp);
+/* Write *lp in fixed-format MPS readable by HELP's MPSRead.cpp. */
+int  help_write_mps(const help_lp* lp, const char* path, const char* name);
+
+/* ------------------------------------------------------------------ */
+/* HSD driver (restatement of HELP HSDSolver.cpp) over any backend.    */
+/* ------------------------------------------------------------------ */
+typedef struct help_backend {
+    void* ctx;
+    int (*construct)(void* ctx, const help_lp* lp);
+    int (*destruct)(void* ctx);
+    int (*renew)(void* ctx, const double* d);
+    int (*solve)(void* ctx, const double* d, const double* b1,
+                 const double* b2, double* x1, double* x2);
+} help_backend;
+
+#define HELP_HSD_OK            0
+#define HELP_HSD_STALLED       1
+#define HELP_HSD_ITER_LIMIT    2
+#define HELP_HSD_UNKNOWN_ERROR 3
+
+typedef struct help_hsd_result {
+    int32_t status;          /* HELP_HSD_* (LP.h:163-167)                        */
+    int32_t iterations;
+    double  primal_obj, dual_obj, rel_gap, primal_infeas, dual_infeas, mu;
+    double  seconds;         /* wall time of the HSD_Main span                    */
+    double  seconds_renew, seconds_solve;
+    int32_t n_renew, n_solve;
+    int32_t backend_error;   /* first nonzero backend status, 0 if none          */
+} help_hsd_result;
+
+/* Optional per-iteration log: called after each Newton-parameter update
+ * with (iter, pobj, dobj, gap, pinf, dinf). */
+typedef void (*help_hsd_log_fn)(void* user, int32_t iter, double pobj,
+                                double dobj, double gap, double pinf,
+                                double dinf);
+
+int help_hsd_solve(const help_lp* lp, const help_backend* be,
+                   int32_t max_iter, help_hsd_log_fn log, void* log_user,
+                   help_hsd_result* res);
+
+/* The MI355X backend as a help_backend (ctx owned by the returned table;
+ * release with help_gpu_backend_free). */
+int  help_gpu_backend_new(const help_opts* opts, help_backend* out);
+void help_gpu_backend_free(help_backend* be);
+/* Last help_gpu handle created by a help_gpu_backend (NULL before construct). */
+help_gpu* help_gpu_backend_handle(const help_backend* be);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HELP_MI355X_H */
