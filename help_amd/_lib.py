@@ -35,7 +35,8 @@ class Opts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("beta", C.c_double), ("refine_max", C.c_int32),
                 ("refine_tol", C.c_double), ("pcg_k", C.c_int32), ("pcg_max_iter", C.c_int32),
                 ("pcg_tol", C.c_double), ("pcg_fp32_precond", C.c_int32),
-                ("fault_flags", C.c_int32), ("verbose", C.c_int32), ("symbolic_only", C.c_int32)]
+                ("fault_flags", C.c_int32), ("verbose", C.c_int32), ("symbolic_only", C.c_int32),
+                ("front_path", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -115,6 +116,8 @@ SIGNATURES = {
     "help_gpu_adat_values": (C.c_int, [_P, _D, _D]),
     "help_gpu_spmv": (C.c_int, [_P, C.c_int32, _D, _D]),
     "help_gpu_factor_column": (C.c_int, [_P, C.c_int32, _I32, _D, _I32]),
+    "help_gpu_supernodes": (C.c_int, [_P, _I32, _I32, _I32, _I32]),
+    "help_selftest_mfma": (C.c_int, [C.c_int32, _D, _D, _D]),
     "help_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(LP)]),
     "help_lp_free": (None, [C.POINTER(LP)]),
     "help_write_mps": (C.c_int, [C.POINTER(LP), C.c_char_p, C.c_char_p]),

@@ -123,6 +123,17 @@ class LinearEquation:
                                                      L.i32ptr(cc)), "symbolic_arrays")
         return perm, parent, cc
 
+    def supernodes(self):
+        info = self.symbolic_info()
+        ns = info.n_supernodes
+        sf = np.empty(ns + 1, np.int32)
+        nr = np.empty(ns, np.int32)
+        lv = np.empty(ns, np.int32)
+        tl = np.empty(ns, np.int32)
+        self._check(L.lib().help_gpu_supernodes(self.h, L.i32ptr(sf), L.i32ptr(nr), L.i32ptr(lv),
+                                                L.i32ptr(tl)), "supernodes")
+        return sf, nr, lv, tl
+
     def adat_pattern(self):
         info = self.symbolic_info()
         rp = np.empty(self.m + 1, np.int64)

@@ -142,8 +142,14 @@ void build_assembly_maps(help_gpu* h) {
             if (it == re || *it != row) throw std::runtime_error("assembly map: row not in structure");
             int64_t pos = S.loff[sn] + (int64_t)(col - S.sfirst[sn]) * S.snrow[sn] + (it - rb);
             h->slot_dest[s] = pos;
-            if (k == i) h->diagpos[i] = pos;
         }
+    // the diagonal of every column (also of an empty row of A, whose ADA^T row
+    // has no slot but still receives beta): own columns lead the structure
+    for (int32_t i = 0; i < h->m; ++i) {
+        int32_t col = S.iperm[i], sn = S.col2sn[col];
+        int64_t r = col - S.sfirst[sn];
+        h->diagpos[i] = S.loff[sn] + r * S.snrow[sn] + r;
+    }
     h->colof.resize(h->nnz);
     for (int32_t j = 0; j < h->n; ++j)
         for (int64_t p = h->A.colptr[j]; p < h->A.colptr[j + 1]; ++p) h->colof[p] = j;
@@ -169,7 +175,7 @@ void build_schedules(help_gpu* h) {
         for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
             int32_t s = S.levelsn[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-            if (is_small_front(c, nc)) h->list_small.push_back(s);
+            if (h->opts.front_path == 0 && is_small_front(c, nc)) h->list_small.push_back(s);
             else large.push_back(s);
         }
         if ((int32_t)h->list_small.size() > off)
@@ -644,6 +650,50 @@ int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows, double* vals, 
         return HELP_OK;
     } catch (const std::exception& e) {
         return fail(h, e);
+    }
+}
+
+int help_gpu_supernodes(const help_gpu* h, int32_t* sfirst, int32_t* snrow, int32_t* level,
+                        int32_t* tiled) {
+    if (!h) return HELP_ERR_ARG;
+    const Supernodal& S = h->S;
+    if (sfirst) std::memcpy(sfirst, S.sfirst.data(), sizeof(int32_t) * (S.nsuper + 1));
+    if (snrow) std::memcpy(snrow, S.snrow.data(), sizeof(int32_t) * S.nsuper);
+    if (level) std::memcpy(level, S.level.data(), sizeof(int32_t) * S.nsuper);
+    if (tiled) {
+        for (int32_t s = 0; s < S.nsuper; ++s) tiled[s] = 1;
+        for (int32_t s : h->list_small) tiled[s] = 0;
+    }
+    return HELP_OK;
+}
+
+int help_selftest_mfma(int32_t fp64, const double* a, const double* b, double* out) {
+    if (!a || !b || !out) return HELP_ERR_ARG;
+    try {
+        if (help_device_count() == 0) throw HipError("no HIP device visible", HELP_ERR_NOGPU);
+        if (fp64) {
+            DBuf<double> da, db, dout;
+            da.alloc(64); db.alloc(64); dout.alloc(256);
+            HIPCHK(hipMemcpy(da.p, a, 64 * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(db.p, b, 64 * sizeof(double), hipMemcpyHostToDevice));
+            launch_mfma_probe<double>(da.p, db.p, dout.p, nullptr);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpy(out, dout.p, 256 * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> fa(a, a + 64), fb(b, b + 64), fo(256);
+            DBuf<float> da, db, dout;
+            da.alloc(64); db.alloc(64); dout.alloc(256);
+            HIPCHK(hipMemcpy(da.p, fa.data(), 64 * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(db.p, fb.data(), 64 * sizeof(float), hipMemcpyHostToDevice));
+            launch_mfma_probe<float>(da.p, db.p, dout.p, nullptr);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpy(fo.data(), dout.p, 256 * sizeof(float), hipMemcpyDeviceToHost));
+            for (int i = 0; i < 256; ++i) out[i] = fo[i];
+        }
+        return HELP_OK;
+    } catch (const HipError& e) {
+        g_create_error = e.what();
+        return e.code;
     }
 }
 

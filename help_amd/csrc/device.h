@@ -92,6 +92,8 @@ template <typename T>
 void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, hipStream_t st);
 template <typename T>
 void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st);
+template <typename T>
+void launch_mfma_probe(const T* a, const T* b, T* out, hipStream_t st);
 
 // ---- supernodal triangular solves (kernels_solve.hip) ----
 constexpr int kMaxSnCols = 2048;   // supernodes are split to at most this many columns

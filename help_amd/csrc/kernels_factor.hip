@@ -226,9 +226,11 @@ __global__ __launch_bounds__(256) void k_trsm_rows(const PanelTask* __restrict__
 // ------------------------------------------------------------- MFMA GEMM
 // C[row0:+64, col0:+64] -= P[rows, k0:k1] * P[cols, k0:k1]^T, lower part only,
 // with P the L panel of the front.  4 waves, each a 32x32 quarter built from
-// 2x2 MFMA 16x16x4 blocks.  Lane layout of V_MFMA_*_16X16X4 (one block):
-// A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane
-// j + 16(i/4), element i%4.
+// 2x2 MFMA 16x16x4 blocks.  Lane layout (measured on gfx950 by
+// help_selftest_mfma): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k;
+// the result D[i][j] sits in lane j + 16(i%4), element i/4 for
+// V_MFMA_F64_16X16X4_F64 and in lane j + 16(i/4), element i%4 for
+// V_MFMA_F32_16X16X4_F32.
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
@@ -238,12 +240,14 @@ template <> struct Mfma<double> {
     static __device__ inline acc_t fma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ inline int row(int lane, int r) { return 4 * r + (lane >> 4); }
 };
 template <> struct Mfma<float> {
     typedef float4_t acc_t;
     static __device__ inline acc_t fma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int kKc = 16;   // inner-dimension chunk staged in LDS
@@ -297,7 +301,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int r = 0; r < 4; ++r) {
-                int32_t i = tk.row0 + wr + a * 16 + 4 * (lane >> 4) + r;
+                int32_t i = tk.row0 + wr + a * 16 + Mfma<T>::row(lane, r);
                 int32_t j = tk.col0 + wc + b * 16 + (lane & 15);
                 if (i >= c || j >= tk.col_end || i < j) continue;
                 T v = acc[a][b][r];
@@ -306,7 +310,24 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
             }
 }
 
+// one wave: lane l feeds a[l], b[l]; writes its 4 result elements
+template <typename T>
+__global__ void k_mfma_probe(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int l = threadIdx.x;
+    acc_t acc = acc_t{0, 0, 0, 0};
+    acc = Mfma<T>::fma(a[l], b[l], acc);
+    for (int r = 0; r < 4; ++r) out[4 * l + r] = acc[r];
+}
+
 }  // namespace
+
+template <typename T>
+void launch_mfma_probe(const T* a, const T* b, T* out, hipStream_t st) {
+    k_mfma_probe<T><<<1, 64, 0, st>>>(a, b, out);
+}
+template void launch_mfma_probe<double>(const double*, const double*, double*, hipStream_t);
+template void launch_mfma_probe<float>(const float*, const float*, float*, hipStream_t);
 
 template <typename T>
 void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, int32_t* err,

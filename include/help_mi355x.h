@@ -66,6 +66,7 @@ typedef struct help_opts {
     int32_t fault_flags;     /* HELP_FAULT_*                                    */
     int32_t verbose;
     int32_t symbolic_only;   /* 1: analysis only, no device state (host tests)  */
+    int32_t front_path;      /* 0 auto; 1 every front through the tiled path   */
 } help_opts;
 
 /* Defaults: FP64 direct, beta=1e-5, refine 20 / 1e-12, pcg_k 500,
@@ -158,6 +159,16 @@ int help_gpu_spmv(help_gpu* h, int32_t trans, const double* x, double* y);
  * rows[nr], vals[nr]; returns nr via *nr (<= colcount bound). */
 int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows,
                            double* vals, int32_t* nr);
+
+/* Supernode partition: sfirst[nsuper+1], snrow[nsuper], level[nsuper],
+ * tiled[nsuper] (1 when the front takes the tiled many-workgroup path). */
+int help_gpu_supernodes(const help_gpu* h, int32_t* sfirst, int32_t* snrow,
+                        int32_t* level, int32_t* tiled);
+
+/* Lane-layout probe of the 16x16x4 MFMA used by the factor (fp64 != 0:
+ * V_MFMA_F64_16X16X4_F64, else V_MFMA_F32_16X16X4_F32): lane l feeds
+ * a[l] and b[l]; out[4*l + r] receives element r of lane l's result. */
+int help_selftest_mfma(int32_t fp64, const double* a, const double* b, double* out);
 
 /* ------------------------------------------------------------------ */
 /* Instance generators (host; deterministic splitmix64/xoshiro256**).  */
