@@ -36,7 +36,7 @@ class Opts(C.Structure):
                 ("refine_tol", C.c_double), ("pcg_k", C.c_int32), ("pcg_max_iter", C.c_int32),
                 ("pcg_tol", C.c_double), ("pcg_fp32_precond", C.c_int32),
                 ("fault_flags", C.c_int32), ("verbose", C.c_int32), ("symbolic_only", C.c_int32),
-                ("front_path", C.c_int32)]
+                ("front_path", C.c_int32), ("profile", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -126,7 +126,18 @@ SIGNATURES = {
     "help_gpu_backend_new": (C.c_int, [C.POINTER(Opts), C.POINTER(Backend)]),
     "help_gpu_backend_free": (None, [C.POINTER(Backend)]),
     "help_gpu_backend_handle": (_P, [C.POINTER(Backend)]),
+    "help_gpu_backend_kernel_stats": (C.c_int, [C.POINTER(Backend), C.c_int32, _D,
+                                                C.POINTER(C.c_int64), _D]),
+    "help_gpu_profile": (C.c_int, [_P, C.c_int32]),
+    "help_gpu_kernel_stats": (C.c_int, [_P, C.c_int32, _D, C.POINTER(C.c_int64), _D,
+                                        C.POINTER(C.c_char_p)]),
+    "help_device_synchronize": (C.c_int, []),
 }
+
+KERNEL_NAMES = ["k_small_fronts", "k_extend_add", "k_potrf_diag", "k_trsm_rows", "k_gemm", "k_fwd",
+                "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub"]
+# work unit of each kernel class: flops for the factor kernels, HBM bytes otherwise
+KERNEL_BOUND = ["mfma", "hbm", "mfma", "mfma", "mfma", "hbm", "hbm", "hbm", "hbm", "hbm", "hbm", "hbm"]
 
 _lib = None
 

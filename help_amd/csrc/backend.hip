@@ -58,14 +58,22 @@ struct DBuf {   // owning device buffer
     ~DBuf() { free(); }
 };
 
+// kernel classes (profiling index = HELP_K_* in the public header)
 enum OpKind : int32_t {
-    OP_SMALL = 0, OP_EA, OP_POTRF, OP_TRSM, OP_GEMM,        // factor
-    OP_FWD, OP_FWD_GEMV, OP_BWD_GEMV, OP_BWD                 // solve
+    OP_SMALL = HELP_K_SMALL_FRONTS, OP_EA = HELP_K_EXTEND_ADD, OP_POTRF = HELP_K_POTRF,
+    OP_TRSM = HELP_K_TRSM, OP_GEMM = HELP_K_GEMM,
+    OP_FWD = HELP_K_FWD, OP_FWD_GEMV = HELP_K_FWD_GEMV, OP_BWD_GEMV = HELP_K_BWD_GEMV,
+    OP_BWD = HELP_K_BWD
 };
 struct Op {
     OpKind kind;
     int32_t off, count;
+    double work;   // algorithmic flops (factor classes) or bytes (the others)
 };
+
+const char* kKindName[HELP_K_COUNT] = {
+    "k_small_fronts", "k_extend_add", "k_potrf_diag", "k_trsm_rows", "k_gemm",
+    "k_fwd", "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub"};
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -113,11 +121,19 @@ struct help_gpu {
     DBuf<GemmTask> d_gemm;
     DBuf<SolveTask> d_fgemv, d_bgemv;
     DevSym sym{};
+    // profiling
+    bool prof = false;
+    struct KStat { double ms = 0, work = 0; int64_t launches = 0; } kstat[HELP_K_COUNT];
+    struct Pending { int32_t kind; double work; hipEvent_t a, b; };
+    std::vector<hipEvent_t> evpool;
+    std::vector<Pending> pending;
+    double work_adat = 0, work_rhs = 0, work_backsub = 0;
 
     ~help_gpu() {
         if (st) { (void)hipStreamSynchronize(st); }
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : evpool) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
 };
@@ -158,6 +174,53 @@ void build_assembly_maps(help_gpu* h) {
     for (int64_t t = 0; t < h->nnz; ++t) h->csrval[t] = h->A.val[h->R.cscpos[t]];
 }
 
+// ---- profiling: events around launches on the handle's stream ----
+struct ProfScope {
+    help_gpu* h;
+    int32_t kind;
+    double work;
+    hipEvent_t b = nullptr;
+    ProfScope(help_gpu* h_, int32_t k, double w) : h(h_), kind(k), work(w) {
+        if (!h->prof) return;
+        size_t used = h->pending.size() * 2;
+        while (h->evpool.size() < used + 2) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            h->evpool.push_back(e);
+        }
+        hipEvent_t a = h->evpool[used];
+        b = h->evpool[used + 1];
+        HIPCHK(hipEventRecord(a, h->st));
+        h->pending.push_back({kind, work, a, b});
+    }
+    ~ProfScope() {
+        if (h->prof && b) (void)hipEventRecord(b, h->st);
+    }
+};
+
+// after the stream has been synchronized
+void collect_prof(help_gpu* h) {
+    for (auto& p : h->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            h->kstat[p.kind].ms += ms;
+            h->kstat[p.kind].work += p.work;
+            h->kstat[p.kind].launches += 1;
+        }
+    }
+    h->pending.clear();
+}
+
+// lower-triangle entries (i >= j) of a rows x cols tile at (r0, c0) below row/col limits
+double tile_lower_entries(int32_t r0, int32_t c0, int32_t rlim, int32_t clim) {
+    double cnt = 0;
+    for (int32_t j = c0; j < std::min(c0 + kTile, clim); ++j) {
+        int32_t lo = std::max(r0, j), hi = std::min(r0 + kTile, rlim);
+        if (hi > lo) cnt += hi - lo;
+    }
+    return cnt;
+}
+
 bool is_small_front(int32_t c, int32_t ncol) {
     return c <= kSmallRows && (int64_t)c * ncol <= kSmallElems;
 }
@@ -169,81 +232,131 @@ bool is_big_solve(int32_t c, int32_t ncol) {
 void build_schedules(help_gpu* h) {
     const Supernodal& S = h->S;
     h->sbig.assign(S.nsuper, 0);
+    auto ncol = [&](int32_t s) { return S.sfirst[s + 1] - S.sfirst[s]; };
+    // Cholesky flops of a supernode's columns: sum over its columns of colcount^2
+    auto sn_flops = [&](int32_t s) {
+        double f = 0;
+        for (int32_t k = 0; k < ncol(s); ++k) { double cj = S.snrow[s] - k; f += cj * cj; }
+        return f;
+    };
+    // bytes of reading the children's update matrices and updating the front
+    auto ea_bytes = [&](int32_t s) {
+        double b = 0;
+        for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
+            double cu = S.snrow[S.child[t]] - ncol(S.child[t]);
+            b += 3.0 * 8.0 * cu * (cu + 1) / 2;
+        }
+        return b;
+    };
     for (int32_t l = 0; l < S.nlevels; ++l) {
         std::vector<int32_t> large;
         int32_t off = (int32_t)h->list_small.size();
+        double w = 0;
         for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
             int32_t s = S.levelsn[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-            if (h->opts.front_path == 0 && is_small_front(c, nc)) h->list_small.push_back(s);
+            if (h->opts.front_path == 0 && is_small_front(c, nc)) { h->list_small.push_back(s); w += sn_flops(s); }
             else large.push_back(s);
         }
         if ((int32_t)h->list_small.size() > off)
-            h->fops.push_back({OP_SMALL, off, (int32_t)h->list_small.size() - off});
+            h->fops.push_back({OP_SMALL, off, (int32_t)h->list_small.size() - off, w});
         if (large.empty()) continue;
         // extend-add by 32-column chunks of each front
         off = (int32_t)h->ea.size();
+        w = 0;
         for (int32_t s : large) {
             int32_t c = S.snrow[s];
             if (S.childptr[s + 1] == S.childptr[s]) continue;
+            w += ea_bytes(s);
             for (int32_t c0 = 0; c0 < c; c0 += 32) h->ea.push_back({s, c0, std::min(c, c0 + 32)});
         }
-        if ((int32_t)h->ea.size() > off) h->fops.push_back({OP_EA, off, (int32_t)h->ea.size() - off});
+        if ((int32_t)h->ea.size() > off) h->fops.push_back({OP_EA, off, (int32_t)h->ea.size() - off, w});
         int32_t maxn = 0;
         for (int32_t s : large) maxn = std::max(maxn, S.sfirst[s + 1] - S.sfirst[s]);
         for (int32_t b = 0; b < maxn; b += kTile) {
             int32_t po = (int32_t)h->potrf.size(), to = (int32_t)h->trsm.size(),
                     go = (int32_t)h->gemm.size();
+            double wp = 0, wt = 0, wg = 0;
             for (int32_t s : large) {
                 int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
                 if (b >= nc) continue;
                 int32_t nb = std::min(kTile, nc - b);
                 h->potrf.push_back({s, b, 0});
-                for (int32_t r0 = b + nb, rb = 1; r0 < c; r0 += kTile, ++rb) h->trsm.push_back({s, b, rb});
+                wp += (double)nb * nb * nb / 3.0;
+                for (int32_t r0 = b + nb, rb = 1; r0 < c; r0 += kTile, ++rb) {
+                    h->trsm.push_back({s, b, rb});
+                    wt += (double)nb * nb * std::min(kTile, c - r0);
+                }
                 for (int32_t c0 = b + nb; c0 < nc; c0 += kTile)
-                    for (int32_t r0 = c0; r0 < c; r0 += kTile)
+                    for (int32_t r0 = c0; r0 < c; r0 += kTile) {
                         h->gemm.push_back({s, r0, c0, b, b + nb, nc});
+                        wg += 2.0 * nb * tile_lower_entries(r0, c0, c, nc);
+                    }
             }
-            h->fops.push_back({OP_POTRF, po, (int32_t)h->potrf.size() - po});
-            if ((int32_t)h->trsm.size() > to) h->fops.push_back({OP_TRSM, to, (int32_t)h->trsm.size() - to});
-            if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go});
+            h->fops.push_back({OP_POTRF, po, (int32_t)h->potrf.size() - po, wp});
+            if ((int32_t)h->trsm.size() > to) h->fops.push_back({OP_TRSM, to, (int32_t)h->trsm.size() - to, wt});
+            if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
         }
         // U -= L21 L21^T with the whole panel as inner dimension
         int32_t go = (int32_t)h->gemm.size();
+        double wg = 0;
         for (int32_t s : large) {
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
             for (int32_t c0 = nc; c0 < c; c0 += kTile)
-                for (int32_t r0 = c0; r0 < c; r0 += kTile) h->gemm.push_back({s, r0, c0, 0, nc, c});
+                for (int32_t r0 = c0; r0 < c; r0 += kTile) {
+                    h->gemm.push_back({s, r0, c0, 0, nc, c});
+                    wg += 2.0 * nc * tile_lower_entries(r0, c0, c, c);
+                }
         }
-        if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go});
+        if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
     }
     // triangular solves
+    // algorithmic bytes of the solves: each stored factor entry is read once
+    // per direction (element size of the factor type)
+    const double wsz = h->use_fp32 ? 4.0 : 8.0;
+    auto tri_bytes = [&](int32_t s, bool with_l21) {
+        double n = ncol(s), c = S.snrow[s];
+        return wsz * (n * (n + 1) / 2 + (with_l21 ? (c - n) * n : 0.0));
+    };
     for (int32_t l = 0; l < S.nlevels; ++l) {
         int32_t off = (int32_t)h->list_fwd.size(), to = (int32_t)h->fgemv.size();
+        double w = 0, wg = 0;
         for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
             int32_t s = S.levelsn[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
             h->list_fwd.push_back(s);
-            if (is_big_solve(c, nc)) {
+            bool big = is_big_solve(c, nc);
+            w += tri_bytes(s, !big);
+            if (big) {
                 h->sbig[s] = 1;
+                wg += wsz * (double)(c - nc) * nc;
                 for (int32_t b = 0; nc + b * 64 < c; ++b) h->fgemv.push_back({s, b});
             }
         }
-        h->sops_fwd.push_back({OP_FWD, off, (int32_t)h->list_fwd.size() - off});
-        if ((int32_t)h->fgemv.size() > to) h->sops_fwd.push_back({OP_FWD_GEMV, to, (int32_t)h->fgemv.size() - to});
+        h->sops_fwd.push_back({OP_FWD, off, (int32_t)h->list_fwd.size() - off, w});
+        if ((int32_t)h->fgemv.size() > to) h->sops_fwd.push_back({OP_FWD_GEMV, to, (int32_t)h->fgemv.size() - to, wg});
     }
     for (int32_t l = S.nlevels - 1; l >= 0; --l) {
         int32_t off = (int32_t)h->list_bwd.size(), to = (int32_t)h->bgemv.size();
+        double w = 0, wg = 0;
         for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
             int32_t s = S.levelsn[t];
-            int32_t nc = S.sfirst[s + 1] - S.sfirst[s];
+            int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
             h->list_bwd.push_back(s);
-            if (h->sbig[s])
+            w += tri_bytes(s, !h->sbig[s]);
+            if (h->sbig[s]) {
+                wg += wsz * (double)(c - nc) * nc;
                 for (int32_t b = 0; b * 64 < nc; ++b) h->bgemv.push_back({s, b});
+            }
         }
-        if ((int32_t)h->bgemv.size() > to) h->sops_bwd.push_back({OP_BWD_GEMV, to, (int32_t)h->bgemv.size() - to});
-        h->sops_bwd.push_back({OP_BWD, off, (int32_t)h->list_bwd.size() - off});
+        if ((int32_t)h->bgemv.size() > to) h->sops_bwd.push_back({OP_BWD_GEMV, to, (int32_t)h->bgemv.size() - to, wg});
+        h->sops_bwd.push_back({OP_BWD, off, (int32_t)h->list_bwd.size() - off, w});
     }
+    // SpMV-class kernels: 12 bytes per nonzero (value + index) plus vectors
+    const double nnz = (double)h->nnz, m = h->m, n = h->n;
+    h->work_adat = 12.0 * nnz + 8.0 * n + 12.0 * (double)h->P.colind.size();
+    h->work_rhs = 12.0 * nnz + 8.0 * (m + 1) + 16.0 * n + 8.0 * m + 4.0 * m;
+    h->work_backsub = 12.0 * nnz + 8.0 * (n + 1) + 8.0 * m + 24.0 * n;
 }
 
 void setup_device(help_gpu* h) {
@@ -308,6 +421,7 @@ template <typename T>
 void run_factor(help_gpu* h, T* L, T* U) {
     hipStream_t st = h->st;
     for (const Op& o : h->fops) {
+        ProfScope ps(h, o.kind, o.work);
         switch (o.kind) {
             case OP_SMALL: launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, h->d_err.p, st); break;
             case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->sym, L, U, st); break;
@@ -325,10 +439,12 @@ template <typename T>
 void run_trisolve(help_gpu* h, const T* L, double* x) {
     hipStream_t st = h->st;
     for (const Op& o : h->sops_fwd) {
+        ProfScope ps(h, o.kind, o.work);
         if (o.kind == OP_FWD) launch_fwd<T>(h->d_list_fwd.p + o.off, o.count, h->sym, h->d_big.p, L, x, h->d_W.p, st);
         else launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, x, h->d_W.p, st);
     }
     for (const Op& o : h->sops_bwd) {
+        ProfScope ps(h, o.kind, o.work);
         if (o.kind == OP_BWD) launch_bwd<T>(h->d_list_bwd.p + o.off, o.count, h->sym, h->d_big.p, L, x, h->d_t.p, st);
         else launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, st);
     }
@@ -413,6 +529,7 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
         build_assembly_maps(h);
         build_schedules(h);
         if (!h->opts.symbolic_only) setup_device(h);
+        h->prof = h->opts.profile != 0;
     } catch (const std::exception& e) {
         g_create_error = e.what();
         int code = HELP_ERR_ARG;
@@ -448,14 +565,20 @@ int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* rs) {
         if (h->use_fp32) {
             HIPCHK(hipMemsetAsync(h->d_L32.p, 0, sizeof(float) * nl, st));
             HIPCHK(hipMemsetAsync(h->d_U32.p, 0, sizeof(float) * std::max<int64_t>(nu, 1), st));
-            launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
-                                 h->d_dinv.p, nslot, h->d_dest.p, nullptr, h->d_L32.p, st);
+            {
+                ProfScope ps(h, HELP_K_ADAT, h->work_adat);
+                launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
+                                     h->d_dinv.p, nslot, h->d_dest.p, nullptr, h->d_L32.p, st);
+            }
             launch_add_beta(h->d_diagpos.p, m, h->opts.beta, nullptr, h->d_L32.p, st);
         } else {
             HIPCHK(hipMemsetAsync(h->d_L64.p, 0, sizeof(double) * nl, st));
             HIPCHK(hipMemsetAsync(h->d_U64.p, 0, sizeof(double) * std::max<int64_t>(nu, 1), st));
-            launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
-                                 h->d_dinv.p, nslot, h->d_dest.p, h->d_L64.p, nullptr, st);
+            {
+                ProfScope ps(h, HELP_K_ADAT, h->work_adat);
+                launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
+                                     h->d_dinv.p, nslot, h->d_dest.p, h->d_L64.p, nullptr, st);
+            }
             launch_add_beta(h->d_diagpos.p, m, h->opts.beta, h->d_L64.p, nullptr, st);
         }
         HIPCHK(hipEventRecord(h->ev[1], st));
@@ -465,6 +588,7 @@ int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* rs) {
         int32_t nerr = 0;
         HIPCHK(hipMemcpyAsync(&nerr, h->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        if (h->prof) collect_prof(h);
         h->last_tiny = nerr;
         h->factored = true;
         if (rs) {
@@ -492,8 +616,11 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
         HIPCHK(hipMemcpyAsync(h->d_b2.p, b2, sizeof(double) * m, hipMemcpyHostToDevice, st));
         HIPCHK(hipEventRecord(h->ev[0], st));
         // r = A (b1 ./ d) - b2, permuted
-        launch_rhs(m, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p, h->d_d.p, h->d_b2.p,
-                   h->d_iperm.p, h->d_rp.p, st);
+        {
+            ProfScope ps(h, HELP_K_RHS, h->work_rhs);
+            launch_rhs(m, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p, h->d_d.p, h->d_b2.p,
+                       h->d_iperm.p, h->d_rp.p, st);
+        }
         HIPCHK(hipEventRecord(h->ev[1], st));
         if (!h->use_fp32) {
             run_trisolve<double>(h, h->d_L64.p, h->d_rp.p);
@@ -536,13 +663,17 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
             loc.rel_residual = rel;
         }
         HIPCHK(hipEventRecord(h->ev[2], st));
-        launch_backsub(n, h->d_colptr.p, h->d_rowind.p, h->d_val.p, h->d_b1.p, h->d_d.p, h->d_x2.p,
-                       h->d_x1.p, st);
+        {
+            ProfScope ps(h, HELP_K_BACKSUB, h->work_backsub);
+            launch_backsub(n, h->d_colptr.p, h->d_rowind.p, h->d_val.p, h->d_b1.p, h->d_d.p, h->d_x2.p,
+                           h->d_x1.p, st);
+        }
         HIPCHK(hipEventRecord(h->ev[3], st));
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(x1, h->d_x1.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(x2, h->d_x2.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        if (h->prof) collect_prof(h);
         loc.ms_rhs = elapsed_ms(h->ev[0], h->ev[1]);
         loc.ms_solve = elapsed_ms(h->ev[1], h->ev[2]);
         loc.ms_back = elapsed_ms(h->ev[2], h->ev[3]);
@@ -653,6 +784,28 @@ int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows, double* vals, 
     }
 }
 
+int help_gpu_profile(help_gpu* h, int32_t enable) {
+    if (!h) return HELP_ERR_ARG;
+    h->prof = enable != 0;
+    for (auto& k : h->kstat) k = help_gpu::KStat();
+    h->pending.clear();
+    return HELP_OK;
+}
+
+int help_gpu_kernel_stats(const help_gpu* h, int32_t kind, double* ms, int64_t* launches,
+                          double* work, const char** name) {
+    if (!h || kind < 0 || kind >= HELP_K_COUNT) return HELP_ERR_ARG;
+    if (ms) *ms = h->kstat[kind].ms;
+    if (launches) *launches = h->kstat[kind].launches;
+    if (work) *work = h->kstat[kind].work;
+    if (name) *name = kKindName[kind];
+    return HELP_OK;
+}
+
+int help_device_synchronize(void) {
+    return hipDeviceSynchronize() == hipSuccess ? HELP_OK : HELP_ERR_HIP;
+}
+
 int help_gpu_supernodes(const help_gpu* h, int32_t* sfirst, int32_t* snrow, int32_t* level,
                         int32_t* tiled) {
     if (!h) return HELP_ERR_ARG;
@@ -702,17 +855,27 @@ struct GpuBackendCtx {
     help_opts opts;
     help_gpu* h = nullptr;
     int32_t last_status = 0;
+    help_gpu::KStat kstat[HELP_K_COUNT];
 };
+
+static void gb_release(GpuBackendCtx* c) {
+    if (!c->h) return;
+    for (int k = 0; k < HELP_K_COUNT; ++k) {
+        c->kstat[k].ms += c->h->kstat[k].ms;
+        c->kstat[k].work += c->h->kstat[k].work;
+        c->kstat[k].launches += c->h->kstat[k].launches;
+    }
+    help_gpu_destroy(c->h);
+    c->h = nullptr;
+}
 
 static int gb_construct(void* ctx, const help_lp* lp) {
     auto* c = (GpuBackendCtx*)ctx;
-    if (c->h) { help_gpu_destroy(c->h); c->h = nullptr; }
+    gb_release(c);
     return help_gpu_create(lp->m, lp->n, lp->nnz, lp->colptr, lp->rowind, lp->val, &c->opts, &c->h);
 }
 static int gb_destruct(void* ctx) {
-    auto* c = (GpuBackendCtx*)ctx;
-    help_gpu_destroy(c->h);
-    c->h = nullptr;
+    gb_release((GpuBackendCtx*)ctx);
     return 0;
 }
 static int gb_renew(void* ctx, const double* d) {
@@ -750,6 +913,19 @@ void help_gpu_backend_free(help_backend* be) {
 help_gpu* help_gpu_backend_handle(const help_backend* be) {
     if (!be || !be->ctx) return nullptr;
     return ((GpuBackendCtx*)be->ctx)->h;
+}
+
+int help_gpu_backend_kernel_stats(const help_backend* be, int32_t kind, double* ms,
+                                  int64_t* launches, double* work) {
+    if (!be || !be->ctx || kind < 0 || kind >= HELP_K_COUNT) return HELP_ERR_ARG;
+    auto* c = (GpuBackendCtx*)be->ctx;
+    double t = c->kstat[kind].ms, w = c->kstat[kind].work;
+    int64_t n = c->kstat[kind].launches;
+    if (c->h) { t += c->h->kstat[kind].ms; w += c->h->kstat[kind].work; n += c->h->kstat[kind].launches; }
+    if (ms) *ms = t;
+    if (launches) *launches = n;
+    if (work) *work = w;
+    return HELP_OK;
 }
 
 }  // extern "C"

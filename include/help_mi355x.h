@@ -67,6 +67,7 @@ typedef struct help_opts {
     int32_t verbose;
     int32_t symbolic_only;   /* 1: analysis only, no device state (host tests)  */
     int32_t front_path;      /* 0 auto; 1 every front through the tiled path   */
+    int32_t profile;         /* 1: kernel-class profiling from creation on      */
 } help_opts;
 
 /* Defaults: FP64 direct, beta=1e-5, refine 20 / 1e-12, pcg_k 500,
@@ -160,6 +161,29 @@ int help_gpu_spmv(help_gpu* h, int32_t trans, const double* x, double* y);
 int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows,
                            double* vals, int32_t* nr);
 
+/* Kernel-class profiling: with profiling on, every launch of the library is
+ * bracketed by HIP events on the handle's stream and its time and algorithmic
+ * work (flops for the factor classes, HBM bytes for the others) accumulate
+ * per class.  help_gpu_profile resets the counters. */
+#define HELP_K_SMALL_FRONTS  0   /* one workgroup per small front (flops)   */
+#define HELP_K_EXTEND_ADD    1   /* large-front extend-add (bytes)          */
+#define HELP_K_POTRF         2   /* 64x64 diagonal-block Cholesky (flops)   */
+#define HELP_K_TRSM          3   /* panel row-block solve (flops)           */
+#define HELP_K_GEMM          4   /* MFMA SYRK/GEMM update (flops)           */
+#define HELP_K_FWD           5   /* forward solve per supernode (bytes)     */
+#define HELP_K_FWD_GEMV      6   /* forward L21 product (bytes)             */
+#define HELP_K_BWD_GEMV      7   /* backward L21^T product (bytes)          */
+#define HELP_K_BWD           8   /* backward solve per supernode (bytes)    */
+#define HELP_K_ADAT          9   /* ADA^T numeric assembly (bytes)          */
+#define HELP_K_RHS          10   /* r = A D^-1 b1 - b2 (bytes)              */
+#define HELP_K_BACKSUB      11   /* x1 = D^-1 (b1 - A^T x2) (bytes)         */
+#define HELP_K_COUNT        12
+int help_gpu_profile(help_gpu* h, int32_t enable);
+int help_gpu_kernel_stats(const help_gpu* h, int32_t kind, double* ms, int64_t* launches,
+                          double* work, const char** name);
+/* Synchronize the current HIP device (all streams). */
+int help_device_synchronize(void);
+
 /* Supernode partition: sfirst[nsuper+1], snrow[nsuper], level[nsuper],
  * tiled[nsuper] (1 when the front takes the tiled many-workgroup path). */
 int help_gpu_supernodes(const help_gpu* h, int32_t* sfirst, int32_t* snrow,
@@ -249,6 +273,10 @@ int  help_gpu_backend_new(const help_opts* opts, help_backend* out);
 void help_gpu_backend_free(help_backend* be);
 /* Last help_gpu handle created by a help_gpu_backend (NULL before construct). */
 help_gpu* help_gpu_backend_handle(const help_backend* be);
+/* Kernel-class totals over every handle this backend has destructed (with
+ * opts.profile = 1), plus its live handle. */
+int help_gpu_backend_kernel_stats(const help_backend* be, int32_t kind, double* ms,
+                                  int64_t* launches, double* work);
 
 #ifdef __cplusplus
 }
