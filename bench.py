@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
     ap.add_argument("--mode", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--driver", default="gpu", choices=["gpu", "host"],
+                    help="gpu: device-resident HSD vectors; host: HELP-order host driver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     args = ap.parse_args()
@@ -130,7 +132,7 @@ def main():
 
     for w in range(args.warmup):
         gb = GpuBackend(mode=mode)
-        res, _ = solve_lp(inst, gb.table)
+        res, _ = solve_lp(inst, gb.table, device=args.driver == "gpu")
         gb.close()
         print(f"[bench] warmup {w}: status {res.status} iters {res.iterations} {res.seconds:.2f}s",
               file=sys.stderr, flush=True)
@@ -141,7 +143,7 @@ def main():
     t_start = time.time()
     for k in range(args.steps):
         ts = time.time()
-        res, _ = solve_lp(inst, gb.table)
+        res, _ = solve_lp(inst, gb.table, device=args.driver == "gpu")
         L.lib().help_device_synchronize()
         times.append(time.time() - ts)
         results.append(res)
@@ -157,6 +159,7 @@ def main():
     kernels = {r["kernel"]: dict(ms_total=round(r["ms"], 3), launches=r["launches"],
                                  rate=round(r["work"] / (r["ms"] * 1e-3) / (1e9 if r["bound"] == "hbm" else 1e12), 3),
                                  unit="GB/s" if r["bound"] == "hbm" else "TFLOP/s") for r in rows}
+    # driver work outside the backend (construct, vector updates, reductions)
     host_driver = max(0.0, res.seconds - res.seconds_renew - res.seconds_solve)
     cpu = None
     if not args.no_cpu_baseline:
@@ -175,7 +178,12 @@ def main():
                    "status": L.HSD_STATUS.get(res.status, str(res.status)), "iterations": res.iterations,
                    "rel_gap": res.rel_gap, "primal_obj": res.primal_obj,
                    "renew_s": round(res.seconds_renew, 3), "solve_s": round(res.seconds_solve, 3),
-                   "host_driver_s": round(host_driver, 3), "n_renew": res.n_renew, "n_solve": res.n_solve},
+                   "driver": args.driver, "driver_s": round(host_driver, 3),
+                   "construct_s": round(res.seconds_construct, 3),
+                   "driver_newton_s": round(res.seconds_newton, 3),
+                   "driver_direction_s": round(res.seconds_direction, 3),
+                   "driver_step_s": round(res.seconds_step, 3),
+                   "n_renew": res.n_renew, "n_solve": res.n_solve},
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
     }
     print(json.dumps(line), flush=True)

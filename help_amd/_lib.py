@@ -88,6 +88,8 @@ class HsdResult(C.Structure):
                 ("dual_obj", C.c_double), ("rel_gap", C.c_double), ("primal_infeas", C.c_double),
                 ("dual_infeas", C.c_double), ("mu", C.c_double), ("seconds", C.c_double),
                 ("seconds_renew", C.c_double), ("seconds_solve", C.c_double),
+                ("seconds_construct", C.c_double), ("seconds_newton", C.c_double),
+                ("seconds_direction", C.c_double), ("seconds_step", C.c_double),
                 ("n_renew", C.c_int32), ("n_solve", C.c_int32), ("backend_error", C.c_int32)]
 
 
@@ -121,8 +123,10 @@ SIGNATURES = {
     "help_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(LP)]),
     "help_lp_free": (None, [C.POINTER(LP)]),
     "help_write_mps": (C.c_int, [C.POINTER(LP), C.c_char_p, C.c_char_p]),
-    "help_hsd_solve": (C.c_int, [C.POINTER(LP), C.POINTER(Backend), C.c_int32, LOG_FN, _P,
-                                 C.POINTER(HsdResult)]),
+    "help_hsd_solve": (C.c_int, [C.POINTER(LP), C.POINTER(Backend), C.c_int32, C.c_int32, LOG_FN,
+                                 _P, C.POINTER(HsdResult)]),
+    "help_hsd_solve_gpu": (C.c_int, [C.POINTER(LP), C.POINTER(Backend), C.c_int32, LOG_FN, _P,
+                                     C.POINTER(HsdResult)]),
     "help_gpu_backend_new": (C.c_int, [C.POINTER(Opts), C.POINTER(Backend)]),
     "help_gpu_backend_free": (None, [C.POINTER(Backend)]),
     "help_gpu_backend_handle": (_P, [C.POINTER(Backend)]),

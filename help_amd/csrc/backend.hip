@@ -21,122 +21,14 @@
 
 using namespace help;
 
-namespace {
+#include "handle.h"
 
+namespace help {
 thread_local std::string g_create_error;
-
-struct HipError : std::runtime_error {
-    int code;
-    HipError(const std::string& s, int c) : std::runtime_error(s), code(c) {}
-};
-
-#define HIPCHK(expr)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_), HELP_ERR_HIP);  \
-    } while (0)
-
-template <typename T>
-struct DBuf {   // owning device buffer
-    T* p = nullptr;
-    size_t n = 0;
-    void alloc(size_t cnt) {
-        free();
-        n = cnt;
-        if (cnt) HIPCHK(hipMalloc((void**)&p, cnt * sizeof(T)));
-    }
-    void upload(const std::vector<T>& v, hipStream_t st) {
-        alloc(v.size());
-        if (n) HIPCHK(hipMemcpyAsync(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice, st));
-    }
-    void free() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DBuf() { free(); }
-};
-
-// kernel classes (profiling index = HELP_K_* in the public header)
-enum OpKind : int32_t {
-    OP_SMALL = HELP_K_SMALL_FRONTS, OP_EA = HELP_K_EXTEND_ADD, OP_POTRF = HELP_K_POTRF,
-    OP_TRSM = HELP_K_TRSM, OP_GEMM = HELP_K_GEMM,
-    OP_FWD = HELP_K_FWD, OP_FWD_GEMV = HELP_K_FWD_GEMV, OP_BWD_GEMV = HELP_K_BWD_GEMV,
-    OP_BWD = HELP_K_BWD
-};
-struct Op {
-    OpKind kind;
-    int32_t off, count;
-    double work;   // algorithmic flops (factor classes) or bytes (the others)
-};
-
-const char* kKindName[HELP_K_COUNT] = {
-    "k_small_fronts", "k_extend_add", "k_potrf_diag", "k_trsm_rows", "k_gemm",
-    "k_fwd", "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub"};
-
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-}  // namespace
-
-struct help_gpu {
-    int32_t m = 0, n = 0;
-    int64_t nnz = 0;
-    help_opts opts{};
-    CscMatrix A;
-    CsrMatrix R;
-    AdatPattern P;
-    Supernodal S;
-    std::vector<int64_t> slot_dest, diagpos;
-    std::vector<int32_t> colof, csrcol;
-    std::vector<double> csrval;
-    std::vector<int8_t> sbig;
-    // schedules
-    std::vector<Op> fops, sops_fwd, sops_bwd;
-    std::vector<int32_t> list_small, list_fwd, list_bwd;
-    std::vector<EaTask> ea;
-    std::vector<PanelTask> potrf, trsm;
-    std::vector<GemmTask> gemm;
-    std::vector<SolveTask> fgemv, bgemv;
-    std::string err;
-    bool device = false;
-    bool factored = false;
-    bool use_fp32 = false;
-    int32_t last_tiny = 0;
-    // device state
-    hipStream_t st = nullptr;
-    hipEvent_t ev[6] = {};
-    DBuf<int64_t> d_colptr, d_rowptr, d_pairptr, d_dest, d_diagpos, d_loff, d_uoff, d_rowoff, d_reloff;
-    DBuf<int32_t> d_rowind, d_colind, d_pair_p, d_pair_q, d_colof, d_iperm, d_sfirst, d_snrow,
-        d_rowidx, d_relmap, d_childptr, d_child, d_list_small, d_list_fwd, d_list_bwd, d_err;
-    DBuf<double> d_val, d_csrval, d_d, d_dinv, d_b1, d_b2, d_x1, d_x2, d_rp, d_W, d_t, d_tn, d_r,
-        d_c, d_slotvals;
-    DBuf<double> d_L64, d_U64;
-    DBuf<float> d_L32, d_U32;
-    DBuf<int8_t> d_big;
-    DBuf<EaTask> d_ea;
-    DBuf<PanelTask> d_potrf, d_trsm;
-    DBuf<GemmTask> d_gemm;
-    DBuf<SolveTask> d_fgemv, d_bgemv;
-    DevSym sym{};
-    // profiling
-    bool prof = false;
-    struct KStat { double ms = 0, work = 0; int64_t launches = 0; } kstat[HELP_K_COUNT];
-    struct Pending { int32_t kind; double work; hipEvent_t a, b; };
-    std::vector<hipEvent_t> evpool;
-    std::vector<Pending> pending;
-    double work_adat = 0, work_rhs = 0, work_backsub = 0;
-
-    ~help_gpu() {
-        if (st) { (void)hipStreamSynchronize(st); }
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        for (auto& e : evpool) (void)hipEventDestroy(e);
-        if (st) (void)hipStreamDestroy(st);
-    }
-};
+using help::g_create_error;
+using help::now_s;
+using help::kKindName;
 
 namespace {
 
@@ -233,6 +125,12 @@ void build_schedules(help_gpu* h) {
     const Supernodal& S = h->S;
     h->sbig.assign(S.nsuper, 0);
     auto ncol = [&](int32_t s) { return S.sfirst[s + 1] - S.sfirst[s]; };
+    // diagonal-block inverses: 64x64 blocks, the last one nb x nb
+    h->dinvoff.assign(S.nsuper + 1, 0);
+    for (int32_t s = 0; s < S.nsuper; ++s) {
+        int64_t nc = ncol(s), full = nc / kTile, rem = nc % kTile;
+        h->dinvoff[s + 1] = h->dinvoff[s] + full * kTile * kTile + rem * rem;
+    }
     // Cholesky flops of a supernode's columns: sum over its columns of colcount^2
     auto sn_flops = [&](int32_t s) {
         double f = 0;
@@ -252,14 +150,21 @@ void build_schedules(help_gpu* h) {
         std::vector<int32_t> large;
         int32_t off = (int32_t)h->list_small.size();
         double w = 0;
+        int32_t lds = 0;
         for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
             int32_t s = S.levelsn[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-            if (h->opts.front_path == 0 && is_small_front(c, nc)) { h->list_small.push_back(s); w += sn_flops(s); }
-            else large.push_back(s);
+            if (h->opts.front_path == 0 && is_small_front(c, nc)) {
+                h->list_small.push_back(s);
+                w += sn_flops(s);
+                int32_t nbm = std::min(kTile, nc);
+                lds = std::max(lds, c * nc + nbm * (nbm + 1));
+            } else {
+                large.push_back(s);
+            }
         }
         if ((int32_t)h->list_small.size() > off)
-            h->fops.push_back({OP_SMALL, off, (int32_t)h->list_small.size() - off, w});
+            h->fops.push_back({OP_SMALL, off, (int32_t)h->list_small.size() - off, w, lds});
         if (large.empty()) continue;
         // extend-add by 32-column chunks of each front
         off = (int32_t)h->ea.size();
@@ -346,7 +251,7 @@ void build_schedules(help_gpu* h) {
             w += tri_bytes(s, !h->sbig[s]);
             if (h->sbig[s]) {
                 wg += wsz * (double)(c - nc) * nc;
-                for (int32_t b = 0; b * 64 < nc; ++b) h->bgemv.push_back({s, b});
+                for (int32_t b = 0; b * 16 < nc; ++b) h->bgemv.push_back({s, b});
             }
         }
         if ((int32_t)h->bgemv.size() > to) h->sops_bwd.push_back({OP_BWD_GEMV, to, (int32_t)h->bgemv.size() - to, wg});
@@ -390,6 +295,7 @@ void setup_device(help_gpu* h) {
     h->d_relmap.upload(S.relmap, st);
     h->d_childptr.upload(S.childptr, st);
     h->d_child.upload(S.child, st);
+    h->d_dinvoff.upload(h->dinvoff, st);
     h->d_list_small.upload(h->list_small, st);
     h->d_list_fwd.upload(h->list_fwd, st);
     h->d_list_bwd.upload(h->list_bwd, st);
@@ -404,29 +310,40 @@ void setup_device(help_gpu* h) {
     const int32_t m = h->m, n = h->n;
     h->d_d.alloc(n); h->d_dinv.alloc(n); h->d_b1.alloc(n); h->d_x1.alloc(n); h->d_tn.alloc(n);
     h->d_b2.alloc(m); h->d_x2.alloc(m); h->d_rp.alloc(m); h->d_t.alloc(m); h->d_r.alloc(m); h->d_c.alloc(m);
+    h->d_z.alloc(m); h->d_p.alloc(m); h->d_q.alloc(m);
+    h->d_part.alloc(kDotPartials); h->d_scal.alloc(8);
     h->d_W.alloc(std::max<int64_t>(1, S.reloff[S.nsuper]));
     int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
-    if (h->use_fp32) { h->d_L32.alloc(std::max<int64_t>(1, nl)); h->d_U32.alloc(std::max<int64_t>(1, nu)); }
-    else { h->d_L64.alloc(std::max<int64_t>(1, nl)); h->d_U64.alloc(std::max<int64_t>(1, nu)); }
+    int64_t nd = h->dinvoff[S.nsuper];
+    if (h->use_fp32) {
+        h->d_L32.alloc(std::max<int64_t>(1, nl));
+        h->d_U32.alloc(std::max<int64_t>(1, nu));
+        h->d_Dinv32.alloc(std::max<int64_t>(1, nd));
+    } else {
+        h->d_L64.alloc(std::max<int64_t>(1, nl));
+        h->d_U64.alloc(std::max<int64_t>(1, nu));
+        h->d_Dinv64.alloc(std::max<int64_t>(1, nd));
+    }
     DevSym& y = h->sym;
     y.nsuper = S.nsuper;
     y.sfirst = h->d_sfirst.p; y.snrow = h->d_snrow.p; y.loff = h->d_loff.p; y.uoff = h->d_uoff.p;
     y.rowoff = h->d_rowoff.p; y.rowidx = h->d_rowidx.p; y.reloff = h->d_reloff.p;
     y.relmap = h->d_relmap.p; y.childptr = h->d_childptr.p; y.child = h->d_child.p;
+    y.dinvoff = h->d_dinvoff.p;
     HIPCHK(hipStreamSynchronize(st));
     h->device = true;
 }
 
 template <typename T>
-void run_factor(help_gpu* h, T* L, T* U) {
+void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
     hipStream_t st = h->st;
     for (const Op& o : h->fops) {
         ProfScope ps(h, o.kind, o.work);
         switch (o.kind) {
-            case OP_SMALL: launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, h->d_err.p, st); break;
+            case OP_SMALL: launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st); break;
             case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->sym, L, U, st); break;
-            case OP_POTRF: launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, h->d_err.p, st); break;
-            case OP_TRSM: launch_trsm_rows<T>(h->d_trsm.p + o.off, o.count, h->sym, L, st); break;
+            case OP_POTRF: launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, Dinv, h->d_err.p, st); break;
+            case OP_TRSM: launch_trsm_rows<T>(h->d_trsm.p + o.off, o.count, h->sym, L, Dinv, st); break;
             case OP_GEMM: launch_gemm<T>(h->d_gemm.p + o.off, o.count, h->sym, L, U, st); break;
             default: break;
         }
@@ -436,19 +353,78 @@ void run_factor(help_gpu* h, T* L, T* U) {
 
 // x (permuted numbering, FP64) <- (L L^T)^-1 x
 template <typename T>
-void run_trisolve(help_gpu* h, const T* L, double* x) {
+void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x) {
     hipStream_t st = h->st;
     for (const Op& o : h->sops_fwd) {
         ProfScope ps(h, o.kind, o.work);
-        if (o.kind == OP_FWD) launch_fwd<T>(h->d_list_fwd.p + o.off, o.count, h->sym, h->d_big.p, L, x, h->d_W.p, st);
+        if (o.kind == OP_FWD) launch_fwd<T>(h->d_list_fwd.p + o.off, o.count, h->sym, h->d_big.p, L, Dinv, x, h->d_W.p, st);
         else launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, x, h->d_W.p, st);
     }
     for (const Op& o : h->sops_bwd) {
         ProfScope ps(h, o.kind, o.work);
-        if (o.kind == OP_BWD) launch_bwd<T>(h->d_list_bwd.p + o.off, o.count, h->sym, h->d_big.p, L, x, h->d_t.p, st);
+        if (o.kind == OP_BWD) launch_bwd<T>(h->d_list_bwd.p + o.off, o.count, h->sym, h->d_big.p, L, Dinv, x, h->d_t.p, st);
         else launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, st);
     }
     HIPCHK(hipGetLastError());
+}
+
+double dot_dev(help_gpu* h, const double* a, const double* b, int64_t n) {
+    launch_dot(a, b, n, h->d_part.p, h->d_scal.p, h->st);
+    double v = 0;
+    HIPCHK(hipMemcpyAsync(&v, h->d_scal.p, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return v;
+}
+
+// z = P (L L^T)^-1 P^T r with the FP32 factor (vectors FP64)
+void apply_fp32_factor(help_gpu* h, const double* r, double* z) {
+    launch_scatter_perm(h->m, r, h->d_iperm.p, h->d_rp.p, h->st);
+    run_trisolve<float>(h, h->d_L32.p, h->d_Dinv32.p, h->d_rp.p);
+    launch_gather_perm(h->m, h->d_rp.p, h->d_iperm.p, z, h->st);
+}
+
+// Preconditioned CG in FP64 on M = A D^-1 A^T + beta I with the FP32
+// factor as preconditioner.  On entry d_rp holds the permuted right-hand
+// side; on exit d_x2 holds the solution (original order).  Stops at
+// ||r|| <= refine_tol ||rhs|| or refine_max iterations; returns iterations.
+int pcg_fp32_factor(help_gpu* h, double& rel) {
+    hipStream_t st = h->st;
+    const int32_t m = h->m, n = h->n;
+    double *rhs = h->d_c.p, *r = h->d_r.p, *x = h->d_x2.p, *z = h->d_z.p, *p = h->d_p.p, *q = h->d_q.p;
+    launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, rhs, st);
+    run_trisolve<float>(h, h->d_L32.p, h->d_Dinv32.p, h->d_rp.p);
+    launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, x, st);
+    launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
+    auto matvec = [&](const double* v, double* out) {
+        launch_normal_matvec(m, n, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_colptr.p,
+                             h->d_rowind.p, h->d_val.p, h->d_dinv.p, h->opts.beta, v, h->d_tn.p, out, st);
+    };
+    matvec(x, q);
+    launch_sub(m, rhs, q, r, st);
+    const double rr0 = dot_dev(h, rhs, rhs, m);
+    if (rr0 == 0.0) { rel = 0; return 0; }
+    apply_fp32_factor(h, r, z);
+    HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+    double rz = dot_dev(h, r, z, m);
+    int it = 0;
+    for (;;) {
+        double rr = dot_dev(h, r, r, m);
+        rel = std::sqrt(rr / rr0);
+        if (rel <= h->opts.refine_tol || it >= h->opts.refine_max || !(rz > 0)) break;
+        ++it;
+        matvec(p, q);
+        double pq = dot_dev(h, p, q, m);
+        if (!(pq > 0)) break;
+        double alpha = rz / pq;
+        launch_axpy(m, alpha, p, x, st);
+        launch_axpy(m, -alpha, q, r, st);
+        apply_fp32_factor(h, r, z);
+        double rz_new = dot_dev(h, r, z, m);
+        double beta = rz_new / rz;
+        rz = rz_new;
+        launch_xpby(m, z, beta, p, st);
+    }
+    return it;
 }
 
 float elapsed_ms(hipEvent_t a, hipEvent_t b) {
@@ -555,7 +531,7 @@ int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* rs) {
     try {
         hipStream_t st = h->st;
         const int32_t m = h->m, n = h->n;
-        HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * n, hipMemcpyDefault, st));
         HIPCHK(hipEventRecord(h->ev[0], st));
         launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
         HIPCHK(hipMemsetAsync(h->d_err.p, 0, sizeof(int32_t), st));
@@ -582,8 +558,8 @@ int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* rs) {
             launch_add_beta(h->d_diagpos.p, m, h->opts.beta, h->d_L64.p, nullptr, st);
         }
         HIPCHK(hipEventRecord(h->ev[1], st));
-        if (h->use_fp32) run_factor<float>(h, h->d_L32.p, h->d_U32.p);
-        else run_factor<double>(h, h->d_L64.p, h->d_U64.p);
+        if (h->use_fp32) run_factor<float>(h, h->d_L32.p, h->d_U32.p, h->d_Dinv32.p);
+        else run_factor<double>(h, h->d_L64.p, h->d_U64.p, h->d_Dinv64.p);
         HIPCHK(hipEventRecord(h->ev[2], st));
         int32_t nerr = 0;
         HIPCHK(hipMemcpyAsync(&nerr, h->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -611,9 +587,9 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
         hipStream_t st = h->st;
         const int32_t m = h->m, n = h->n;
         help_solve_stats loc{};
-        HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * n, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(h->d_b1.p, b1, sizeof(double) * n, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(h->d_b2.p, b2, sizeof(double) * m, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * n, hipMemcpyDefault, st));
+        HIPCHK(hipMemcpyAsync(h->d_b1.p, b1, sizeof(double) * n, hipMemcpyDefault, st));
+        HIPCHK(hipMemcpyAsync(h->d_b2.p, b2, sizeof(double) * m, hipMemcpyDefault, st));
         HIPCHK(hipEventRecord(h->ev[0], st));
         // r = A (b1 ./ d) - b2, permuted
         {
@@ -623,42 +599,13 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
         }
         HIPCHK(hipEventRecord(h->ev[1], st));
         if (!h->use_fp32) {
-            run_trisolve<double>(h, h->d_L64.p, h->d_rp.p);
+            run_trisolve<double>(h, h->d_L64.p, h->d_Dinv64.p, h->d_rp.p);
             launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, h->d_x2.p, st);
         } else {
-            // FP32 factor + FP64 iterative refinement on (A D^-1 A^T + beta I)
-            launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, h->d_r.p, st);   // rhs, original order
-            std::vector<double> rhs(m), res(m), x(m);
-            HIPCHK(hipMemcpyAsync(rhs.data(), h->d_r.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-            run_trisolve<float>(h, h->d_L32.p, h->d_rp.p);
-            launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, h->d_x2.p, st);
-            launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
-            HIPCHK(hipStreamSynchronize(st));
-            double rn0 = 0;
-            for (double v : rhs) rn0 += v * v;
-            rn0 = std::sqrt(rn0);
-            int it = 0;
+            // FP32 factor as the preconditioner of FP64 conjugate gradients on
+            // (A D^-1 A^T + beta I), started from the FP32-factor solution
             double rel = 0;
-            for (;; ++it) {
-                launch_normal_matvec(m, n, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_colptr.p,
-                                     h->d_rowind.p, h->d_val.p, h->d_dinv.p, h->opts.beta, h->d_x2.p,
-                                     h->d_tn.p, h->d_c.p, st);
-                HIPCHK(hipMemcpyAsync(res.data(), h->d_c.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-                HIPCHK(hipStreamSynchronize(st));
-                double rn = 0;
-                for (int32_t i = 0; i < m; ++i) { res[i] = rhs[i] - res[i]; rn += res[i] * res[i]; }
-                rel = rn0 > 0 ? std::sqrt(rn) / rn0 : 0.0;
-                if (rel <= h->opts.refine_tol || it >= h->opts.refine_max) break;
-                HIPCHK(hipMemcpyAsync(h->d_r.p, res.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
-                launch_scatter_perm(m, h->d_r.p, h->d_iperm.p, h->d_rp.p, st);
-                run_trisolve<float>(h, h->d_L32.p, h->d_rp.p);
-                launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, h->d_c.p, st);
-                HIPCHK(hipMemcpyAsync(x.data(), h->d_x2.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-                HIPCHK(hipMemcpyAsync(res.data(), h->d_c.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-                HIPCHK(hipStreamSynchronize(st));
-                for (int32_t i = 0; i < m; ++i) x[i] += res[i];
-                HIPCHK(hipMemcpyAsync(h->d_x2.p, x.data(), sizeof(double) * m, hipMemcpyHostToDevice, st));
-            }
+            int it = pcg_fp32_factor(h, rel);
             loc.refine_steps = it;
             loc.rel_residual = rel;
         }
@@ -670,8 +617,8 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
         }
         HIPCHK(hipEventRecord(h->ev[3], st));
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(x1, h->d_x1.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(x2, h->d_x2.p, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(x1, h->d_x1.p, sizeof(double) * n, hipMemcpyDefault, st));
+        HIPCHK(hipMemcpyAsync(x2, h->d_x2.p, sizeof(double) * m, hipMemcpyDefault, st));
         HIPCHK(hipStreamSynchronize(st));
         if (h->prof) collect_prof(h);
         loc.ms_rhs = elapsed_ms(h->ev[0], h->ev[1]);

@@ -19,6 +19,7 @@ struct DevSym {
     const int32_t* relmap;
     const int32_t* childptr;  // nsuper+1
     const int32_t* child;
+    const int64_t* dinvoff;   // nsuper+1 offsets into the diagonal-block inverses
 };
 
 // C -= A B^T task on one 64x64 tile of a front.  Front coordinates: rows and
@@ -48,7 +49,8 @@ struct EaTask {
 
 constexpr int kTile = 64;      // GEMM tile and panel width
 constexpr int kSmallRows = 256;
-constexpr int kSmallElems = 7680;  // panel elements resident in LDS for a small front (60 KB FP64)
+constexpr int kSmallElems = 4032;  // panel elements of a small front; with the 64x65
+                                   // inverse scratch at most 8192 (64 KB FP64) in LDS
 
 // ---- sparse kernels (kernels_sparse.hip) ----
 void launch_dinv(const double* d, double* dinv, int32_t n, hipStream_t st);
@@ -72,6 +74,13 @@ void launch_gather_perm(int32_t m, const double* xp, const int32_t* iperm, doubl
                         hipStream_t st);
 void launch_scatter_perm(int32_t m, const double* x, const int32_t* iperm, double* xp,
                          hipStream_t st);
+// FP64 vector kernels (deterministic dot: fixed grid of partials + one block)
+constexpr int kDotPartials = 512;
+void launch_dot(const double* a, const double* b, int64_t n, double* part /* kDotPartials */,
+                double* out, hipStream_t st);
+void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t st);   // y += a x
+void launch_xpby(int64_t n, const double* x, double beta, double* y, hipStream_t st);    // y = x + b y
+void launch_sub(int64_t n, const double* x, const double* y, double* z, hipStream_t st); // z = x - y
 // y = (A D^-1 A^T + beta I) x in original numbering (refinement operator)
 void launch_normal_matvec(int32_t m, int32_t n, const int64_t* rowptr, const int32_t* colind,
                           const double* csrval, const int64_t* colptr, const int32_t* rowind,
@@ -79,17 +88,22 @@ void launch_normal_matvec(int32_t m, int32_t n, const int64_t* rowptr, const int
                           double* tmp_n, double* y, hipStream_t st);
 
 // ---- supernodal factorization (kernels_factor.hip) ----
+// lds_elems: dynamic LDS per workgroup in elements of T, at least
+// max over the fronts of c*n + min(64,n)*(min(64,n)+1)
 template <typename T>
-void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U,
-                         int32_t* err, hipStream_t st);
+void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, T* Dinv,
+                         int32_t* err, int32_t lds_elems, hipStream_t st);
 template <typename T>
 void launch_extend_add(const EaTask* tasks, int32_t count, DevSym sym, T* L, T* U,
                        hipStream_t st);
+// factor the 64x64 diagonal block at column b and store its inverse
 template <typename T>
-void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, int32_t* err,
-                       hipStream_t st);
+void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv,
+                       int32_t* err, hipStream_t st);
+// panel rows below the diagonal block times the stored inverse (MFMA)
 template <typename T>
-void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, hipStream_t st);
+void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, const T* Dinv,
+                      hipStream_t st);
 template <typename T>
 void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st);
 template <typename T>
@@ -107,18 +121,18 @@ struct SolveTask {
 // vectors and solve with L11; small supernodes also apply L21 (big[s] == 0).
 template <typename T>
 void launch_fwd(const int32_t* list, int32_t count, DevSym sym, const int8_t* big, const T* L,
-                double* x, double* W, hipStream_t st);
+                const T* Dinv, double* x, double* W, hipStream_t st);
 // Forward, big supernodes: W_s += L21 x_s by 64-row blocks.
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
                      const double* x, double* W, hipStream_t st);
-// Backward, big supernodes: t[f+j] = sum_{i>=n} L21(i,j) x[row_i] by 64-column blocks.
+// Backward, big supernodes: t[f+j] = sum_{i>=n} L21(i,j) x[row_i] by 16-column blocks.
 template <typename T>
 void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
                      const double* x, double* t, hipStream_t st);
 // Backward: x_s = L11^-T (x_s - L21^T x_rows) (big supernodes read t).
 template <typename T>
 void launch_bwd(const int32_t* list, int32_t count, DevSym sym, const int8_t* big, const T* L,
-                double* x, const double* t, hipStream_t st);
+                const T* Dinv, double* x, const double* t, hipStream_t st);
 
 }  // namespace help

@@ -1,0 +1,146 @@
+// Private definitions shared by the backend translation units (the handle,
+// device buffers, error plumbing).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "help_mi355x.h"
+#include "symbolic.h"
+
+namespace help {
+// last help_gpu_create error (no handle to hold it)
+extern thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(const std::string& s, int c) : std::runtime_error(s), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_), HELP_ERR_HIP);  \
+    } while (0)
+
+template <typename T>
+struct DBuf {   // owning device buffer
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t cnt) {
+        free();
+        n = cnt;
+        if (cnt) HIPCHK(hipMalloc((void**)&p, cnt * sizeof(T)));
+    }
+    void upload(const std::vector<T>& v, hipStream_t st) {
+        alloc(v.size());
+        if (n) HIPCHK(hipMemcpyAsync(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { free(); }
+};
+
+// kernel classes (profiling index = HELP_K_* in the public header)
+enum OpKind : int32_t {
+    OP_SMALL = HELP_K_SMALL_FRONTS, OP_EA = HELP_K_EXTEND_ADD, OP_POTRF = HELP_K_POTRF,
+    OP_TRSM = HELP_K_TRSM, OP_GEMM = HELP_K_GEMM,
+    OP_FWD = HELP_K_FWD, OP_FWD_GEMV = HELP_K_FWD_GEMV, OP_BWD_GEMV = HELP_K_BWD_GEMV,
+    OP_BWD = HELP_K_BWD
+};
+struct Op {
+    OpKind kind;
+    int32_t off, count;
+    double work;       // algorithmic flops (factor classes) or bytes (the others)
+    int32_t lds = 0;   // dynamic LDS elements per workgroup (small fronts)
+};
+
+inline const char* const kKindName[HELP_K_COUNT] = {
+    "k_small_fronts", "k_extend_add", "k_potrf_diag", "k_trsm_rows", "k_gemm",
+    "k_fwd", "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub"};
+
+inline double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace help
+
+using help::AdatPattern;
+using help::CscMatrix;
+using help::CsrMatrix;
+using help::DBuf;
+using help::DevSym;
+using help::EaTask;
+using help::GemmTask;
+using help::HipError;
+using help::Op;
+using help::PanelTask;
+using help::SolveTask;
+using help::Supernodal;
+
+struct help_gpu {
+    int32_t m = 0, n = 0;
+    int64_t nnz = 0;
+    help_opts opts{};
+    CscMatrix A;
+    CsrMatrix R;
+    AdatPattern P;
+    Supernodal S;
+    std::vector<int64_t> slot_dest, diagpos;
+    std::vector<int32_t> colof, csrcol;
+    std::vector<double> csrval;
+    std::vector<int8_t> sbig;
+    // schedules
+    std::vector<Op> fops, sops_fwd, sops_bwd;
+    std::vector<int32_t> list_small, list_fwd, list_bwd;
+    std::vector<EaTask> ea;
+    std::vector<PanelTask> potrf, trsm;
+    std::vector<GemmTask> gemm;
+    std::vector<SolveTask> fgemv, bgemv;
+    std::string err;
+    bool device = false;
+    bool factored = false;
+    bool use_fp32 = false;
+    int32_t last_tiny = 0;
+    // device state
+    hipStream_t st = nullptr;
+    hipEvent_t ev[6] = {};
+    std::vector<int64_t> dinvoff;
+    DBuf<int64_t> d_colptr, d_rowptr, d_pairptr, d_dest, d_diagpos, d_loff, d_uoff, d_rowoff, d_reloff,
+        d_dinvoff;
+    DBuf<int32_t> d_rowind, d_colind, d_pair_p, d_pair_q, d_colof, d_iperm, d_sfirst, d_snrow,
+        d_rowidx, d_relmap, d_childptr, d_child, d_list_small, d_list_fwd, d_list_bwd, d_err;
+    DBuf<double> d_val, d_csrval, d_d, d_dinv, d_b1, d_b2, d_x1, d_x2, d_rp, d_W, d_t, d_tn, d_r,
+        d_c, d_z, d_p, d_q, d_part, d_scal, d_slotvals;
+    DBuf<double> d_L64, d_U64, d_Dinv64;
+    DBuf<float> d_L32, d_U32, d_Dinv32;
+    DBuf<int8_t> d_big;
+    DBuf<EaTask> d_ea;
+    DBuf<PanelTask> d_potrf, d_trsm;
+    DBuf<GemmTask> d_gemm;
+    DBuf<SolveTask> d_fgemv, d_bgemv;
+    DevSym sym{};
+    // profiling
+    bool prof = false;
+    struct KStat { double ms = 0, work = 0; int64_t launches = 0; } kstat[HELP_K_COUNT];
+    struct Pending { int32_t kind; double work; hipEvent_t a, b; };
+    std::vector<hipEvent_t> evpool;
+    std::vector<Pending> pending;
+    double work_adat = 0, work_rhs = 0, work_backsub = 0;
+
+    ~help_gpu() {
+        if (st) { (void)hipStreamSynchronize(st); }
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : evpool) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};

@@ -6,6 +6,12 @@
 //
 // Column layout (Presolve.cpp:1101-1170): x[0, n_ub) have lower and upper
 // bounds, x[n_ub, n_lb) a lower bound only, x[n_lb, n_lb + n_fr) are free.
+//
+// Two summation modes: the exact mode sums every reduction sequentially as
+// the reference does (bit-identical iterates); the default mode splits long
+// sums into 64 fixed chunks summed in parallel and then in chunk order, so
+// its results are deterministic and independent of the thread count.  The
+// ratio test's minimum is order independent in both.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -15,6 +21,30 @@
 #include "help_mi355x.h"
 
 namespace {
+
+bool g_exact = true;          // summation mode of the current help_hsd_solve call
+constexpr int kChunks = 64;
+constexpr int kParMin = 65536; // shorter sums stay sequential in both modes
+
+template <class F>
+double rsum(int n, F f) {
+    if (g_exact || n < kParMin) {
+        double s = 0;
+        for (int i = 0; i < n; ++i) s += f(i);
+        return s;
+    }
+    double part[kChunks];
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < kChunks; ++k) {
+        int lo = (int)((int64_t)n * k / kChunks), hi = (int)((int64_t)n * (k + 1) / kChunks);
+        double s = 0;
+        for (int i = lo; i < hi; ++i) s += f(i);
+        part[k] = s;
+    }
+    double s = 0;
+    for (int k = 0; k < kChunks; ++k) s += part[k];
+    return s;
+}
 
 // LP.h:81-96
 constexpr double kStepGamma = 0.99995;
@@ -31,9 +61,7 @@ double now_s() {
 }
 
 double dot(int n, const double* a, const double* b) {   // Helper.cpp:48-54
-    double r = 0;
-    for (int i = 0; i < n; ++i) r += a[i] * b[i];
-    return r;
+    return rsum(n, [&](int i) { return a[i] * b[i]; });
 }
 
 struct Hsd {
@@ -57,12 +85,38 @@ struct Hsd {
     std::vector<double> dx, dxu, dy, dyu, dz, dzu, dx_b, dxu_b, dy_b, dyu_b, dz_b, dzu_b;
     double dtau = 0, dkappa = 0;
     int backend_error = 0;
-    double t_renew = 0, t_solve = 0;
+    double t_renew = 0, t_solve = 0, t_newton = 0, t_direction = 0, t_step = 0;
+    struct Timer {   // adds the scope's wall time to *acc
+        double* acc;
+        double t0;
+        explicit Timer(double* a) : acc(a), t0(now_s()) {}
+        ~Timer() { *acc += now_s() - t0; }
+    };
     int n_renew = 0, n_solve = 0;
+
+    // A by rows (columns ascending), for the row-parallel forms of the
+    // reference's column-order scatters: each row receives its terms in the
+    // same order, so the results are bit-identical.
+    std::vector<int64_t> rptr;
+    std::vector<int32_t> rcol;
+    std::vector<double> rval;
 
     explicit Hsd(const help_lp* l, const help_backend* b_) : lp(l), be(b_) {
         m = l->m; n = l->n; n_lb = l->n_lb; n_ub = l->n_ub; n_fr = l->n_fr;
         c = l->c; b = l->b; u = l->ub;
+        const int64_t nnz = l->colptr[n];
+        rptr.assign(m + 1, 0);
+        for (int64_t p = 0; p < nnz; ++p) rptr[l->rowind[p] + 1]++;
+        for (int i = 0; i < m; ++i) rptr[i + 1] += rptr[i];
+        rcol.resize(nnz);
+        rval.resize(nnz);
+        std::vector<int64_t> fill(rptr.begin(), rptr.end() - 1);
+        for (int j = 0; j < n; ++j)
+            for (int64_t p = l->colptr[j]; p < l->colptr[j + 1]; ++p) {
+                int64_t t = fill[l->rowind[p]]++;
+                rcol[t] = j;
+                rval[t] = l->val[p];
+            }
         auto z0 = [](std::vector<double>& v, int k) { v.assign(std::max(k, 1), 0.0); };
         z0(x, n); z0(y, m); z0(z, n); z0(xu, n_ub); z0(yu, n_ub); z0(zu, n_ub); z0(xf, n_fr);
         z0(D, n); z0(D_u, n_ub); z0(r_p, m); z0(r_d, n);
@@ -76,17 +130,21 @@ struct Hsd {
         z0(dx_b, n); z0(dxu_b, n_ub); z0(dy_b, m); z0(dyu_b, n_ub); z0(dz_b, n); z0(dzu_b, n_ub);
     }
 
-    // Helper.cpp:110-155 (SetATimesVector)
+    // Helper.cpp:110-155 (SetATimesVector); dest_i receives its terms in
+    // ascending column order, exactly as the reference's column scatter does
     void a_times(int sign, const double* v, double* dest) const {
-        for (int j = 0; j < n; ++j) {
-            double vj = v[j];
-            for (int64_t p = lp->colptr[j]; p < lp->colptr[j + 1]; ++p) {
-                if (sign == 1) dest[lp->rowind[p]] += lp->val[p] * vj;
-                else dest[lp->rowind[p]] -= lp->val[p] * vj;
+#pragma omp parallel for schedule(static, 4096)
+        for (int i = 0; i < m; ++i) {
+            double acc = dest[i];
+            for (int64_t t = rptr[i]; t < rptr[i + 1]; ++t) {
+                if (sign == 1) acc += rval[t] * v[rcol[t]];
+                else acc -= rval[t] * v[rcol[t]];
             }
+            dest[i] = acc;
         }
     }
     void at_times(int sign, const double* v, double* dest) const {
+#pragma omp parallel for schedule(static, 4096)
         for (int j = 0; j < n; ++j) {
             double dj = dest[j];
             for (int64_t p = lp->colptr[j]; p < lp->colptr[j + 1]; ++p) {
@@ -119,8 +177,40 @@ struct Hsd {
     }
 
     // HSDSolver.cpp:53-150
+    // min and sum of (a_i + sa*da_i) * (b_i + sb*db_i) over i < k, chunked
+    void prod_min_sum(int k, const double* a, double sa, const double* da, const double* b, double sb,
+                      const double* db, double& mn, double& sm) const {
+        double pmin[kChunks], psum[kChunks];
+#pragma omp parallel for schedule(static)
+        for (int q = 0; q < kChunks; ++q) {
+            int lo = (int)((int64_t)k * q / kChunks), hi = (int)((int64_t)k * (q + 1) / kChunks);
+            double mloc = HUGE_VAL, sloc = 0;
+            for (int i = lo; i < hi; ++i) {
+                double t = da ? (a[i] + sa * da[i]) * (b[i] + sb * db[i]) : a[i] * b[i];
+                mloc = std::min(mloc, t);
+                sloc += t;
+            }
+            pmin[q] = mloc;
+            psum[q] = sloc;
+        }
+        double s = 0;
+        for (int q = 0; q < kChunks; ++q) { mn = std::min(mn, pmin[q]); s += psum[q]; }
+        sm += s;
+    }
+
     void newton_parameters() {
+        Timer tm(&t_newton);
         double xz_min = tau * kappa, xz_sum = xz_min;
+        if (!g_exact && n_lb >= kParMin) {
+            bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad)
+            for (int i = 0; i < n_lb; ++i) {
+                D[i] = z[i] / x[i];
+                if (D[i] <= 0.0) bad = true;
+            }
+            if (bad) { status = HELP_HSD_STALLED; return; }
+            prod_min_sum(n_lb, z.data(), 0, nullptr, x.data(), 0, nullptr, xz_min, xz_sum);
+        } else
         for (int i = 0; i < n_lb; ++i) {
             D[i] = z[i] / x[i];
             double t = z[i] * x[i];
@@ -209,27 +299,46 @@ struct Hsd {
     int direction(double eta, double DOT, double* hrxz, double* hrxuzu, double hrtk, double* odx,
                   double* odxu, double& odtau, double* ody, double* odyu, double* odz, double* odzu,
                   double& odkappa, double& rpn, double& rdn, double& rgn) {
+        // wall time of the direction minus the backend solve inside it
+        struct SolveSub {
+            double* dir;
+            double* solve;
+            double s0;
+            ~SolveSub() { *dir -= *solve - s0; }
+        };
+        Timer tm(&t_direction);
+        SolveSub sub{&t_direction, &t_solve, t_solve};
         scaled(n, eta, r_d.data(), dir1.data());
         scaled(m, eta, r_p.data(), dir2.data());
         for (int i = 0; i < n_ub; ++i) {
             dir1u[i] = eta * (zu[i] + yu[i]);
             dir2u[i] = eta * (tau * u[i] - x[i] - xu[i]);
         }
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n_lb; ++i) hrxz[i] /= std::max(x[i], z[i]);
         for (int i = 0; i < n_ub; ++i) hrxuzu[i] /= std::max(xu[i], zu[i]);
         for (int i = 0; i < n_lb; ++i)
             if (x[i] >= z[i]) dir1[i] += hrxz[i];
         for (int i = 0; i < n_ub; ++i)
             if (xu[i] >= zu[i]) dir1u[i] += hrxuzu[i];
-        for (int i = 0; i < n_lb; ++i)
-            if (x[i] < z[i])
-                for (int64_t p = lp->colptr[i]; p < lp->colptr[i + 1]; ++p)
-                    dir2[lp->rowind[p]] -= lp->val[p] * hrxz[i];
+        // HSDSolver.cpp:339-344 as a row gather (same per-row term order)
+#pragma omp parallel for schedule(static, 4096)
+        for (int r = 0; r < m; ++r) {
+            double acc = dir2[r];
+            for (int64_t t = rptr[r]; t < rptr[r + 1]; ++t) {
+                int j = rcol[t];
+                if (j < n_lb && x[j] < z[j]) acc -= rval[t] * hrxz[j];
+            }
+            dir2[r] = acc;
+        }
         for (int i = 0; i < n_ub; ++i) {
             if (x[i] < z[i]) dir2u[i] -= hrxz[i];
             if (xu[i] < zu[i]) dir2u[i] -= hrxuzu[i];
         }
         double UP = eta * r_g + hrtk / tau;
+        if (!g_exact && n_lb >= kParMin)
+            UP += rsum(n_lb, [&](int i) { return x[i] < z[i] ? c[i] * hrxz[i] : 0.0; });
+        else
         for (int i = 0; i < n_lb; ++i)
             if (x[i] < z[i]) UP += c[i] * hrxz[i];
         double DOWN = D_g - DOT;
@@ -238,6 +347,7 @@ struct Hsd {
         if (DOWN <= 1e-13) odtau = 0;
         else odtau = (UP - mix_dots(uu.data(), vv.data(), vvu.data())) / std::max(D_g, DOWN);
         const double dt = odtau;
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n_lb; ++i) {
             double T = uu[i] - pp[i] * dt;
             if (x[i] >= z[i]) { odx[i] = T; odz[i] = hrxz[i] - z[i] * T / x[i]; }
@@ -256,6 +366,7 @@ struct Hsd {
         for (int i = 0; i < n_ub; ++i) odyu[i] = vvu[i] - qqu[i] * dt;
         odkappa = (hrtk - kappa * dt) / tau;
         scaled(n, eta, r_d.data(), dir1.data());
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n_lb; ++i) dir1[i] += -c[i] * dt + odz[i];
         double rt = rho / tau;
         for (int i = 0; i < n_fr; ++i)
@@ -281,6 +392,12 @@ struct Hsd {
     // HSDSolver.cpp:478-488
     double stepsize(const double* v, const double* vu, const double* dv, const double* dvu) const {
         double r = 1.0;
+        if (!g_exact && n_lb >= kParMin) {
+            // the sequential test keeps the smallest ratio -v/dv below 1
+#pragma omp parallel for schedule(static) reduction(min : r)
+            for (int i = 0; i < n_lb; ++i)
+                if (dv[i] < -kVarTol && v[i] + dv[i] < 0.0) r = std::min(r, -v[i] / dv[i]);
+        } else
         for (int i = 0; i < n_lb; ++i)
             if (dv[i] < -kVarTol && v[i] + r * dv[i] < 0.0) r = -v[i] / dv[i];
         for (int i = 0; i < n_ub; ++i)
@@ -292,6 +409,9 @@ struct Hsd {
     int check_step(double ap, double at, double ad, double ak, double beta) const {
         double xz_min = (tau + at * dtau) * (kappa + ak * dkappa);
         double xz_sum = xz_min;
+        if (!g_exact && n_lb >= kParMin)
+            prod_min_sum(n_lb, x.data(), ap, dx.data(), z.data(), ad, dz.data(), xz_min, xz_sum);
+        else
         for (int i = 0; i < n_lb; ++i) {
             double t = (x[i] + ap * dx[i]) * (z[i] + ad * dz[i]);
             xz_min = std::min(xz_min, t);
@@ -307,7 +427,8 @@ struct Hsd {
     }
 
     // HSDSolver.cpp:521-535
-    void get_stepsize(double& ps, double& ds) const {
+    void get_stepsize(double& ps, double& ds) {
+        Timer tm(&t_step);
         ps = stepsize(x.data(), xu.data(), dx.data(), dxu.data());
         ds = stepsize(z.data(), zu.data(), dz.data(), dzu.data());
         if (dtau < -kVarTol) { ps = std::min(ps, -tau / dtau); ds = std::min(ds, -tau / dtau); }
@@ -316,6 +437,7 @@ struct Hsd {
 
     // HSDSolver.cpp:537-594
     void update_step(double p_step, double d_step) {
+        Timer tm(&t_step);
         double tau_step, kappa_step;
         int PGD = (p_step * dtau) > (d_step * dtau);
         if (PGD) { tau_step = d_step; kappa_step = p_step; }
@@ -329,6 +451,7 @@ struct Hsd {
         double pf, df;
         if (PGD) { pf = tau_d / tau_p; df = 1; }
         else { pf = 1; df = tau_p / tau_d; }
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n_lb; ++i) {
             x[i] = (x[i] + p_step * dx[i]) * pf;
             z[i] = (z[i] + d_step * dz[i]) * df;
@@ -433,6 +556,10 @@ struct Hsd {
             if (newton_ok) {
                 get_stepsize(p_step, d_step);
                 double new_mu = (tau + p_step * dtau) * (kappa + d_step * dkappa);
+                if (!g_exact && n_lb >= kParMin) {
+                    double mn = HUGE_VAL;
+                    prod_min_sum(n_lb, x.data(), p_step, dx.data(), z.data(), d_step, dz.data(), mn, new_mu);
+                } else
                 for (int i = 0; i < n_lb; ++i) new_mu += (x[i] + p_step * dx[i]) * (z[i] + d_step * dz[i]);
                 for (int i = 0; i < n_ub; ++i) new_mu += (xu[i] + p_step * dxu[i]) * (zu[i] + d_step * dzu[i]);
                 new_mu /= 1.0 + n_lb + n_ub;
@@ -447,6 +574,7 @@ struct Hsd {
             eta = 1.0 - gamma;
             double alpha_p = gamma * mu;
             double dmu = std::max((dot(n_lb, dx.data(), dz.data()) + dot(n_ub, dxu.data(), dzu.data()) + dtau * dkappa) / (1.0 + n_lb + n_ub), 0.0);
+#pragma omp parallel for schedule(static)
             for (int i = 0; i < n_lb; ++i) hxz[i] = -x[i] * z[i] + alpha_p - phi * (dx[i] * dz[i] - dmu);
             for (int i = 0; i < n_fr; ++i) hxz[i + n_lb] = 0;
             for (int i = 0; i < n_ub; ++i) hxuzu[i] = -xu[i] * zu[i] + alpha_p - phi * (dxu[i] * dzu[i] - dmu);
@@ -472,15 +600,17 @@ struct Hsd {
 }  // namespace
 
 extern "C" int help_hsd_solve(const help_lp* lp, const help_backend* be, int32_t max_iter,
-                              help_hsd_log_fn logf, void* user, help_hsd_result* res) {
+                              int32_t flags, help_hsd_log_fn logf, void* user, help_hsd_result* res) {
     if (!lp || !be || !res || !be->construct || !be->renew || !be->solve || !be->destruct)
         return HELP_ERR_ARG;
+    g_exact = (flags & HELP_HSD_EXACT) != 0;
     if (lp->n_lb + lp->n_fr != lp->n || lp->n_ub > lp->n_lb || (lp->n_ub > 0 && !lp->ub))
         return HELP_ERR_ARG;
     std::memset(res, 0, sizeof(*res));
     if (max_iter <= 0) max_iter = 10000;   // LP.h:84 Max_Iterations
     double t0 = now_s();
     int st = be->construct(be->ctx, lp);
+    res->seconds_construct = now_s() - t0;
     if (st) {
         res->backend_error = st;
         res->seconds = now_s() - t0;
@@ -500,6 +630,9 @@ extern "C" int help_hsd_solve(const help_lp* lp, const help_backend* be, int32_t
     res->mu = h.mu;
     res->seconds_renew = h.t_renew;
     res->seconds_solve = h.t_solve;
+    res->seconds_newton = h.t_newton;
+    res->seconds_direction = h.t_direction;
+    res->seconds_step = h.t_step;
     res->n_renew = h.n_renew;
     res->n_solve = h.n_solve;
     res->backend_error = h.backend_error;

@@ -3,18 +3,21 @@
 // Storage per supernode s (ncol n, nrow c, structure rows sorted):
 //   L panel  c x n, column-major (lda = c), rows = the structure of s;
 //   U        (c-n) x (c-n), column-major lower, the update ("contribution")
-//            matrix passed to the parent by extend-add.
+//            matrix passed to the parent by extend-add;
+//   Dinv     the inverse of every 64-column diagonal block of L11 (nb x nb,
+//            column-major), used by the panel solve and the triangular solves.
 // The symbolic schedule processes the elimination tree level by level;
 // within a level every front is independent.  A parent pulls its children's
 // update matrices one child after the other, so the summation order is fixed
 // and no atomics touch the factor.
 //
-// Small fronts (c <= 256, c*n <= 8192): one workgroup per front does the
-// extend-add, the dense partial factorization in LDS and the update SYRK.
+// Small fronts (c <= 256, c*n <= 7680): one workgroup per front does the
+// extend-add, the dense partial factorization in LDS, the diagonal-block
+// inverses and the update SYRK.
 // Large fronts: extend-add by column chunks, then for each 64-column panel a
-// diagonal-block Cholesky, a row-block triangular solve and an MFMA GEMM on
-// the trailing panel columns, and finally one MFMA SYRK for U with the full
-// inner dimension n.
+// one-wave diagonal-block Cholesky + inverse, the panel rows times the
+// inverse on MFMA, an MFMA GEMM on the trailing panel columns, and finally
+// one MFMA SYRK for U with the full inner dimension n.
 #include "device.h"
 
 namespace help {
@@ -28,20 +31,57 @@ template <> struct Huge<float> { static constexpr float v = 1e30f; };
 // pivot policy: a non-positive (or NaN) pivot is replaced by a huge value, so
 // the column of L becomes ~0 (that row is dropped from the solve), and counted.
 template <typename T>
-__device__ inline T safe_sqrt_pivot(T d, int32_t* err) {
+__device__ inline T safe_sqrt_pivot(T d, int32_t* err, bool count) {
     if (!(d > T(0))) {
-        atomicAdd(err, 1);
+        if (count) atomicAdd(err, 1);
         d = Huge<T>::v;
     }
     return sqrt(d);
 }
 
+__device__ inline int64_t dinv_block_offset(const DevSym& sym, int32_t s, int32_t b) {
+    return sym.dinvoff[s] + (int64_t)(b / kTile) * kTile * kTile;
+}
+
+// X = L^-1 of the factored lower-triangular nb x nb block at Lsrc (LDS,
+// leading dimension lds) by row operations on the identity (256 threads),
+// written to Db (column-major nb x nb).  X is LDS scratch of nb*(nb+1).
+template <typename T>
+__device__ void invert_lower_block(const T* Lsrc, int32_t lds, int32_t nb, T* __restrict__ Db, T* X) {
+    const int tid = threadIdx.x;
+    const int ldx = nb + 1;
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        int j = idx / nb, i = idx - j * nb;
+        X[j * ldx + i] = (i == j) ? T(1) : T(0);
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        const T djj = Lsrc[j * lds + j];
+        if (tid <= j) X[tid * ldx + j] /= djj;      // row j, columns k <= j
+        __syncthreads();
+        // rows i > j, columns k <= j:  X[i][k] -= L[i][j] X[j][k]
+        const int cnt = (nb - j - 1) * (j + 1);
+        for (int idx = tid; idx < cnt; idx += 256) {
+            int k = idx / (nb - j - 1), i = j + 1 + idx - k * (nb - j - 1);
+            X[k * ldx + i] -= Lsrc[j * lds + i] * X[k * ldx + j];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        int j = idx / nb, i = idx - j * nb;
+        Db[j * nb + i] = (i >= j) ? X[j * ldx + i] : T(0);
+    }
+}
+
 // ------------------------------------------------------------------ small
+// Dynamic LDS: the c x n panel followed by the (nb x nb+1) inverse scratch.
 template <typename T>
 __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict__ list, DevSym sym,
                                                       T* __restrict__ L, T* __restrict__ U,
+                                                      T* __restrict__ Dinv,
                                                       int32_t* __restrict__ err) {
-    __shared__ T F[kSmallElems];
+    extern __shared__ unsigned char smem_raw[];
+    T* F = reinterpret_cast<T*>(smem_raw);
     __shared__ T piv;
     const int32_t s = list[blockIdx.x];
     const int32_t f = sym.sfirst[s];
@@ -75,7 +115,7 @@ __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict_
     // dense partial Cholesky of the panel, right-looking by columns
     for (int32_t j = 0; j < n; ++j) {
         if (tid == 0) {
-            T d = safe_sqrt_pivot<T>(F[j * c + j], err);
+            T d = safe_sqrt_pivot<T>(F[j * c + j], err, true);
             F[j * c + j] = d;
             piv = d;
         }
@@ -92,6 +132,12 @@ __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict_
         __syncthreads();
     }
     for (int32_t idx = tid; idx < cn; idx += nt) Lp[idx] = F[idx];
+    // inverses of the diagonal blocks
+    T* X = F + cn;
+    for (int32_t b = 0; b < n; b += kTile) {
+        const int32_t nb = min(kTile, n - b);
+        invert_lower_block<T>(F + (int64_t)b * c + b, c, nb, Dinv + dinv_block_offset(sym, s, b), X);
+    }
     // U -= L21 L21^T
     const int32_t tot = cu * cu;
     for (int32_t idx = tid; idx < tot; idx += nt) {
@@ -140,97 +186,80 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
     }
 }
 
-// ---------------------------------------------- diagonal block Cholesky
+// ------------------------------------ diagonal block Cholesky + inverse
+// Elimination with unit multipliers on the augmented block [A | I]: step j
+// stores m_ij = a_ij / d_j in place, updates the trailing lower part
+// (a_ik -= d_j m_ij m_kj) and applies the same row operation to the identity
+// (x_ik -= m_ij x_jk).  At the end A holds L_u (unit) and D, X holds L_u^-1,
+// so L = L_u D^1/2 and L^-1 = D^-1/2 L_u^-1.  The block is padded with the
+// identity to 64, which leaves the leading nb x nb results unchanged.
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_diag(const PanelTask* __restrict__ tasks, DevSym sym,
-                                                    T* __restrict__ L, int32_t* __restrict__ err) {
-    __shared__ T D[kTile * (kTile + 1)];
-    __shared__ T piv;
-    const PanelTask tk = tasks[blockIdx.x];
-    const int32_t s = tk.s, b = tk.b;
-    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
-    const int32_t c = sym.snrow[s];
-    const int32_t nb = min(kTile, n - b);
-    T* Lp = L + sym.loff[s];
-    const int tid = threadIdx.x, nt = blockDim.x;
+__device__ void factor_invert_block(const T* __restrict__ src, int64_t lds, int32_t nb,
+                                    T* __restrict__ Ldst, T* __restrict__ Db, T* A, T* X,
+                                    int32_t* err) {
     constexpr int ld = kTile + 1;
-    for (int32_t idx = tid; idx < nb * nb; idx += nt) {
-        int32_t j = idx / nb, i = idx - j * nb;
-        D[j * ld + i] = Lp[(int64_t)(b + j) * c + b + i];
+    const int tid = threadIdx.x;
+    const int row = tid & 63, part = tid >> 6;
+    for (int idx = tid; idx < kTile * kTile; idx += 256) {
+        int j = idx >> 6, i = idx & 63;
+        T v = (i == j) ? T(1) : T(0);
+        if (i < nb && j < nb && i >= j) v = src[(int64_t)j * lds + i];
+        A[j * ld + i] = v;
+        X[j * ld + i] = (i == j) ? T(1) : T(0);
     }
     __syncthreads();
-    for (int32_t j = 0; j < nb; ++j) {
-        if (tid == 0) {
-            T d = safe_sqrt_pivot<T>(D[j * ld + j], err);
-            D[j * ld + j] = d;
-            piv = d;
+    for (int j = 0; j < kTile; ++j) {
+        T piv = A[j * ld + j];
+        if (!(piv > T(0))) {
+            if (tid == 0 && j < nb) atomicAdd(err, 1);
+            piv = Huge<T>::v;
         }
         __syncthreads();
-        const T p = piv;
-        for (int32_t i = j + 1 + tid; i < nb; i += nt) D[j * ld + i] /= p;
+        if (part == 0) {
+            if (row > j) A[j * ld + row] /= piv;
+            else if (row == j) A[j * ld + j] = piv;
+        }
         __syncthreads();
-        const int32_t rem = nb - j - 1;
-        for (int32_t idx = tid; idx < rem * nb; idx += nt) {
-            int32_t k = j + 1 + idx / nb, i = idx % nb;
-            if (i >= k) D[k * ld + i] -= D[j * ld + i] * D[j * ld + k];
+        if (row > j) {
+            const T mi = A[j * ld + row];
+            for (int k = part; k <= row; k += 4) {
+                if (k > j) A[k * ld + row] -= piv * mi * A[j * ld + k];
+                else X[k * ld + row] -= mi * X[k * ld + j];
+            }
         }
         __syncthreads();
     }
-    for (int32_t idx = tid; idx < nb * nb; idx += nt) {
-        int32_t j = idx / nb, i = idx - j * nb;
-        if (i >= j) Lp[(int64_t)(b + j) * c + b + i] = D[j * ld + i];
+    for (int idx = tid; idx < kTile * kTile; idx += 256) {
+        int j = idx >> 6, i = idx & 63;
+        if (i >= nb || j >= nb) continue;
+        const T sdj = sqrt(A[j * ld + j]);
+        if (i >= j) Ldst[(int64_t)j * lds + i] = (i == j) ? sdj : A[j * ld + i] * sdj;
+        Db[j * nb + i] = (i >= j) ? X[j * ld + i] / sqrt(A[i * ld + i]) : T(0);
     }
 }
 
-// ----------------------------------------- row-block triangular solve
-// X := X L_bb^-T for the 64 rows of block rb-1 below the diagonal block.
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_rows(const PanelTask* __restrict__ tasks, DevSym sym,
-                                                   T* __restrict__ L) {
-    __shared__ T D[kTile * (kTile + 1)];
-    __shared__ T X[kTile * (kTile + 1)];
+__global__ __launch_bounds__(256) void k_potrf_diag(const PanelTask* __restrict__ tasks, DevSym sym,
+                                                    T* __restrict__ L, T* __restrict__ Dinv,
+                                                    int32_t* __restrict__ err) {
+    constexpr int ld = kTile + 1;
+    __shared__ T A[kTile * ld];
+    __shared__ T X[kTile * ld];
     const PanelTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s, b = tk.b;
     const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
     const int32_t c = sym.snrow[s];
     const int32_t nb = min(kTile, n - b);
-    const int32_t r0 = b + nb + (tk.rb - 1) * kTile;
-    const int32_t nr = min(kTile, c - r0);
-    T* Lp = L + sym.loff[s];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    constexpr int ld = kTile + 1;
-    for (int32_t idx = tid; idx < nb * nb; idx += nt) {
-        int32_t j = idx / nb, i = idx - j * nb;
-        D[j * ld + i] = i >= j ? Lp[(int64_t)(b + j) * c + b + i] : T(0);
-    }
-    for (int32_t idx = tid; idx < nb * nr; idx += nt) {
-        int32_t j = idx / nr, i = idx - j * nr;
-        X[j * ld + i] = Lp[(int64_t)(b + j) * c + r0 + i];
-    }
-    __syncthreads();
-    if (tid < nr) {
-        const int32_t i = tid;
-        for (int32_t j = 0; j < nb; ++j) {
-            T x = X[j * ld + i];
-            for (int32_t k = 0; k < j; ++k) x -= X[k * ld + i] * D[k * ld + j];
-            X[j * ld + i] = x / D[j * ld + j];
-        }
-    }
-    __syncthreads();
-    for (int32_t idx = tid; idx < nb * nr; idx += nt) {
-        int32_t j = idx / nr, i = idx - j * nr;
-        Lp[(int64_t)(b + j) * c + r0 + i] = X[j * ld + i];
-    }
+    T* Lp = L + sym.loff[s] + (int64_t)b * c + b;
+    factor_invert_block<T>(Lp, c, nb, Lp, Dinv + dinv_block_offset(sym, s, b), A, X, err);
 }
 
 // ------------------------------------------------------------- MFMA GEMM
-// C[row0:+64, col0:+64] -= P[rows, k0:k1] * P[cols, k0:k1]^T, lower part only,
-// with P the L panel of the front.  4 waves, each a 32x32 quarter built from
-// 2x2 MFMA 16x16x4 blocks.  Lane layout (measured on gfx950 by
-// help_selftest_mfma): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k;
-// the result D[i][j] sits in lane j + 16(i%4), element i/4 for
-// V_MFMA_F64_16X16X4_F64 and in lane j + 16(i/4), element i%4 for
-// V_MFMA_F32_16X16X4_F32.
+// 4 waves, each a 32x32 quarter of a 64x64 tile built from 2x2 MFMA 16x16x4
+// blocks.  Lane layout (measured on gfx950 by help_selftest_mfma): A[i][k] in
+// lane i + 16k, B[k][j] in lane j + 16k; the result D[i][j] sits in lane
+// j + 16(i%4), element i/4 for V_MFMA_F64_16X16X4_F64 and in lane
+// j + 16(i/4), element i%4 for V_MFMA_F32_16X16X4_F32.
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
@@ -250,14 +279,33 @@ template <> struct Mfma<float> {
     static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// acc += As[k][rows] * Bs[k][cols]^T over k in [0, kc), As/Bs k-major (ld 64)
+template <typename T>
+__device__ inline void mfma_tile(const T* As, const T* Bs, int kc, int lane, int wr, int wc,
+                                 typename Mfma<T>::acc_t (&acc)[2][2]) {
+    for (int kk = 0; kk < kc; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        T a0 = As[kr * kTile + wr + (lane & 15)];
+        T a1 = As[kr * kTile + wr + 16 + (lane & 15)];
+        T b0 = Bs[kr * kTile + wc + (lane & 15)];
+        T b1 = Bs[kr * kTile + wc + 16 + (lane & 15)];
+        acc[0][0] = Mfma<T>::fma(a0, b0, acc[0][0]);
+        acc[0][1] = Mfma<T>::fma(a0, b1, acc[0][1]);
+        acc[1][0] = Mfma<T>::fma(a1, b0, acc[1][0]);
+        acc[1][1] = Mfma<T>::fma(a1, b1, acc[1][1]);
+    }
+}
+
 constexpr int kKc = 16;   // inner-dimension chunk staged in LDS
 
+// C[row0:+64, col0:+64] -= P[rows, k0:k1] * P[cols, k0:k1]^T, lower part only,
+// with P the L panel of the front.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, DevSym sym,
                                               T* __restrict__ L, T* __restrict__ U) {
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ T As[kKc][kTile];
-    __shared__ T Bs[kKc][kTile];
+    __shared__ T As[kKc * kTile];
+    __shared__ T Bs[kKc * kTile];
     const GemmTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s;
     const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
@@ -280,22 +328,11 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
             int32_t kg = kb + k;
             int32_t ia = tk.row0 + li, ib = tk.col0 + li;
             bool kin = kg < tk.k1;
-            As[k][li] = (kin && ia < c) ? Lp[(int64_t)kg * c + ia] : T(0);
-            Bs[k][li] = (kin && ib < c) ? Lp[(int64_t)kg * c + ib] : T(0);
+            As[k * kTile + li] = (kin && ia < c) ? Lp[(int64_t)kg * c + ia] : T(0);
+            Bs[k * kTile + li] = (kin && ib < c) ? Lp[(int64_t)kg * c + ib] : T(0);
         }
         __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kKc; kk += 4) {
-            const int kr = kk + (lane >> 4);
-            T a0 = As[kr][wr + (lane & 15)];
-            T a1 = As[kr][wr + 16 + (lane & 15)];
-            T b0 = Bs[kr][wc + (lane & 15)];
-            T b1 = Bs[kr][wc + 16 + (lane & 15)];
-            acc[0][0] = Mfma<T>::fma(a0, b0, acc[0][0]);
-            acc[0][1] = Mfma<T>::fma(a0, b1, acc[0][1]);
-            acc[1][0] = Mfma<T>::fma(a1, b0, acc[1][0]);
-            acc[1][1] = Mfma<T>::fma(a1, b1, acc[1][1]);
-        }
+        mfma_tile<T>(As, Bs, kKc, lane, wr, wc, acc);
         __syncthreads();
     }
     for (int a = 0; a < 2; ++a)
@@ -307,6 +344,47 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
                 T v = acc[a][b][r];
                 if (j < n) Lw[(int64_t)j * c + i] -= v;
                 else Up[(int64_t)(j - n) * cu + (i - n)] -= v;
+            }
+}
+
+// X := X L_bb^-T for the 64 rows of block rb-1 below the diagonal block,
+// computed as X * (L_bb^-1)^T on MFMA in place.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_gemm(const PanelTask* __restrict__ tasks, DevSym sym,
+                                                   T* __restrict__ L, const T* __restrict__ Dinv) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T As[kTile * kTile];
+    __shared__ T Bs[kTile * kTile];
+    const PanelTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s, b = tk.b;
+    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
+    const int32_t c = sym.snrow[s];
+    const int32_t nb = min(kTile, n - b);
+    const int32_t r0 = b + nb + (tk.rb - 1) * kTile;
+    T* Lp = L + sym.loff[s];
+    const T* Db = Dinv + dinv_block_offset(sym, s, b);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    // As[k][i] = X[i][k] = P[r0+i][b+k];  Bs[k][j] = Dinv[j][k]
+    for (int32_t idx = tid; idx < kTile * kTile; idx += 256) {
+        int32_t k = idx >> 6, i = idx & 63;
+        As[idx] = (k < nb && r0 + i < c) ? Lp[(int64_t)(b + k) * c + r0 + i] : T(0);
+        Bs[idx] = (k < nb && i < nb) ? Db[k * nb + i] : T(0);   // column k of Dinv = Dinv[.][k]
+    }
+    __syncthreads();
+    // Bs currently holds Bs[k][j] = Dinv[j][k]? Dinv column-major: Db[k*nb + j] = Dinv[j][k]. Yes.
+    acc_t acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < 2; ++q) acc[a][q] = acc_t{0, 0, 0, 0};
+    mfma_tile<T>(As, Bs, kTile, lane, wr, wc, acc);
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < 2; ++q)
+            for (int r = 0; r < 4; ++r) {
+                int32_t i = wr + a * 16 + Mfma<T>::row(lane, r);
+                int32_t j = wc + q * 16 + (lane & 15);
+                if (r0 + i >= c || j >= nb) continue;
+                Lp[(int64_t)(b + j) * c + r0 + i] = acc[a][q][r];
             }
 }
 
@@ -330,9 +408,10 @@ template void launch_mfma_probe<double>(const double*, const double*, double*, h
 template void launch_mfma_probe<float>(const float*, const float*, float*, hipStream_t);
 
 template <typename T>
-void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, int32_t* err,
-                         hipStream_t st) {
-    if (count > 0) k_small_fronts<T><<<count, 256, 0, st>>>(list, sym, L, U, err);
+void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, T* Dinv,
+                         int32_t* err, int32_t lds_elems, hipStream_t st) {
+    if (count > 0)
+        k_small_fronts<T><<<count, 256, (size_t)lds_elems * sizeof(T), st>>>(list, sym, L, U, Dinv, err);
 }
 template <typename T>
 void launch_extend_add(const EaTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st) {
@@ -342,25 +421,25 @@ template <typename T>
 void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st) {
     if (count > 0) k_gemm<T><<<count, 256, 0, st>>>(tasks, sym, L, U);
 }
-
 template <typename T>
-void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, int32_t* err,
+void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv, int32_t* err,
                        hipStream_t st) {
-    if (count > 0) k_potrf_diag<T><<<count, 256, 0, st>>>(tasks, sym, L, err);
+    if (count > 0) k_potrf_diag<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv, err);
 }
 template <typename T>
-void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, hipStream_t st) {
-    if (count > 0) k_trsm_rows<T><<<count, 256, 0, st>>>(tasks, sym, L);
+void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, const T* Dinv,
+                      hipStream_t st) {
+    if (count > 0) k_trsm_gemm<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv);
 }
 
-#define HELP_INST(T)                                                                            \
-    template void launch_small_fronts<T>(const int32_t*, int32_t, DevSym, T*, T*, int32_t*,      \
-                                         hipStream_t);                                          \
-    template void launch_extend_add<T>(const EaTask*, int32_t, DevSym, T*, T*, hipStream_t);   \
-    template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, hipStream_t);       \
-    template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, int32_t*,        \
-                                       hipStream_t);                                            \
-    template void launch_trsm_rows<T>(const PanelTask*, int32_t, DevSym, T*, hipStream_t);
+#define HELP_INST(T)                                                                                \
+    template void launch_small_fronts<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,      \
+                                         int32_t, hipStream_t);                                     \
+    template void launch_extend_add<T>(const EaTask*, int32_t, DevSym, T*, T*, hipStream_t);       \
+    template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, hipStream_t);           \
+    template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, T*, int32_t*,        \
+                                       hipStream_t);                                                \
+    template void launch_trsm_rows<T>(const PanelTask*, int32_t, DevSym, T*, const T*, hipStream_t);
 HELP_INST(double)
 HELP_INST(float)
 #undef HELP_INST

@@ -6,6 +6,8 @@
 // CPU restatement of the same HELP loop.
 #pragma clang fp contract(off)
 
+#include <algorithm>
+
 #include "device.h"
 
 namespace help {
@@ -139,7 +141,73 @@ __global__ void k_csr_plus_shift(int32_t m, const int64_t* __restrict__ rowptr,
     y[i] = acc + beta * x[i];
 }
 
+// deterministic dot product: fixed-size grid of block partials, then one block
+constexpr int kDotBlocks = 512;
+
+__global__ __launch_bounds__(256) void k_dot_partial(const double* __restrict__ a,
+                                                     const double* __restrict__ b, int64_t n,
+                                                     double* __restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        acc = acc + a[i] * b[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_dot_final(const double* __restrict__ part, int np,
+                                                   double* __restrict__ out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) acc = acc + part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// y += alpha x
+__global__ void k_axpy(int64_t n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + alpha * x[i];
+}
+// y = x + beta y
+__global__ void k_xpby(int64_t n, const double* __restrict__ x, double beta, double* __restrict__ y) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i] + beta * y[i];
+}
+// z = x - y
+__global__ void k_sub(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                      double* __restrict__ z) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = x[i] - y[i];
+}
+
 }  // namespace
+
+void launch_dot(const double* a, const double* b, int64_t n, double* part, double* out,
+                hipStream_t st) {
+    int nb = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + 255) / 256));
+    k_dot_partial<<<nb, 256, 0, st>>>(a, b, n, part);
+    k_dot_final<<<1, 256, 0, st>>>(part, nb, out);
+}
+void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t st) {
+    if (n > 0) k_axpy<<<grid_for(n), kBlock, 0, st>>>(n, alpha, x, y);
+}
+void launch_xpby(int64_t n, const double* x, double beta, double* y, hipStream_t st) {
+    if (n > 0) k_xpby<<<grid_for(n), kBlock, 0, st>>>(n, x, beta, y);
+}
+void launch_sub(int64_t n, const double* x, const double* y, double* z, hipStream_t st) {
+    if (n > 0) k_sub<<<grid_for(n), kBlock, 0, st>>>(n, x, y, z);
+}
 
 void launch_dinv(const double* d, double* dinv, int32_t n, hipStream_t st) {
     if (n > 0) k_dinv<<<grid_for(n), kBlock, 0, st>>>(d, dinv, n);

@@ -31,8 +31,13 @@ class GpuBackend:
             pass
 
 
-def solve_lp(inst, table, max_iter=0, log=None):
-    """Run the HSD driver. ``table`` is an L.Backend. Returns (result, log_rows)."""
+def solve_lp(inst, table, max_iter=0, log=None, exact=False, device=False):
+    """Run the HSD driver. ``table`` is an L.Backend. Returns (result, log_rows).
+
+    exact=True sums every reduction in the reference's sequential order, so
+    the iterates are bit-identical to HELP's own driver with the same backend.
+    device=True runs the GPU-resident driver (table must be a GpuBackend's).
+    """
     rows = []
 
     def _cb(user, it, pobj, dobj, gap, pinf, dinf):
@@ -43,7 +48,14 @@ def solve_lp(inst, table, max_iter=0, log=None):
     cb = L.LOG_FN(_cb)
     res = L.HsdResult()
     s = inst.as_struct()
-    st = L.lib().help_hsd_solve(C.byref(s), C.byref(table), int(max_iter), cb, None, C.byref(res))
+    if device:
+        st = L.lib().help_hsd_solve_gpu(C.byref(s), C.byref(table), int(max_iter), cb, None,
+                                        C.byref(res))
+        if st != L.HELP_OK:
+            raise HelpError(st, f"help_hsd_solve_gpu failed (backend error {res.backend_error})")
+        return res, rows
+    st = L.lib().help_hsd_solve(C.byref(s), C.byref(table), int(max_iter), 1 if exact else 0, cb,
+                                None, C.byref(res))
     if st != L.HELP_OK:
         raise HelpError(st, f"help_hsd_solve failed (backend error {res.backend_error})")
     return res, rows

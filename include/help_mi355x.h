@@ -108,8 +108,9 @@ int help_gpu_destroy(help_gpu* h);
  * A D^-1 A^T + beta I from d[n] (> 0) and factor it. */
 int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* st);
 
-/* Replaces SolveLinearEquation (UseCholmod.cpp:157-238). Host arrays:
- * d[n] (same values as the last renew), b1[n], b2[m] in; x1[n], x2[m] out. */
+/* Replaces SolveLinearEquation (UseCholmod.cpp:157-238): d[n] (same values
+ * as the last renew), b1[n], b2[m] in; x1[n], x2[m] out.  Host or device
+ * pointers (unified addressing), here and in help_gpu_renew. */
 int help_gpu_solve(help_gpu* h, const double* d, const double* b1,
                    const double* b2, double* x1, double* x2,
                    help_solve_stats* st);
@@ -253,6 +254,10 @@ typedef struct help_hsd_result {
     double  primal_obj, dual_obj, rel_gap, primal_infeas, dual_infeas, mu;
     double  seconds;         /* wall time of the HSD_Main span                    */
     double  seconds_renew, seconds_solve;
+    double  seconds_construct;   /* backend construct (symbolic analysis, setup)  */
+    double  seconds_newton;      /* driver: Newton parameters and residuals       */
+    double  seconds_direction;   /* driver: search directions (minus the solves)  */
+    double  seconds_step;        /* driver: ratio tests and step update           */
     int32_t n_renew, n_solve;
     int32_t backend_error;   /* first nonzero backend status, 0 if none          */
 } help_hsd_result;
@@ -263,9 +268,23 @@ typedef void (*help_hsd_log_fn)(void* user, int32_t iter, double pobj,
                                 double dobj, double gap, double pinf,
                                 double dinf);
 
+/* flags: HELP_HSD_EXACT sums every reduction sequentially in the reference's
+ * order (iterates bit-identical to HELP's driver with the same backend);
+ * without it long sums use 64 fixed chunks in parallel (deterministic,
+ * thread-count independent).  Calls are not reentrant. */
+#define HELP_HSD_EXACT 1
 int help_hsd_solve(const help_lp* lp, const help_backend* be,
-                   int32_t max_iter, help_hsd_log_fn log, void* log_user,
-                   help_hsd_result* res);
+                   int32_t max_iter, int32_t flags, help_hsd_log_fn log,
+                   void* log_user, help_hsd_result* res);
+
+/* The same HSD algorithm with every vector resident on the GPU (D update,
+ * residuals, directions, ratio test and step update in FP64 kernels; the
+ * host keeps the scalars).  be must come from help_gpu_backend_new; its
+ * renew/solve are called with device pointers.  Reductions are
+ * deterministic tree sums, so iterates agree with help_hsd_solve to
+ * rounding.  No counterpart in the reference, whose driver is host code. */
+int help_hsd_solve_gpu(const help_lp* lp, const help_backend* be, int32_t max_iter,
+                       help_hsd_log_fn log, void* log_user, help_hsd_result* res);
 
 /* The MI355X backend as a help_backend (ctx owned by the returned table;
  * release with help_gpu_backend_free). */

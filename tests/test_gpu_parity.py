@@ -162,12 +162,13 @@ def test_solve_mcf_large_fronts():
     be.destruct()
 
 
-@pytest.mark.parametrize("mode", [L.MODE_DIRECT_FP64, L.MODE_DIRECT_FP32])
-def test_hsd_end_to_end(cfg0, mode):
+@pytest.mark.parametrize("mode,device", [(L.MODE_DIRECT_FP64, False), (L.MODE_DIRECT_FP32, False),
+                                         (L.MODE_DIRECT_FP64, True), (L.MODE_DIRECT_FP32, True)])
+def test_hsd_end_to_end(cfg0, mode, device):
     from help_amd.hsd import GpuBackend, solve_lp
-    _flush(f"hsd cfg0 mode={mode}")
+    _flush(f"hsd cfg0 mode={mode} device_driver={device}")
     gb = GpuBackend(mode=mode)
-    res, rows = solve_lp(cfg0, gb.table)
+    res, rows = solve_lp(cfg0, gb.table, device=device)
     gb.close()
     be = LinearEquation(symbolic_only=True)
     be.construct(cfg0)
