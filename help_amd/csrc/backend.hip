@@ -223,13 +223,26 @@ void build_schedules(help_gpu* h) {
         double n = ncol(s), c = S.snrow[s];
         return wsz * (n * (n + 1) / 2 + (with_l21 ? (c - n) * n : 0.0));
     };
+    // each level's list: the tiny supernodes (one wave each) first
+    auto level_list = [&](int32_t l, std::vector<int32_t>& out) {
+        int32_t ntiny = 0;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
+                int32_t s = S.levelsn[t];
+                bool tiny = S.snrow[s] <= kTinyRows;
+                if (tiny != (pass == 0)) continue;
+                out.push_back(s);
+                ntiny += tiny;
+            }
+        return ntiny;
+    };
     for (int32_t l = 0; l < S.nlevels; ++l) {
         int32_t off = (int32_t)h->list_fwd.size(), to = (int32_t)h->fgemv.size();
         double w = 0, wg = 0;
-        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
-            int32_t s = S.levelsn[t];
+        int32_t ntiny = level_list(l, h->list_fwd);
+        for (int32_t t = off; t < (int32_t)h->list_fwd.size(); ++t) {
+            int32_t s = h->list_fwd[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-            h->list_fwd.push_back(s);
             bool big = is_big_solve(c, nc);
             w += tri_bytes(s, !big);
             if (big) {
@@ -238,16 +251,16 @@ void build_schedules(help_gpu* h) {
                 for (int32_t b = 0; nc + b * 64 < c; ++b) h->fgemv.push_back({s, b});
             }
         }
-        h->sops_fwd.push_back({OP_FWD, off, (int32_t)h->list_fwd.size() - off, w});
+        h->sops_fwd.push_back({OP_FWD, off, (int32_t)h->list_fwd.size() - off, w, 0, ntiny});
         if ((int32_t)h->fgemv.size() > to) h->sops_fwd.push_back({OP_FWD_GEMV, to, (int32_t)h->fgemv.size() - to, wg});
     }
     for (int32_t l = S.nlevels - 1; l >= 0; --l) {
         int32_t off = (int32_t)h->list_bwd.size(), to = (int32_t)h->bgemv.size();
         double w = 0, wg = 0;
-        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
-            int32_t s = S.levelsn[t];
+        int32_t ntiny = level_list(l, h->list_bwd);
+        for (int32_t t = off; t < (int32_t)h->list_bwd.size(); ++t) {
+            int32_t s = h->list_bwd[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-            h->list_bwd.push_back(s);
             w += tri_bytes(s, !h->sbig[s]);
             if (h->sbig[s]) {
                 wg += wsz * (double)(c - nc) * nc;
@@ -255,7 +268,7 @@ void build_schedules(help_gpu* h) {
             }
         }
         if ((int32_t)h->bgemv.size() > to) h->sops_bwd.push_back({OP_BWD_GEMV, to, (int32_t)h->bgemv.size() - to, wg});
-        h->sops_bwd.push_back({OP_BWD, off, (int32_t)h->list_bwd.size() - off, w});
+        h->sops_bwd.push_back({OP_BWD, off, (int32_t)h->list_bwd.size() - off, w, 0, ntiny});
     }
     // SpMV-class kernels: 12 bytes per nonzero (value + index) plus vectors
     const double nnz = (double)h->nnz, m = h->m, n = h->n;
@@ -357,12 +370,12 @@ void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x) {
     hipStream_t st = h->st;
     for (const Op& o : h->sops_fwd) {
         ProfScope ps(h, o.kind, o.work);
-        if (o.kind == OP_FWD) launch_fwd<T>(h->d_list_fwd.p + o.off, o.count, h->sym, h->d_big.p, L, Dinv, x, h->d_W.p, st);
+        if (o.kind == OP_FWD) launch_fwd<T>(h->d_list_fwd.p + o.off, o.count, o.ntiny, h->sym, h->d_big.p, L, Dinv, x, h->d_W.p, st);
         else launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, x, h->d_W.p, st);
     }
     for (const Op& o : h->sops_bwd) {
         ProfScope ps(h, o.kind, o.work);
-        if (o.kind == OP_BWD) launch_bwd<T>(h->d_list_bwd.p + o.off, o.count, h->sym, h->d_big.p, L, Dinv, x, h->d_t.p, st);
+        if (o.kind == OP_BWD) launch_bwd<T>(h->d_list_bwd.p + o.off, o.count, o.ntiny, h->sym, h->d_big.p, L, Dinv, x, h->d_t.p, st);
         else launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, st);
     }
     HIPCHK(hipGetLastError());

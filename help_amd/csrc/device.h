@@ -117,11 +117,14 @@ struct SolveTask {
     int32_t s;
     int32_t blk;          // 64-row block of L21 (forward) / 64-column block (backward)
 };
+constexpr int kTinyRows = 64;      // supernodes with at most this many rows take one wave
 // Forward: for every supernode of the level gather the children's update
 // vectors and solve with L11; small supernodes also apply L21 (big[s] == 0).
+// The first `ntiny` entries of `list` have snrow <= kTinyRows and are
+// solved one per wave, four per workgroup; the rest one per workgroup.
 template <typename T>
-void launch_fwd(const int32_t* list, int32_t count, DevSym sym, const int8_t* big, const T* L,
-                const T* Dinv, double* x, double* W, hipStream_t st);
+void launch_fwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big,
+                const T* L, const T* Dinv, double* x, double* W, hipStream_t st);
 // Forward, big supernodes: W_s += L21 x_s by 64-row blocks.
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
@@ -130,9 +133,10 @@ void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T*
 template <typename T>
 void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
                      const double* x, double* t, hipStream_t st);
-// Backward: x_s = L11^-T (x_s - L21^T x_rows) (big supernodes read t).
+// Backward: x_s = L11^-T (x_s - L21^T x_rows) (big supernodes read t);
+// `list` and `ntiny` as in launch_fwd.
 template <typename T>
-void launch_bwd(const int32_t* list, int32_t count, DevSym sym, const int8_t* big, const T* L,
-                const T* Dinv, double* x, const double* t, hipStream_t st);
+void launch_bwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big,
+                const T* L, const T* Dinv, double* x, const double* t, hipStream_t st);
 
 }  // namespace help

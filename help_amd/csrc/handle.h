@@ -61,6 +61,7 @@ struct Op {
     int32_t off, count;
     double work;       // algorithmic flops (factor classes) or bytes (the others)
     int32_t lds = 0;   // dynamic LDS elements per workgroup (small fronts)
+    int32_t ntiny = 0; // solves: leading supernodes of the list solved one per wave
 };
 
 inline const char* const kKindName[HELP_K_COUNT] = {

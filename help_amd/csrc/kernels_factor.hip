@@ -150,6 +150,9 @@ __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict_
 }
 
 // ------------------------------------------------------------ extend-add
+// Task = 32 columns [c0, c1) of a front.  For each child in order, the
+// child columns that land in the range are spread over the 4 waves (a
+// column per wave), the lanes take the rows, 4 rows per lane in flight.
 template <typename T>
 __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks, DevSym sym,
                                                     T* __restrict__ L, T* __restrict__ U) {
@@ -160,26 +163,36 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
     const int32_t cu = c - n;
     T* Lp = L + sym.loff[s];
     T* Up = U + sym.uoff[s];
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
         const int32_t ch = sym.child[t];
         const int32_t chn = sym.sfirst[ch + 1] - sym.sfirst[ch];
         const int32_t chc = sym.snrow[ch] - chn;
         const int32_t* rel = sym.relmap + sym.reloff[ch];
         const T* Uc = U + sym.uoff[ch];
-        // rel is ascending: find the child columns that land in [c0, c1)
+        // rel is ascending: the child columns that land in [c0, c1)
         int32_t lo = 0, hi = chc;
         while (lo < hi) { int32_t mid = (lo + hi) >> 1; if (rel[mid] < tk.c0) lo = mid + 1; else hi = mid; }
-        for (int32_t jj = lo; jj < chc; ++jj) {
+        for (int32_t jj = lo + wave; jj < chc; jj += 4) {
             const int32_t cc = rel[jj];
             if (cc >= tk.c1) break;
             const T* col = Uc + (int64_t)jj * chc;
-            if (cc < n) {
-                T* dst = Lp + (int64_t)cc * c;
-                for (int32_t ii = jj + tid; ii < chc; ii += nt) dst[rel[ii]] += col[ii];
-            } else {
-                T* dst = Up + (int64_t)(cc - n) * cu - n;
-                for (int32_t ii = jj + tid; ii < chc; ii += nt) dst[rel[ii]] += col[ii];
+            T* dst = (cc < n) ? Lp + (int64_t)cc * c : Up + (int64_t)(cc - n) * cu - n;
+            for (int32_t i0 = jj + lane; i0 < chc; i0 += 256) {
+                int32_t rr[4];
+                T v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int32_t ii = i0 + 64 * u;
+                    rr[u] = (ii < chc) ? rel[ii] : -1;
+                    v[u] = (ii < chc) ? col[ii] : T(0);
+                }
+                T d[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) d[u] = (rr[u] >= 0) ? dst[rr[u]] : T(0);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (rr[u] >= 0) dst[rr[u]] = d[u] + v[u];
             }
         }
         __syncthreads();
@@ -238,20 +251,95 @@ __device__ void factor_invert_block(const T* __restrict__ src, int64_t lds, int3
     }
 }
 
+// Register-resident form of factor_invert_block with 512 threads: thread
+// (r, p) holds A[r][k] and X[r][k] for the 8 columns k = 8p + q.  Step j
+// broadcasts column j of A and row j of X through double-buffered LDS (one
+// barrier per step); every thread then applies the same predicated update
+// to its 8 columns, so the step is a short branch-free sequence.  Entries
+// above the diagonal of A are updated too but never read.
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_diag(const PanelTask* __restrict__ tasks, DevSym sym,
+__global__ __launch_bounds__(512) void k_potrf_diag(const PanelTask* __restrict__ tasks, DevSym sym,
                                                     T* __restrict__ L, T* __restrict__ Dinv,
                                                     int32_t* __restrict__ err) {
-    constexpr int ld = kTile + 1;
-    __shared__ T A[kTile * ld];
-    __shared__ T X[kTile * ld];
+    constexpr int Q = 8;
+    __shared__ T colb[2][kTile];
+    __shared__ T xrow[2][kTile];
+    __shared__ T dvec[kTile];
     const PanelTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s, b = tk.b;
     const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
     const int32_t c = sym.snrow[s];
     const int32_t nb = min(kTile, n - b);
     T* Lp = L + sym.loff[s] + (int64_t)b * c + b;
-    factor_invert_block<T>(Lp, c, nb, Lp, Dinv + dinv_block_offset(sym, s, b), A, X, err);
+    T* Db = Dinv + dinv_block_offset(sym, s, b);
+    const int tid = threadIdx.x;
+    const int r = tid & 63, p = tid >> 6;
+    const int k0 = Q * p;
+    T a[Q], x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int k = k0 + q;
+        T v = (r == k) ? T(1) : T(0);
+        if (r < nb && k < nb && r >= k) v = Lp[(int64_t)k * c + r];
+        a[q] = v;
+        x[q] = (r == k) ? T(1) : T(0);
+    }
+    int nbad = 0;
+    for (int j = 0; j < kTile; ++j) {
+        const int buf = j & 1;
+        const int jq = j - k0;               // register of column j in this wave (0..Q-1 if owned)
+        if (jq >= 0 && jq < Q) {
+            T v = a[0];
+#pragma unroll
+            for (int q = 1; q < Q; ++q) v = (q == jq) ? a[q] : v;
+            colb[buf][r] = v;
+        }
+        if (r == j) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) xrow[buf][k0 + q] = x[q];
+        }
+        __syncthreads();
+        T piv = colb[buf][j];
+        if (!(piv > T(0))) { if (j < nb) ++nbad; piv = Huge<T>::v; }
+        const T rpiv = T(1) / piv;
+        const T cr = colb[buf][r];
+        const T m = (r > j) ? cr * rpiv : T(0);
+        T ca[Q], cx[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) { ca[q] = colb[buf][k0 + q]; cx[q] = xrow[buf][k0 + q]; }
+        // columns right of j update A, the others X (both operands finite)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const bool right = k0 + q > j;
+            const T ma = right ? m : T(0), mx = right ? T(0) : m;
+            a[q] -= ma * ca[q];
+            x[q] -= mx * cx[q];
+        }
+        if (jq >= 0 && jq < Q) {
+            const T nv = (r > j) ? m : piv;
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (q == jq && r >= j) a[q] = nv;
+        }
+    }
+    if (tid == 0 && nbad) atomicAdd(err, nbad);
+    // diagonal d_k lives in thread (k, k/Q), register k%Q
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (r == k0 + q) dvec[r] = a[q];
+    __syncthreads();
+    const T sdr = sqrt(dvec[r]);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int k = k0 + q;
+        if (r >= nb || k >= nb) continue;
+        if (r >= k) {
+            Lp[(int64_t)k * c + r] = (r == k) ? sdr : a[q] * sqrt(dvec[k]);
+            Db[k * nb + r] = x[q] / sdr;
+        } else {
+            Db[k * nb + r] = T(0);
+        }
+    }
 }
 
 // ------------------------------------------------------------- MFMA GEMM
@@ -424,7 +512,7 @@ void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, h
 template <typename T>
 void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv, int32_t* err,
                        hipStream_t st) {
-    if (count > 0) k_potrf_diag<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv, err);
+    if (count > 0) k_potrf_diag<T><<<count, 512, 0, st>>>(tasks, sym, L, Dinv, err);
 }
 template <typename T>
 void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, const T* Dinv,

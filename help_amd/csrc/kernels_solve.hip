@@ -10,6 +10,12 @@
 // so a block costs a few barriers instead of 64 dependent steps.  Vectors
 // are FP64 whatever the factor type.  Big supernodes hand the L21 products
 // to separate many-workgroup kernels.
+//
+// The solves are bound by load latency, not bandwidth: a supernode is one
+// dependent chain, so every inner loop keeps several independent loads in
+// flight (multiple accumulators, summed in a fixed order).  Supernodes with
+// at most 64 rows (the large majority, near the leaves) take one wave each,
+// four to a workgroup, with the rows on the lanes and no workgroup barrier.
 #include "device.h"
 
 namespace help {
@@ -17,11 +23,144 @@ namespace help {
 namespace {
 
 constexpr int kMaxLdsCols = kMaxSnCols;   // x segment of a supernode held in LDS
-constexpr int kMaxLdsRows = 4096;         // gathered x(rows) held in LDS
+constexpr int kMaxLdsRows = 2048;         // gathered x(rows) held in LDS by k_bwd
+constexpr int kMaxGemvRows = 4096;        // gathered x(rows) held in LDS by k_bwd_gemv
 
 __device__ inline const void* dinv_block(const DevSym& sym, const void* base, int32_t s, int32_t jb,
                                          int esz) {
     return (const char*)base + (sym.dinvoff[s] + (int64_t)(jb / 64) * 64 * 64) * esz;
+}
+
+// orders this wave's LDS accesses before and after (lanes exchange data
+// through LDS without a workgroup barrier)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// sum_{k<cnt} a[k*stride] * v[k] with two accumulators
+template <typename T>
+__device__ inline double dot_strided(const T* __restrict__ a, int64_t stride, const double* v, int32_t cnt) {
+    double a0 = 0.0, a1 = 0.0;
+    int32_t k = 0;
+#pragma unroll 4
+    for (; k + 1 < cnt; k += 2) {
+        a0 += (double)a[(int64_t)k * stride] * v[k];
+        a1 += (double)a[(int64_t)(k + 1) * stride] * v[k + 1];
+    }
+    if (k < cnt) a0 += (double)a[(int64_t)k * stride] * v[k];
+    return a0 + a1;
+}
+
+// ------------------------------------------------------------- tiny (c <= 64)
+template <typename T>
+__device__ void fwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, const T* __restrict__ Dinv,
+                         double* __restrict__ x, double* __restrict__ W, double* xw) {
+    const int lane = threadIdx.x & 63;
+    const int32_t f = sym.sfirst[s];
+    const int32_t n = sym.sfirst[s + 1] - f;
+    const int32_t c = sym.snrow[s];
+    const T* Lp = L + sym.loff[s];
+    xw[lane] = (lane < n) ? x[f + lane] : 0.0;
+    wave_sync();
+    for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
+        const int32_t ch = sym.child[t];
+        const int32_t chc = sym.snrow[ch] - (sym.sfirst[ch + 1] - sym.sfirst[ch]);
+        if (lane < chc) {
+            const int32_t rr = sym.relmap[sym.reloff[ch] + lane];
+            const double v = W[sym.reloff[ch] + lane];
+            if (rr < n) xw[rr] -= v;
+            else xw[rr] += v;
+        }
+        wave_sync();
+    }
+    // one diagonal block: x1 = Dinv x1
+    const T* Db = (const T*)dinv_block(sym, Dinv, s, 0, sizeof(T));
+    const double y = (lane < n) ? dot_strided<T>(Db + lane, n, xw, n) : 0.0;
+    wave_sync();
+    if (lane < n) xw[lane] = y;
+    wave_sync();
+    if (lane < n) {
+        x[f + lane] = y;
+    } else if (lane < c) {
+        const double a = dot_strided<T>(Lp + lane, c, xw, n);
+        W[sym.reloff[s] + lane - n] = xw[lane] + a;
+    }
+}
+
+template <typename T>
+__device__ void bwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, const T* __restrict__ Dinv,
+                         double* __restrict__ x, double* xw) {
+    const int lane = threadIdx.x & 63;
+    const int32_t f = sym.sfirst[s];
+    const int32_t n = sym.sfirst[s + 1] - f;
+    const int32_t c = sym.snrow[s];
+    const T* Lp = L + sym.loff[s];
+    const int32_t* rows = sym.rowidx + sym.rowoff[s];
+    xw[lane] = (lane >= n && lane < c) ? x[rows[lane]] : 0.0;
+    wave_sync();
+    double v = 0.0;
+    if (lane < n) {
+        const T* col = Lp + (int64_t)lane * c;
+        v = x[f + lane] - dot_strided<T>(col + n, 1, xw + n, c - n);
+    }
+    wave_sync();
+    if (lane < n) xw[lane] = v;
+    wave_sync();
+    if (lane < n) {
+        const T* Db = (const T*)dinv_block(sym, Dinv, s, 0, sizeof(T));
+        x[f + lane] = dot_strided<T>(Db + (int64_t)lane * n, 1, xw, n);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
+                                                  const T* __restrict__ L, const T* __restrict__ Dinv,
+                                                  double* __restrict__ x, double* __restrict__ W) {
+    __shared__ double xw[4][64];
+    const int wave = threadIdx.x >> 6;
+    const int32_t idx = blockIdx.x * 4 + wave;
+    if (idx < count) fwd_tiny<T>(list[idx], sym, L, Dinv, x, W, xw[wave]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
+                                                  const T* __restrict__ L, const T* __restrict__ Dinv,
+                                                  double* __restrict__ x) {
+    __shared__ double xw[4][64];
+    const int wave = threadIdx.x >> 6;
+    const int32_t idx = blockIdx.x * 4 + wave;
+    if (idx < count) bwd_tiny<T>(list[idx], sym, L, Dinv, x, xw[wave]);
+}
+
+// ------------------------------------------------- one workgroup per supernode
+// x1 = Dinv_b x1 for the 64-column block at jb: thread (r, part) sums the
+// columns k = part (mod 4), the four parts combine in a fixed order.
+template <typename T, bool Trans>
+__device__ inline void apply_dinv(const T* __restrict__ Db, int32_t nb, double* xs, double (*red)[64]) {
+    const int r = threadIdx.x & 63, part = threadIdx.x >> 6;
+    double a0 = 0.0, a1 = 0.0;
+    if (nb == 64) {   // all loads issued together
+#pragma unroll
+        for (int u = 0; u < 16; u += 2) {
+            const int k0 = part + 4 * u, k1 = k0 + 4;
+            a0 += (double)(Trans ? Db[r * 64 + k0] : Db[k0 * 64 + r]) * xs[k0];
+            a1 += (double)(Trans ? Db[r * 64 + k1] : Db[k1 * 64 + r]) * xs[k1];
+        }
+    } else if (r < nb) {
+        for (int k = part; k < nb; k += 4)
+            a0 += (double)(Trans ? Db[r * nb + k] : Db[k * nb + r]) * xs[k];
+    }
+    red[part][r] = a0 + a1;
+    __syncthreads();
+    if (part == 0 && r < nb) xs[r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+    __syncthreads();
 }
 
 template <typename T>
@@ -39,7 +178,6 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t* __restrict__ list, D
     const T* Lp = L + sym.loff[s];
     double* Ws = W + sym.reloff[s];
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int r = tid & 63, part = tid >> 6;
     for (int32_t t = tid; t < cu; t += nt) Ws[t] = 0.0;
     for (int32_t t = tid; t < n; t += nt) xs[t] = x[f + t];
     __syncthreads();
@@ -59,17 +197,24 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t* __restrict__ list, D
     const int32_t rend = big[s] ? n : c;
     for (int32_t jb = 0; jb < n; jb += 64) {
         const int32_t nb = min(64, n - jb);
-        const T* Db = (const T*)dinv_block(sym, Dinv, s, jb, sizeof(T));
-        double acc = 0.0;
-        if (r < nb)
-            for (int32_t k = part; k < nb; k += 4) acc += (double)Db[k * nb + r] * xs[jb + k];
-        red[part][r] = acc;
-        __syncthreads();
-        if (part == 0 && r < nb) xs[jb + r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
-        __syncthreads();
+        apply_dinv<T, false>((const T*)dinv_block(sym, Dinv, s, jb, sizeof(T)), nb, xs + jb, red);
+        // rows below the block: one row per thread, 64 loads in flight
+        const double* xv = xs + jb;
         for (int32_t i = jb + nb + tid; i < rend; i += nt) {
-            double a = 0.0;
-            for (int32_t j = 0; j < nb; ++j) a += (double)Lp[(int64_t)(jb + j) * c + i] * xs[jb + j];
+            const T* lp = Lp + (int64_t)jb * c + i;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            if (nb == 64) {
+#pragma unroll
+                for (int j = 0; j < 64; j += 4) {
+                    a0 += (double)lp[(int64_t)j * c] * xv[j];
+                    a1 += (double)lp[(int64_t)(j + 1) * c] * xv[j + 1];
+                    a2 += (double)lp[(int64_t)(j + 2) * c] * xv[j + 2];
+                    a3 += (double)lp[(int64_t)(j + 3) * c] * xv[j + 3];
+                }
+            } else {
+                a0 = dot_strided<T>(lp, c, xv, nb);
+            }
+            const double a = (a0 + a1) + (a2 + a3);
             if (i < n) xs[i] -= a;
             else Ws[i - n] += a;
         }
@@ -98,10 +243,20 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const SolveTask* __restrict__ 
     // 64 rows, 4 threads per row splitting the columns, fixed combine order
     const int32_t r = tid & 63, part = tid >> 6;
     const int32_t i = n + tk.blk * 64 + r;
-    double acc = 0.0;
-    if (i < c)
-        for (int32_t j = part; j < n; j += 4) acc += (double)Lp[(int64_t)j * c + i] * xs[j];
-    red[part][r] = acc;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (i < c) {
+        const T* lp = Lp + i;
+        int32_t j = part;
+#pragma unroll 2
+        for (; j + 12 < n; j += 16) {
+            a0 += (double)lp[(int64_t)j * c] * xs[j];
+            a1 += (double)lp[(int64_t)(j + 4) * c] * xs[j + 4];
+            a2 += (double)lp[(int64_t)(j + 8) * c] * xs[j + 8];
+            a3 += (double)lp[(int64_t)(j + 12) * c] * xs[j + 12];
+        }
+        for (; j < n; j += 4) a0 += (double)lp[(int64_t)j * c] * xs[j];
+    }
+    red[part][r] = (a0 + a1) + (a2 + a3);
     __syncthreads();
     if (part == 0 && i < c) Ws[i - n] += (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
 }
@@ -112,7 +267,7 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const SolveTask* __restrict__ 
                                                   const T* __restrict__ L,
                                                   const double* __restrict__ x,
                                                   double* __restrict__ tout) {
-    __shared__ double xr[kMaxLdsRows];
+    __shared__ double xr[kMaxGemvRows];
     const SolveTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s;
     const int32_t f = sym.sfirst[s];
@@ -123,21 +278,31 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const SolveTask* __restrict__ 
     const int32_t* rows = sym.rowidx + sym.rowoff[s];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const bool lds = cu <= kMaxLdsRows;
+    const bool lds = cu <= kMaxGemvRows;
     if (lds)
         for (int32_t i = tid; i < cu; i += 256) xr[i] = x[rows[n + i]];
     __syncthreads();
-    for (int32_t q = 0; q < 4; ++q) {
-        const int32_t j = tk.blk * 16 + wave * 4 + q;
-        if (j >= n) break;
-        const T* col = Lp + (int64_t)j * c + n;
-        double acc = 0.0;
-        if (lds)
-            for (int32_t i = lane; i < cu; i += 64) acc += (double)col[i] * xr[i];
-        else
-            for (int32_t i = lane; i < cu; i += 64) acc += (double)col[i] * x[rows[n + i]];
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-        if (lane == 0) tout[f + j] = acc;
+    // each wave: 4 columns at once, lanes over the rows
+    const int32_t j0 = tk.blk * 16 + wave * 4;
+    if (j0 >= n) return;
+    const int32_t nq = min(4, n - j0);
+    const T* col = Lp + (int64_t)j0 * c + n;
+    // columns past the supernode's last repeat its last one (loads stay
+    // unconditional); their sums are not stored
+    int64_t coff[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) coff[q] = (int64_t)min(q, nq - 1) * c;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int32_t i = lane; i < cu; i += 64) {
+        const double xv = lds ? xr[i] : x[rows[n + i]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] += (double)col[coff[q] + i] * xv;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double v = wave_sum(acc[q]);
+        if (lane == q && q < nq) tout[f + j0 + q] = v;
     }
 }
 
@@ -147,24 +312,51 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, D
                                              const T* __restrict__ L, const T* __restrict__ Dinv,
                                              double* __restrict__ x, const double* __restrict__ tin) {
     __shared__ double xs[kMaxLdsCols];
+    __shared__ double xr[kMaxLdsRows];
     __shared__ double red[4][64];
     const int32_t s = list[blockIdx.x];
     const int32_t f = sym.sfirst[s];
     const int32_t n = sym.sfirst[s + 1] - f;
     const int32_t c = sym.snrow[s];
+    const int32_t cu = c - n;
     const T* Lp = L + sym.loff[s];
     const int32_t* rows = sym.rowidx + sym.rowoff[s];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int r = tid & 63, part = tid >> 6;
     if (big[s]) {
         for (int32_t t = tid; t < n; t += nt) xs[t] = x[f + t] - tin[f + t];
     } else {
+        const bool lds = cu <= kMaxLdsRows;
+        if (lds)
+            for (int32_t i = tid; i < cu; i += nt) xr[i] = x[rows[n + i]];
+        __syncthreads();
+        // few rows below the diagonal block: a thread per column
+        if (cu <= 64) {
+            for (int32_t j = tid; j < n; j += nt)
+                xs[j] = x[f + j] - dot_strided<T>(Lp + (int64_t)j * c + n, 1, xr, cu);
+        } else
         for (int32_t j = wave; j < n; j += 4) {
-            const T* col = Lp + (int64_t)j * c;
-            double acc = 0.0;
-            for (int32_t i = n + lane; i < c; i += 64) acc += (double)col[i] * x[rows[i]];
-            for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+            const T* col = Lp + (int64_t)j * c + n;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int32_t i = lane;
+            if (lds) {
+                for (; i + 192 < cu; i += 256) {
+                    a0 += (double)col[i] * xr[i];
+                    a1 += (double)col[i + 64] * xr[i + 64];
+                    a2 += (double)col[i + 128] * xr[i + 128];
+                    a3 += (double)col[i + 192] * xr[i + 192];
+                }
+                for (; i < cu; i += 64) a0 += (double)col[i] * xr[i];
+            } else {
+                for (; i + 192 < cu; i += 256) {
+                    a0 += (double)col[i] * x[rows[n + i]];
+                    a1 += (double)col[i + 64] * x[rows[n + i + 64]];
+                    a2 += (double)col[i + 128] * x[rows[n + i + 128]];
+                    a3 += (double)col[i + 192] * x[rows[n + i + 192]];
+                }
+                for (; i < cu; i += 64) a0 += (double)col[i] * x[rows[n + i]];
+            }
+            const double acc = wave_sum((a0 + a1) + (a2 + a3));
             if (lane == 0) xs[j] = x[f + j] - acc;
         }
     }
@@ -175,23 +367,28 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, D
         const int32_t nb = min(64, n - jb);
         const int32_t below = jb + nb;
         if (below < n) {
-            for (int32_t q = wave; q < nb; q += 4) {
-                const T* col = Lp + (int64_t)(jb + q) * c;
-                double acc = 0.0;
-                for (int32_t i = below + lane; i < n; i += 64) acc += (double)col[i] * xs[i];
-                for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-                if (lane == 0) xs[jb + q] -= acc;
+            // xs[jb+q] -= sum_{below <= i < n} L(i, jb+q) xs[i]; wave w takes
+            // the 16 columns 16w.., lanes over the rows, 16 loads in flight
+            // (only the last block can be partial, and nothing lies below it)
+            const int32_t q0 = wave * 16;
+            const T* base = Lp + (int64_t)(jb + q0) * c;
+            double acc[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+#pragma unroll 2
+            for (int32_t i = below + lane; i < n; i += 64) {
+                const double xv = xs[i];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) acc[u] += (double)base[(int64_t)u * c + i] * xv;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const double v = wave_sum(acc[u]);
+                if (lane == u) xs[jb + q0 + u] -= v;
             }
             __syncthreads();
         }
-        const T* Db = (const T*)dinv_block(sym, Dinv, s, jb, sizeof(T));
-        double acc = 0.0;
-        if (r < nb)
-            for (int32_t k = part; k < nb; k += 4) acc += (double)Db[r * nb + k] * xs[jb + k];
-        red[part][r] = acc;
-        __syncthreads();
-        if (part == 0 && r < nb) xs[jb + r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
-        __syncthreads();
+        apply_dinv<T, true>((const T*)dinv_block(sym, Dinv, s, jb, sizeof(T)), nb, xs + jb, red);
     }
     for (int32_t t = tid; t < n; t += nt) x[f + t] = xs[t];
 }
@@ -199,9 +396,10 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, D
 }  // namespace
 
 template <typename T>
-void launch_fwd(const int32_t* list, int32_t count, DevSym sym, const int8_t* big, const T* L,
+void launch_fwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big, const T* L,
                 const T* Dinv, double* x, double* W, hipStream_t st) {
-    if (count > 0) k_fwd<T><<<count, 256, 0, st>>>(list, sym, big, L, Dinv, x, W);
+    if (ntiny > 0) k_fwd_tiny<T><<<(ntiny + 3) / 4, 256, 0, st>>>(list, ntiny, sym, L, Dinv, x, W);
+    if (count > ntiny) k_fwd<T><<<count - ntiny, 256, 0, st>>>(list + ntiny, sym, big, L, Dinv, x, W);
 }
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* x,
@@ -214,20 +412,21 @@ void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T*
     if (count > 0) k_bwd_gemv<T><<<count, 256, 0, st>>>(tasks, sym, L, x, t);
 }
 template <typename T>
-void launch_bwd(const int32_t* list, int32_t count, DevSym sym, const int8_t* big, const T* L,
+void launch_bwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big, const T* L,
                 const T* Dinv, double* x, const double* t, hipStream_t st) {
-    if (count > 0) k_bwd<T><<<count, 256, 0, st>>>(list, sym, big, L, Dinv, x, t);
+    if (count > ntiny) k_bwd<T><<<count - ntiny, 256, 0, st>>>(list + ntiny, sym, big, L, Dinv, x, t);
+    if (ntiny > 0) k_bwd_tiny<T><<<(ntiny + 3) / 4, 256, 0, st>>>(list, ntiny, sym, L, Dinv, x);
 }
 
 #define HELP_INST(T)                                                                                  \
-    template void launch_fwd<T>(const int32_t*, int32_t, DevSym, const int8_t*, const T*, const T*,   \
-                                double*, double*, hipStream_t);                                       \
+    template void launch_fwd<T>(const int32_t*, int32_t, int32_t, DevSym, const int8_t*, const T*,   \
+                                const T*, double*, double*, hipStream_t);                             \
     template void launch_fwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,      \
                                      double*, hipStream_t);                                           \
     template void launch_bwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,      \
                                      double*, hipStream_t);                                           \
-    template void launch_bwd<T>(const int32_t*, int32_t, DevSym, const int8_t*, const T*, const T*,   \
-                                double*, const double*, hipStream_t);
+    template void launch_bwd<T>(const int32_t*, int32_t, int32_t, DevSym, const int8_t*, const T*,    \
+                                const T*, double*, const double*, hipStream_t);
 HELP_INST(double)
 HELP_INST(float)
 #undef HELP_INST
