@@ -111,6 +111,8 @@ def main():
                     help="gpu: device-resident HSD vectors; host: HELP-order host driver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--refine-max", type=int, default=None,
+                    help="FP64 refinement iterations per solve (default: the library's)")
     args = ap.parse_args()
     if args.gpus != 1:
         msg = ("bench.py measures one MI355X: this backend is single-GPU "
@@ -130,15 +132,16 @@ def main():
     print(f"[bench] config {args.config} m={inst.m} n={inst.n} nnz={inst.nnz} mode={args.mode} "
           f"(generated in {gen_s:.1f}s)", file=sys.stderr, flush=True)
 
+    extra = {} if args.refine_max is None else {"refine_max": args.refine_max}
     for w in range(args.warmup):
-        gb = GpuBackend(mode=mode)
+        gb = GpuBackend(mode=mode, **extra)
         res, _ = solve_lp(inst, gb.table, device=args.driver == "gpu")
         gb.close()
         print(f"[bench] warmup {w}: status {res.status} iters {res.iterations} {res.seconds:.2f}s",
               file=sys.stderr, flush=True)
 
     L.lib().help_device_synchronize()
-    gb = GpuBackend(mode=mode, profile=0 if args.no_profile else 1)
+    gb = GpuBackend(mode=mode, profile=0 if args.no_profile else 1, **extra)
     times, results = [], []
     t_start = time.time()
     for k in range(args.steps):
@@ -152,6 +155,7 @@ def main():
               file=sys.stderr, flush=True)
     total = time.time() - t_start
     rows = kernel_table(L, gb.table)
+    n_sol, n_ref, max_rel = gb.solve_stats()
     gb.close()
     res = results[-1]
     value = sum(times) / len(times)
@@ -183,7 +187,8 @@ def main():
                    "driver_newton_s": round(res.seconds_newton, 3),
                    "driver_direction_s": round(res.seconds_direction, 3),
                    "driver_step_s": round(res.seconds_step, 3),
-                   "n_renew": res.n_renew, "n_solve": res.n_solve},
+                   "n_renew": res.n_renew, "n_solve": res.n_solve,
+                   "refine_iterations": n_ref, "max_rel_residual": max_rel},
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
     }
     print(json.dumps(line), flush=True)

@@ -90,7 +90,8 @@ class HsdResult(C.Structure):
                 ("seconds_renew", C.c_double), ("seconds_solve", C.c_double),
                 ("seconds_construct", C.c_double), ("seconds_newton", C.c_double),
                 ("seconds_direction", C.c_double), ("seconds_step", C.c_double),
-                ("n_renew", C.c_int32), ("n_solve", C.c_int32), ("backend_error", C.c_int32)]
+                ("n_renew", C.c_int32), ("n_solve", C.c_int32), ("backend_error", C.c_int32),
+                ("stall_reason", C.c_int32), ("newton_ratio", C.c_double)]
 
 
 LOG_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_double, C.c_double, C.c_double,
@@ -132,6 +133,8 @@ SIGNATURES = {
     "help_gpu_backend_handle": (_P, [C.POINTER(Backend)]),
     "help_gpu_backend_kernel_stats": (C.c_int, [C.POINTER(Backend), C.c_int32, _D,
                                                 C.POINTER(C.c_int64), _D]),
+    "help_gpu_backend_solve_stats": (C.c_int, [C.POINTER(Backend), C.POINTER(C.c_int64),
+                                               C.POINTER(C.c_int64), _D]),
     "help_gpu_profile": (C.c_int, [_P, C.c_int32]),
     "help_gpu_kernel_stats": (C.c_int, [_P, C.c_int32, _D, C.POINTER(C.c_int64), _D,
                                         C.POINTER(C.c_char_p)]),

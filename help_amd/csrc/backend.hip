@@ -13,6 +13,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "device.h"
@@ -389,23 +390,31 @@ double dot_dev(help_gpu* h, const double* a, const double* b, int64_t n) {
     return v;
 }
 
-// z = P (L L^T)^-1 P^T r with the FP32 factor (vectors FP64)
-void apply_fp32_factor(help_gpu* h, const double* r, double* z) {
+// z = P (L L^T)^-1 P^T r with the factor of type T (vectors FP64)
+template <typename T>
+void apply_factor(help_gpu* h, const double* r, double* z) {
+    const T* Lf = std::is_same<T, float>::value ? (const T*)h->d_L32.p : (const T*)h->d_L64.p;
+    const T* Df = std::is_same<T, float>::value ? (const T*)h->d_Dinv32.p : (const T*)h->d_Dinv64.p;
     launch_scatter_perm(h->m, r, h->d_iperm.p, h->d_rp.p, h->st);
-    run_trisolve<float>(h, h->d_L32.p, h->d_Dinv32.p, h->d_rp.p);
+    run_trisolve<T>(h, Lf, Df, h->d_rp.p);
     launch_gather_perm(h->m, h->d_rp.p, h->d_iperm.p, z, h->st);
 }
 
-// Preconditioned CG in FP64 on M = A D^-1 A^T + beta I with the FP32
-// factor as preconditioner.  On entry d_rp holds the permuted right-hand
-// side; on exit d_x2 holds the solution (original order).  Stops at
-// ||r|| <= refine_tol ||rhs|| or refine_max iterations; returns iterations.
-int pcg_fp32_factor(help_gpu* h, double& rel) {
+// Preconditioned CG in FP64 on M = A D^-1 A^T + beta I with the factor of
+// type T as preconditioner, started from the factor's solution.  On entry
+// d_rp holds the permuted right-hand side; on exit d_x2 holds the solution
+// (original order).  Stops at ||r|| <= refine_tol ||rhs||, after
+// refine_max iterations, or (FP64 factor) when an iteration fails to halve
+// the residual: the rounding floor; returns iterations.
+template <typename T>
+int pcg_factor(help_gpu* h, double& rel) {
     hipStream_t st = h->st;
     const int32_t m = h->m, n = h->n;
     double *rhs = h->d_c.p, *r = h->d_r.p, *x = h->d_x2.p, *z = h->d_z.p, *p = h->d_p.p, *q = h->d_q.p;
+    const T* Lf = std::is_same<T, float>::value ? (const T*)h->d_L32.p : (const T*)h->d_L64.p;
+    const T* Df = std::is_same<T, float>::value ? (const T*)h->d_Dinv32.p : (const T*)h->d_Dinv64.p;
     launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, rhs, st);
-    run_trisolve<float>(h, h->d_L32.p, h->d_Dinv32.p, h->d_rp.p);
+    run_trisolve<T>(h, Lf, Df, h->d_rp.p);
     launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, x, st);
     launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
     auto matvec = [&](const double* v, double* out) {
@@ -416,14 +425,14 @@ int pcg_fp32_factor(help_gpu* h, double& rel) {
     launch_sub(m, rhs, q, r, st);
     const double rr0 = dot_dev(h, rhs, rhs, m);
     if (rr0 == 0.0) { rel = 0; return 0; }
-    apply_fp32_factor(h, r, z);
+    double rr = dot_dev(h, r, r, m);
+    rel = std::sqrt(rr / rr0);
+    if (rel <= h->opts.refine_tol || h->opts.refine_max <= 0) return 0;
+    apply_factor<T>(h, r, z);
     HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
     double rz = dot_dev(h, r, z, m);
     int it = 0;
-    for (;;) {
-        double rr = dot_dev(h, r, r, m);
-        rel = std::sqrt(rr / rr0);
-        if (rel <= h->opts.refine_tol || it >= h->opts.refine_max || !(rz > 0)) break;
+    while (it < h->opts.refine_max && rz > 0) {
         ++it;
         matvec(p, q);
         double pq = dot_dev(h, p, q, m);
@@ -431,7 +440,14 @@ int pcg_fp32_factor(help_gpu* h, double& rel) {
         double alpha = rz / pq;
         launch_axpy(m, alpha, p, x, st);
         launch_axpy(m, -alpha, q, r, st);
-        apply_fp32_factor(h, r, z);
+        const double prev = rel;
+        rr = dot_dev(h, r, r, m);
+        rel = std::sqrt(rr / rr0);
+        if (rel <= h->opts.refine_tol) break;
+        // with the FP64 factor one iteration reaches the rounding floor; the
+        // CG residual norm is not monotone, so the FP32 factor runs on
+        if (std::is_same<T, double>::value && rel > 0.5 * prev) break;
+        apply_factor<T>(h, r, z);
         double rz_new = dot_dev(h, r, z, m);
         double beta = rz_new / rz;
         rz = rz_new;
@@ -611,14 +627,11 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
                        h->d_iperm.p, h->d_rp.p, st);
         }
         HIPCHK(hipEventRecord(h->ev[1], st));
-        if (!h->use_fp32) {
-            run_trisolve<double>(h, h->d_L64.p, h->d_Dinv64.p, h->d_rp.p);
-            launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, h->d_x2.p, st);
-        } else {
-            // FP32 factor as the preconditioner of FP64 conjugate gradients on
-            // (A D^-1 A^T + beta I), started from the FP32-factor solution
+        {
+            // the factor's solution, refined by FP64 conjugate gradients on
+            // (A D^-1 A^T + beta I) preconditioned with the same factor
             double rel = 0;
-            int it = pcg_fp32_factor(h, rel);
+            int it = h->use_fp32 ? pcg_factor<float>(h, rel) : pcg_factor<double>(h, rel);
             loc.refine_steps = it;
             loc.rel_residual = rel;
         }
@@ -816,6 +829,8 @@ struct GpuBackendCtx {
     help_gpu* h = nullptr;
     int32_t last_status = 0;
     help_gpu::KStat kstat[HELP_K_COUNT];
+    int64_t n_solve = 0, refine_steps = 0;
+    double max_rel_residual = 0;
 };
 
 static void gb_release(GpuBackendCtx* c) {
@@ -845,7 +860,14 @@ static int gb_renew(void* ctx, const double* d) {
 static int gb_solve(void* ctx, const double* d, const double* b1, const double* b2, double* x1,
                     double* x2) {
     auto* c = (GpuBackendCtx*)ctx;
-    return help_gpu_solve(c->h, d, b1, b2, x1, x2, nullptr);
+    help_solve_stats ss{};
+    int st = help_gpu_solve(c->h, d, b1, b2, x1, x2, &ss);
+    if (st == HELP_OK) {
+        c->n_solve += 1;
+        c->refine_steps += ss.refine_steps;
+        c->max_rel_residual = std::max(c->max_rel_residual, ss.rel_residual);
+    }
+    return st;
 }
 
 int help_gpu_backend_new(const help_opts* opts, help_backend* out) {
@@ -873,6 +895,16 @@ void help_gpu_backend_free(help_backend* be) {
 help_gpu* help_gpu_backend_handle(const help_backend* be) {
     if (!be || !be->ctx) return nullptr;
     return ((GpuBackendCtx*)be->ctx)->h;
+}
+
+int help_gpu_backend_solve_stats(const help_backend* be, int64_t* n_solve, int64_t* refine_steps,
+                                 double* max_rel_residual) {
+    if (!be || !be->ctx) return HELP_ERR_ARG;
+    auto* c = (GpuBackendCtx*)be->ctx;
+    if (n_solve) *n_solve = c->n_solve;
+    if (refine_steps) *refine_steps = c->refine_steps;
+    if (max_rel_residual) *max_rel_residual = c->max_rel_residual;
+    return HELP_OK;
 }
 
 int help_gpu_backend_kernel_stats(const help_backend* be, int32_t kind, double* ms,

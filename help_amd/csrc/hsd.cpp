@@ -73,6 +73,8 @@ struct Hsd {
     double tau = 1, kappa = 1, rho = 1, inf0 = 1;
     double sum_r_norm = 0, pobj = 0, dobj = 0, comp_gap = 0, rel_gap = 0, pinf = 0, dinf = 0, infe = 0;
     int status = HELP_HSD_OK;
+    int stall_reason = HELP_STALL_NONE;
+    double newton_ratio = 0;
     std::vector<double> D, D_u, r_p, r_d;
     double D_g = 0, r_g = 0, mu = 0, rfval = 0;
     std::vector<double> sle1, sle2;                       // HSD_SLE_RHS1/2
@@ -208,7 +210,7 @@ struct Hsd {
                 D[i] = z[i] / x[i];
                 if (D[i] <= 0.0) bad = true;
             }
-            if (bad) { status = HELP_HSD_STALLED; return; }
+            if (bad) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_NEGATIVE; return; }
             prod_min_sum(n_lb, z.data(), 0, nullptr, x.data(), 0, nullptr, xz_min, xz_sum);
         } else
         for (int i = 0; i < n_lb; ++i) {
@@ -216,7 +218,7 @@ struct Hsd {
             double t = z[i] * x[i];
             xz_min = std::min(xz_min, t);
             xz_sum += t;
-            if (D[i] <= 0.0) { status = HELP_HSD_STALLED; return; }
+            if (D[i] <= 0.0) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_NEGATIVE; return; }
         }
         for (int i = 0; i < n_ub; ++i) {
             D_u[i] = zu[i] / xu[i];
@@ -224,7 +226,7 @@ struct Hsd {
             double t = zu[i] * xu[i];
             xz_min = std::min(xz_min, t);
             xz_sum += t;
-            if (D_u[i] <= 0.0) { status = HELP_HSD_STALLED; return; }
+            if (D_u[i] <= 0.0) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_NEGATIVE; return; }
         }
         double tmp = 0;
         for (int i = 0; i < n_fr; ++i) {
@@ -569,6 +571,8 @@ struct Hsd {
                 gamma = std::max(1e-6, gamma);
             } else {
                 status = HELP_HSD_STALLED;
+                stall_reason = HELP_STALL_DIRECTION;
+                newton_ratio = (rpn + rdn + rgn) / sum_r_norm;
                 break;
             }
             eta = 1.0 - gamma;
@@ -621,6 +625,8 @@ extern "C" int help_hsd_solve(const help_lp* lp, const help_backend* be, int32_t
     be->destruct(be->ctx);
     res->seconds = now_s() - t0;
     res->status = h.status;
+    res->stall_reason = h.stall_reason;
+    res->newton_ratio = h.newton_ratio;
     res->iterations = it;
     res->primal_obj = h.pobj;
     res->dual_obj = h.dobj;

@@ -65,16 +65,30 @@ __global__ __launch_bounds__(256) void k_reduce(int64_t n, F f, double* __restri
     }
 }
 
+// Combines the per-block partials of k_reduce: thread t folds partials
+// t, t+256, ... and a fixed LDS tree finishes, so the order depends only on
+// the partial count.
 template <int K>
-__global__ void k_reduce_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
-    const int k = threadIdx.x;
-    if (k > K) return;
-    double acc = k < K ? 0.0 : HUGE_VAL;
-    for (int b = 0; b < nb; ++b) {
-        double v = part[(int64_t)b * (K + 1) + k];
-        acc = k < K ? acc + v : fmin(acc, v);
+__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ part, int nb,
+                                                      double* __restrict__ out) {
+    __shared__ double sm[256];
+    const int t = threadIdx.x;
+    for (int k = 0; k <= K; ++k) {
+        const bool is_min = (k == K);
+        double acc = is_min ? HUGE_VAL : 0.0;
+        for (int b = t; b < nb; b += 256) {
+            const double v = part[(int64_t)b * (K + 1) + k];
+            acc = is_min ? fmin(acc, v) : acc + v;
+        }
+        sm[t] = acc;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w) sm[t] = is_min ? fmin(sm[t], sm[t + w]) : sm[t] + sm[t + w];
+            __syncthreads();
+        }
+        if (t == 0) out[k] = sm[0];
+        __syncthreads();
     }
-    out[k] = acc;
 }
 
 struct HsdGpu {
@@ -95,6 +109,8 @@ struct HsdGpu {
     double tau = 1, kappa = 1, rho = 1, inf0 = 1;
     double sum_r_norm = 0, pobj = 0, dobj = 0, rel_gap = 0, pinf = 0, dinf = 0, infe = 0;
     int status = HELP_HSD_OK;
+    int stall_reason = HELP_STALL_NONE;
+    double newton_ratio = 0;
     double D_g = 0, r_g = 0, mu = 0, rfval = 0, htk = 0, dtau = 0, dkappa = 0;
     int backend_error = 0;
     double t_renew = 0, t_solve = 0, t_newton = 0, t_direction = 0, t_step = 0;
@@ -120,7 +136,7 @@ struct HsdGpu {
         if (k <= 0) return r;
         int nb = (int)std::min<int64_t>(kRedBlocks, (k + 255) / 256);
         k_reduce<K><<<nb, 256, 0, st>>>(k, f, part.p);
-        k_reduce_final<K><<<1, 64, 0, st>>>(part.p, nb, out.p);
+        k_reduce_final<K><<<1, 256, 0, st>>>(part.p, nb, out.p);
         HIPCHK(hipMemcpyAsync(r.data(), out.p, sizeof(double) * (K + 1), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         return r;
@@ -176,14 +192,14 @@ struct HsdGpu {
         map(n_lb, [=] __device__(int64_t i) { Dp[i] = zp[i] / xp[i]; });
         auto r1 = reduce<1>(n_lb, [=] __device__(int64_t i, double* v) { v[0] = zp[i] * xp[i]; });
         auto rd = reduce<1>(n_lb, [=] __device__(int64_t i, double* v) { v[0] = Dp[i]; });
-        if (n_lb > 0 && rd[1] <= 0.0) { status = HELP_HSD_STALLED; return; }
+        if (n_lb > 0 && rd[1] <= 0.0) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_NEGATIVE; return; }
         xz_min = std::min(xz_min, r1[1]);
         xz_sum += r1[0];
         if (n_ub > 0) {
             map(n_ub, [=] __device__(int64_t i) { Dup[i] = zup[i] / xup[i]; Dp[i] += Dup[i]; });
             auto ru = reduce<1>(n_ub, [=] __device__(int64_t i, double* v) { v[0] = zup[i] * xup[i]; });
             auto rdu = reduce<1>(n_ub, [=] __device__(int64_t i, double* v) { v[0] = Dup[i]; });
-            if (rdu[1] <= 0.0) { status = HELP_HSD_STALLED; return; }
+            if (rdu[1] <= 0.0) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_NEGATIVE; return; }
             xz_min = std::min(xz_min, ru[1]);
             xz_sum += ru[0];
         }
@@ -598,6 +614,8 @@ struct HsdGpu {
                 gamma = std::max(1e-6, gamma);
             } else {
                 status = HELP_HSD_STALLED;
+                stall_reason = HELP_STALL_DIRECTION;
+                newton_ratio = (rpn + rdn + rgn) / sum_r_norm;
                 break;
             }
             eta = 1.0 - gamma;
@@ -647,6 +665,8 @@ extern "C" int help_hsd_solve_gpu(const help_lp* lp, const help_backend* be, int
         HsdGpu g(lp, be, h);
         it = g.solve_all(max_iter, logf, user);
         res->status = g.status;
+        res->stall_reason = g.stall_reason;
+        res->newton_ratio = g.newton_ratio;
         res->primal_obj = g.pobj;
         res->dual_obj = g.dobj;
         res->rel_gap = g.rel_gap;

@@ -20,6 +20,12 @@ class GpuBackend:
         if st != L.HELP_OK:
             raise HelpError(st, "help_gpu_backend_new failed")
 
+    def solve_stats(self):
+        """(solves, FP64 refinement iterations, largest final relative residual)"""
+        n, it, rel = C.c_int64(), C.c_int64(), C.c_double()
+        L.lib().help_gpu_backend_solve_stats(C.byref(self.table), C.byref(n), C.byref(it), C.byref(rel))
+        return n.value, it.value, rel.value
+
     def close(self):
         if self.table.ctx:
             L.lib().help_gpu_backend_free(C.byref(self.table))

@@ -1,0 +1,73 @@
+"""Small fixed workloads for profiler runs (kernel traces, hardware counters).
+
+    python -m help_amd.prof_workload solve --config 1 --renews 1 --solves 3
+        construct the backend on a benchmark instance, then renew and solve
+        with a fixed log-uniform d (the factor and solve kernels);
+    python -m help_amd.prof_workload copy --mib 256
+        one device-to-device copy of known size: the reference that scales
+        the FETCH_SIZE / WRITE_SIZE counters to bytes.
+"""
+import argparse
+import sys
+import time
+
+import numpy as np
+
+
+def _solve(args):
+    from . import _lib as L
+    from . import instances
+    from .backend import LinearEquation
+    inst = instances.by_config(args.config)
+    mode = L.MODE_DIRECT_FP32 if args.mode == "fp32" else L.MODE_DIRECT_FP64
+    be = LinearEquation(mode=mode)
+    be.construct(inst)
+    rng = np.random.default_rng(args.seed)
+    d = np.exp(rng.uniform(-args.decades, args.decades, inst.n) * np.log(10.0))
+    b1 = rng.standard_normal(inst.n)
+    b2 = rng.standard_normal(inst.m)
+    for k in range(args.renews):
+        t = time.time()
+        be.renew(d)
+        L.lib().help_device_synchronize()
+        print(f"renew {k}: {1e3 * (time.time() - t):.2f} ms", flush=True)
+    for k in range(args.solves):
+        t = time.time()
+        be.solve(d, b1, b2)
+        print(f"solve {k}: {1e3 * (time.time() - t):.2f} ms", flush=True)
+    be.destruct()
+
+
+def _copy(args):
+    import torch
+    n = args.mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copied {args.mib} MiB", flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m help_amd.prof_workload")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("solve")
+    s.add_argument("--config", type=int, default=1)
+    s.add_argument("--mode", default="fp64", choices=["fp64", "fp32"])
+    s.add_argument("--renews", type=int, default=1)
+    s.add_argument("--solves", type=int, default=3)
+    s.add_argument("--decades", type=float, default=2.0)
+    s.add_argument("--seed", type=int, default=0)
+    c = sub.add_parser("copy")
+    c.add_argument("--mib", type=int, default=256)
+    args = ap.parse_args(argv)
+    if args.cmd == "solve":
+        _solve(args)
+    else:
+        _copy(args)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

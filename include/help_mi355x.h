@@ -260,7 +260,14 @@ typedef struct help_hsd_result {
     double  seconds_step;        /* driver: ratio tests and step update           */
     int32_t n_renew, n_solve;
     int32_t backend_error;   /* first nonzero backend status, 0 if none          */
+    int32_t stall_reason;    /* HELP_STALL_*: why status is STALLED               */
+    double  newton_ratio;    /* at an inaccurate-direction stall: (r_p + r_d + r_g)
+                                of the affine direction / the residual sum it had
+                                to fall below 0.1 of (HSDSolver.cpp:469-471)      */
 } help_hsd_result;
+#define HELP_STALL_NONE        0
+#define HELP_STALL_NEGATIVE    1   /* x_j or z_j not positive (HSDSolver.cpp:63-66,77-80) */
+#define HELP_STALL_DIRECTION   2   /* inaccurate Newton direction (HSDSolver.cpp:814-818) */
 
 /* Optional per-iteration log: called after each Newton-parameter update
  * with (iter, pobj, dobj, gap, pinf, dinf). */
@@ -296,6 +303,11 @@ help_gpu* help_gpu_backend_handle(const help_backend* be);
  * opts.profile = 1), plus its live handle. */
 int help_gpu_backend_kernel_stats(const help_backend* be, int32_t kind, double* ms,
                                   int64_t* launches, double* work);
+/* Solve accounting over the backend's lifetime: solves, FP64 refinement
+ * (factor-preconditioned CG) iterations, and the largest final relative
+ * residual ||r - (A D^-1 A^T + beta I) x2|| / ||r||. */
+int help_gpu_backend_solve_stats(const help_backend* be, int64_t* n_solve, int64_t* refine_steps,
+                                 double* max_rel_residual);
 
 #ifdef __cplusplus
 }
