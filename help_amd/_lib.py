@@ -135,6 +135,7 @@ SIGNATURES = {
                                                 C.POINTER(C.c_int64), _D]),
     "help_gpu_backend_solve_stats": (C.c_int, [C.POINTER(Backend), C.POINTER(C.c_int64),
                                                C.POINTER(C.c_int64), _D]),
+    "help_gpu_pcg_factor": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D, _D, _D]),
     "help_gpu_profile": (C.c_int, [_P, C.c_int32]),
     "help_gpu_kernel_stats": (C.c_int, [_P, C.c_int32, _D, C.POINTER(C.c_int64), _D,
                                         C.POINTER(C.c_char_p)]),
@@ -142,9 +143,11 @@ SIGNATURES = {
 }
 
 KERNEL_NAMES = ["k_small_fronts", "k_extend_add", "k_potrf_diag", "k_trsm_rows", "k_gemm", "k_fwd",
-                "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub"]
+                "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub", "pcg_build",
+                "pcg_solve"]
 # work unit of each kernel class: flops for the factor kernels, HBM bytes otherwise
-KERNEL_BOUND = ["mfma", "hbm", "mfma", "mfma", "mfma", "hbm", "hbm", "hbm", "hbm", "hbm", "hbm", "hbm"]
+KERNEL_BOUND = ["mfma", "hbm", "mfma", "mfma", "mfma", "hbm", "hbm", "hbm", "hbm", "hbm", "hbm", "hbm",
+                "hbm", "hbm"]
 
 _lib = None
 

@@ -134,6 +134,20 @@ class LinearEquation:
                                                 L.i32ptr(tl)), "supernodes")
         return sf, nr, lv, tl
 
+    def pcg_factor(self):
+        """PCG mode: (perm, D, S, L) of the partial-Cholesky preconditioner,
+        L as an m x K array by original row."""
+        K = C.c_int32()
+        self._check(L.lib().help_gpu_pcg_factor(self.h, C.byref(K), None, None, None, None), "pcg_factor")
+        k = K.value
+        perm = np.empty(self.m, np.int32)
+        D = np.empty(max(k, 1))
+        S = np.empty(self.m)
+        Lf = np.empty(max(self.m * k, 1))
+        self._check(L.lib().help_gpu_pcg_factor(self.h, C.byref(K), L.i32ptr(perm), L.dptr(D), L.dptr(S),
+                                                L.dptr(Lf)), "pcg_factor")
+        return perm, D[:k], S, Lf[:self.m * k].reshape(k, self.m).T.copy()
+
     def adat_pattern(self):
         info = self.symbolic_info()
         rp = np.empty(self.m + 1, np.int64)

@@ -73,6 +73,7 @@ struct ProfScope {
     int32_t kind;
     double work;
     hipEvent_t b = nullptr;
+    size_t idx = 0;
     ProfScope(help_gpu* h_, int32_t k, double w) : h(h_), kind(k), work(w) {
         if (!h->prof) return;
         size_t used = h->pending.size() * 2;
@@ -84,7 +85,12 @@ struct ProfScope {
         hipEvent_t a = h->evpool[used];
         b = h->evpool[used + 1];
         HIPCHK(hipEventRecord(a, h->st));
+        idx = h->pending.size();
         h->pending.push_back({kind, work, a, b});
+    }
+    // work known only after the launches (iterative kernels)
+    void set_work(double w) {
+        if (h->prof && b) h->pending[idx].work = w;
     }
     ~ProfScope() {
         if (h->prof && b) (void)hipEventRecord(b, h->st);
@@ -278,20 +284,32 @@ void build_schedules(help_gpu* h) {
     h->work_backsub = 12.0 * nnz + 8.0 * (n + 1) + 8.0 * m + 24.0 * n;
 }
 
-void setup_device(help_gpu* h) {
+// stream, events, A in CSC and CSR, the boundary and work vectors
+void setup_device_common(help_gpu* h) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         throw HipError("no HIP device visible", HELP_ERR_NOGPU);
     HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
     for (auto& e : h->ev) HIPCHK(hipEventCreate(&e));
     hipStream_t st = h->st;
-    const Supernodal& S = h->S;
     h->d_colptr.upload(h->A.colptr, st);
     h->d_rowind.upload(h->A.rowind, st);
     h->d_val.upload(h->A.val, st);
     h->d_rowptr.upload(h->R.rowptr, st);
     h->d_colind.upload(h->csrcol, st);
     h->d_csrval.upload(h->csrval, st);
+    const int32_t m = h->m, n = h->n;
+    h->d_d.alloc(n); h->d_dinv.alloc(n); h->d_b1.alloc(n); h->d_x1.alloc(n); h->d_tn.alloc(n);
+    h->d_b2.alloc(m); h->d_x2.alloc(m); h->d_rp.alloc(m); h->d_t.alloc(m); h->d_r.alloc(m); h->d_c.alloc(m);
+    h->d_z.alloc(m); h->d_p.alloc(m); h->d_q.alloc(m);
+    h->d_part.alloc(kDotPartials); h->d_scal.alloc(8);
+    h->d_err.alloc(1);
+}
+
+// symbolic structures, schedules and factor storage of the direct modes
+void setup_device_factor(help_gpu* h) {
+    hipStream_t st = h->st;
+    const Supernodal& S = h->S;
     h->d_pairptr.upload(h->P.pairptr, st);
     h->d_pair_p.upload(h->P.pair_p, st);
     h->d_pair_q.upload(h->P.pair_q, st);
@@ -320,12 +338,6 @@ void setup_device(help_gpu* h) {
     h->d_gemm.upload(h->gemm, st);
     h->d_fgemv.upload(h->fgemv, st);
     h->d_bgemv.upload(h->bgemv, st);
-    h->d_err.alloc(1);
-    const int32_t m = h->m, n = h->n;
-    h->d_d.alloc(n); h->d_dinv.alloc(n); h->d_b1.alloc(n); h->d_x1.alloc(n); h->d_tn.alloc(n);
-    h->d_b2.alloc(m); h->d_x2.alloc(m); h->d_rp.alloc(m); h->d_t.alloc(m); h->d_r.alloc(m); h->d_c.alloc(m);
-    h->d_z.alloc(m); h->d_p.alloc(m); h->d_q.alloc(m);
-    h->d_part.alloc(kDotPartials); h->d_scal.alloc(8);
     h->d_W.alloc(std::max<int64_t>(1, S.reloff[S.nsuper]));
     int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
     int64_t nd = h->dinvoff[S.nsuper];
@@ -344,7 +356,13 @@ void setup_device(help_gpu* h) {
     y.rowoff = h->d_rowoff.p; y.rowidx = h->d_rowidx.p; y.reloff = h->d_reloff.p;
     y.relmap = h->d_relmap.p; y.childptr = h->d_childptr.p; y.child = h->d_child.p;
     y.dinvoff = h->d_dinvoff.p;
-    HIPCHK(hipStreamSynchronize(st));
+}
+
+void setup_device(help_gpu* h) {
+    setup_device_common(h);
+    if (h->use_pcg) help::pcg_setup(h);
+    else setup_device_factor(h);
+    HIPCHK(hipStreamSynchronize(h->st));
     h->device = true;
 }
 
@@ -518,21 +536,34 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
         if (nnz > INT32_MAX) throw std::runtime_error("nnz exceeds the int32 index range");
         h->m = m; h->n = n; h->nnz = nnz;
         h->use_fp32 = h->opts.mode == HELP_MODE_DIRECT_FP32;
-        if (h->opts.mode == HELP_MODE_PCG || h->opts.mode == HELP_MODE_PCG_LEARNED)
-            throw std::runtime_error("PCG modes are not available in this build");
+        h->use_pcg = h->opts.mode == HELP_MODE_PCG;
+        if (h->opts.mode == HELP_MODE_PCG_LEARNED)
+            throw std::runtime_error("the learned-warm-start mode is not available in this build");
+        if (h->opts.mode < HELP_MODE_DIRECT_FP64 || h->opts.mode > HELP_MODE_PCG_LEARNED)
+            throw std::runtime_error("unknown mode");
         h->A.m = m; h->A.n = n;
         h->A.colptr.assign(colptr, colptr + n + 1);
         h->A.rowind.assign(rowind, rowind + nnz);
         h->A.val.assign(val, val + nnz);
         h->R = csc_to_csr(h->A);
-        h->P = adat_pattern(h->A, h->R, true);
-        std::vector<int64_t> adjptr;
-        std::vector<int32_t> adj;
-        adat_adjacency(m, h->P, adjptr, adj);
-        RelaxParams rp;
-        h->S = analyze(m, adjptr, adj, {}, rp);
-        build_assembly_maps(h);
-        build_schedules(h);
+        if (h->use_pcg) {
+            // matrix-free: no ADA^T pattern, ordering or factor
+            h->csrcol = h->R.colind;
+            h->csrval.resize(h->nnz);
+            for (int64_t t = 0; t < h->nnz; ++t) h->csrval[t] = h->A.val[h->R.cscpos[t]];
+            const double nnz_d = (double)nnz, md = m, nd = n;
+            h->work_rhs = 12.0 * nnz_d + 16.0 * nd + 16.0 * md;
+            h->work_backsub = 12.0 * nnz_d + 8.0 * (nd + 1) + 8.0 * md + 24.0 * nd;
+        } else {
+            h->P = adat_pattern(h->A, h->R, true);
+            std::vector<int64_t> adjptr;
+            std::vector<int32_t> adj;
+            adat_adjacency(m, h->P, adjptr, adj);
+            RelaxParams rp;
+            h->S = analyze(m, adjptr, adj, {}, rp);
+            build_assembly_maps(h);
+            build_schedules(h);
+        }
         if (!h->opts.symbolic_only) setup_device(h);
         h->prof = h->opts.profile != 0;
     } catch (const std::exception& e) {
@@ -564,6 +595,23 @@ int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* rs) {
         HIPCHK(hipEventRecord(h->ev[0], st));
         launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
         HIPCHK(hipMemsetAsync(h->d_err.p, 0, sizeof(int32_t), st));
+        if (h->use_pcg) {
+            HIPCHK(hipEventRecord(h->ev[1], st));
+            {
+                ProfScope ps(h, HELP_K_PCG_BUILD, help::pcg_build_bytes(h));
+                help::pcg_renew(h);
+            }
+            HIPCHK(hipEventRecord(h->ev[2], st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (h->prof) collect_prof(h);
+            h->factored = true;
+            if (rs) {
+                std::memset(rs, 0, sizeof(*rs));
+                rs->ms_assemble = elapsed_ms(h->ev[0], h->ev[1]);
+                rs->ms_precond = elapsed_ms(h->ev[1], h->ev[2]);
+            }
+            return HELP_OK;
+        }
         const Supernodal& S = h->S;
         int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
         int64_t nslot = (int64_t)h->P.colind.size();
@@ -620,14 +668,32 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
         HIPCHK(hipMemcpyAsync(h->d_b1.p, b1, sizeof(double) * n, hipMemcpyDefault, st));
         HIPCHK(hipMemcpyAsync(h->d_b2.p, b2, sizeof(double) * m, hipMemcpyDefault, st));
         HIPCHK(hipEventRecord(h->ev[0], st));
-        // r = A (b1 ./ d) - b2, permuted
+        // r = A (b1 ./ d) - b2, permuted (direct modes) or in the original order (PCG)
         {
             ProfScope ps(h, HELP_K_RHS, h->work_rhs);
-            launch_rhs(m, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p, h->d_d.p, h->d_b2.p,
-                       h->d_iperm.p, h->d_rp.p, st);
+            if (h->use_pcg) {
+                launch_vdiv(n, h->d_b1.p, h->d_d.p, h->d_tn.p, st);
+                launch_csr_split(h->pcg_rows(), h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_tn.p,
+                                 h->d_rp.p, false, st);
+                launch_sub(m, h->d_rp.p, h->d_b2.p, h->d_rp.p, st);
+            } else {
+                launch_rhs(m, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p, h->d_d.p, h->d_b2.p,
+                           h->d_iperm.p, h->d_rp.p, st);
+            }
         }
         HIPCHK(hipEventRecord(h->ev[1], st));
-        {
+        if (h->use_pcg) {
+            // matrix-free PCG on A D^-1 A^T (UsePCG_CPU.cpp:91-178), original row order
+            double rel = 0;
+            int it;
+            {
+                ProfScope ps(h, HELP_K_PCG_SOLVE, 0.0);
+                it = help::pcg_solve(h, h->d_rp.p, nullptr, h->d_x2.p, rel);
+                ps.set_work(help::pcg_iteration_bytes(h) * (it + 1));
+            }
+            loc.pcg_iters = it;
+            loc.rel_residual = rel;
+        } else {
             // the factor's solution, refined by FP64 conjugate gradients on
             // (A D^-1 A^T + beta I) preconditioned with the same factor
             double rel = 0;
@@ -659,6 +725,7 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
 
 int help_gpu_symbolic_info(const help_gpu* h, help_symbolic_info* info) {
     if (!h || !info) return HELP_ERR_ARG;
+    if (h->use_pcg) return HELP_ERR_STATE;   // no ordering or factor in PCG mode
     const Supernodal& S = h->S;
     std::memset(info, 0, sizeof(*info));
     info->nnz_adat_upper = (int64_t)h->P.colind.size();
@@ -676,6 +743,7 @@ int help_gpu_symbolic_info(const help_gpu* h, help_symbolic_info* info) {
 
 int help_gpu_symbolic_arrays(const help_gpu* h, int32_t* perm, int32_t* parent, int32_t* colcount) {
     if (!h) return HELP_ERR_ARG;
+    if (h->use_pcg) return HELP_ERR_STATE;   // no ordering or factor in PCG mode
     if (perm) std::memcpy(perm, h->S.perm.data(), sizeof(int32_t) * h->m);
     if (parent) std::memcpy(parent, h->S.parent.data(), sizeof(int32_t) * h->m);
     if (colcount) std::memcpy(colcount, h->S.colcount.data(), sizeof(int32_t) * h->m);
@@ -684,6 +752,7 @@ int help_gpu_symbolic_arrays(const help_gpu* h, int32_t* perm, int32_t* parent, 
 
 int help_gpu_adat_pattern(const help_gpu* h, int64_t* rowptr, int32_t* colind) {
     if (!h) return HELP_ERR_ARG;
+    if (h->use_pcg) return HELP_ERR_STATE;   // no ordering or factor in PCG mode
     if (rowptr) std::memcpy(rowptr, h->P.rowptr.data(), sizeof(int64_t) * (h->m + 1));
     if (colind) std::memcpy(colind, h->P.colind.data(), sizeof(int32_t) * h->P.colind.size());
     return HELP_OK;
@@ -691,7 +760,7 @@ int help_gpu_adat_pattern(const help_gpu* h, int64_t* rowptr, int32_t* colind) {
 
 int help_gpu_adat_values(help_gpu* h, const double* d, double* vals) {
     if (!h || !d || !vals) return HELP_ERR_ARG;
-    if (!h->device) { h->err = "adat_values: no device state"; return HELP_ERR_STATE; }
+    if (!h->device || h->use_pcg) { h->err = "adat_values: no device state or PCG mode"; return HELP_ERR_STATE; }
     try {
         hipStream_t st = h->st;
         int64_t nslot = (int64_t)h->P.colind.size();
@@ -704,6 +773,17 @@ int help_gpu_adat_values(help_gpu* h, const double* d, double* vals) {
         HIPCHK(hipMemcpyAsync(vals, h->d_slotvals.p, sizeof(double) * nslot, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         h->d_slotvals.free();
+        return HELP_OK;
+    } catch (const std::exception& e) {
+        return fail(h, e);
+    }
+}
+
+int help_gpu_pcg_factor(help_gpu* h, int32_t* K, int32_t* perm, double* D, double* S, double* L) {
+    if (!h) return HELP_ERR_ARG;
+    if (!h->use_pcg || !h->device || !h->factored) { h->err = "pcg_factor: no PCG preconditioner"; return HELP_ERR_STATE; }
+    try {
+        help::pcg_download(h, K, perm, D, S, L);
         return HELP_OK;
     } catch (const std::exception& e) {
         return fail(h, e);
@@ -735,7 +815,7 @@ int help_gpu_spmv(help_gpu* h, int32_t trans, const double* x, double* y) {
 
 int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows, double* vals, int32_t* nr) {
     if (!h || !rows || !vals || !nr || j < 0 || j >= h->m) return HELP_ERR_ARG;
-    if (!h->device || !h->factored) { h->err = "factor_column: no factor"; return HELP_ERR_STATE; }
+    if (!h->device || !h->factored || h->use_pcg) { h->err = "factor_column: no factor"; return HELP_ERR_STATE; }
     try {
         const Supernodal& S = h->S;
         int32_t s = S.col2sn[j];
@@ -782,6 +862,7 @@ int help_device_synchronize(void) {
 int help_gpu_supernodes(const help_gpu* h, int32_t* sfirst, int32_t* snrow, int32_t* level,
                         int32_t* tiled) {
     if (!h) return HELP_ERR_ARG;
+    if (h->use_pcg) return HELP_ERR_STATE;
     const Supernodal& S = h->S;
     if (sfirst) std::memcpy(sfirst, S.sfirst.data(), sizeof(int32_t) * (S.nsuper + 1));
     if (snrow) std::memcpy(snrow, S.snrow.data(), sizeof(int32_t) * S.nsuper);

@@ -10,6 +10,7 @@
 
 #include "device.h"
 #include "help_mi355x.h"
+#include "pcg.h"
 #include "symbolic.h"
 
 namespace help {
@@ -66,7 +67,8 @@ struct Op {
 
 inline const char* const kKindName[HELP_K_COUNT] = {
     "k_small_fronts", "k_extend_add", "k_potrf_diag", "k_trsm_rows", "k_gemm",
-    "k_fwd", "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub"};
+    "k_fwd", "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub",
+    "pcg_build", "pcg_solve"};
 
 inline double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -136,12 +138,28 @@ struct help_gpu {
     std::vector<hipEvent_t> evpool;
     std::vector<Pending> pending;
     double work_adat = 0, work_rhs = 0, work_backsub = 0;
+    // matrix-free PCG mode (pcg.hip)
+    help::PcgState* pcg = nullptr;
+    bool use_pcg = false;
+    const help::RowSplitDev& pcg_rows() const { return help::pcg_row_split(pcg); }
 
     ~help_gpu() {
         if (st) { (void)hipStreamSynchronize(st); }
+        help::pcg_free(pcg);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : evpool) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
 };
+
+namespace help {
+// PCG mode (pcg.hip)
+void pcg_setup(help_gpu* h);
+void pcg_renew(help_gpu* h);
+int pcg_solve(help_gpu* h, const double* rhs, const double* x0, double* x2, double& rel);
+// algorithmic HBM bytes: one partial-Cholesky build, one CG iteration
+double pcg_build_bytes(const help_gpu* h);
+double pcg_iteration_bytes(const help_gpu* h);
+void pcg_download(help_gpu* h, int32_t* K, int32_t* perm, double* D, double* S, double* L);
+}  // namespace help

@@ -1,0 +1,409 @@
+// Kernels of the matrix-free PCG path (HELP PCG.cpp:38-395).
+//
+// * Row-split CSR products: A has rows of wildly different lengths (Zipf row
+//   popularity gives rows with millions of nonzeros), so rows up to
+//   kHeavyRow nonzeros take a thread and longer rows are cut into chunks of
+//   kRowChunk, one workgroup each, combined in chunk order: deterministic.
+// * Greedy partial Cholesky (PCG.cpp:38-103): per pivot an argmax over the
+//   Schur diagonal, the pivot row of A scattered into an n-vector, one
+//   product with A for the pivot's column of ADA^T, and a left-looking
+//   column update against the previous columns (column-major m x K factor,
+//   so the thread-per-row update reads coalesced).
+// * Preconditioner apply (PCG.cpp:183-196) with the factor in FP32: two
+//   passes over the m x K factor (a product and a transposed product).
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "pcg.h"
+
+namespace help {
+
+namespace {
+
+constexpr int kB = 256;
+inline int grid_for(int64_t n) { return (int)std::max<int64_t>(1, (n + kB - 1) / kB); }
+
+__device__ inline double block_sum(double v, double* red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    const double s = red[0];
+    __syncthreads();
+    return s;
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// ------------------------------------------------------ row-split products
+template <bool SQ>
+__global__ __launch_bounds__(256) void k_csr_light(int32_t m, const int64_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ colind,
+                                                   const double* __restrict__ val,
+                                                   const double* __restrict__ x, double* __restrict__ y) {
+    const int32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    const int64_t b = rowptr[i], e = rowptr[i + 1];
+    if (e - b > kHeavyRow) return;
+    double acc = 0.0;
+    for (int64_t p = b; p < e; ++p) {
+        const double v = val[p];
+        acc += (SQ ? v * v : v) * x[colind[p]];
+    }
+    y[i] = acc;
+}
+
+template <bool SQ>
+__global__ __launch_bounds__(256) void k_csr_chunk(const RowChunk* __restrict__ chunks,
+                                                   const int32_t* __restrict__ colind,
+                                                   const double* __restrict__ val,
+                                                   const double* __restrict__ x, double* __restrict__ part) {
+    __shared__ double red[kB];
+    const RowChunk c = chunks[blockIdx.x];
+    double acc = 0.0;
+    for (int64_t p = c.b + threadIdx.x; p < c.e; p += kB) {
+        const double v = val[p];
+        acc += (SQ ? v * v : v) * x[colind[p]];
+    }
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_csr_heavy(int32_t nheavy, const int32_t* __restrict__ hrow, const int32_t* __restrict__ hptr,
+                            const double* __restrict__ part, double* __restrict__ y) {
+    const int32_t h = blockIdx.x * kB + threadIdx.x;
+    if (h >= nheavy) return;
+    double acc = 0.0;
+    for (int32_t c = hptr[h]; c < hptr[h + 1]; ++c) acc += part[c];
+    y[hrow[h]] = acc;
+}
+
+__global__ void k_csc_ts(int32_t n, const int64_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
+                         const double* __restrict__ val, const double* __restrict__ dinv,
+                         const double* __restrict__ x, double* __restrict__ t) {
+    const int32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= n) return;
+    double acc = 0.0;
+    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) acc += val[p] * x[rowind[p]];
+    t[j] = acc * dinv[j];
+}
+
+// ---------------------------------------------------- partial Cholesky build
+__device__ inline bool better(double v, int32_t i, double bv, int32_t bi) {
+    return v > bv || (v == bv && i < bi);
+}
+
+__global__ __launch_bounds__(256) void k_pc_argmax_part(int32_t m, int32_t k, const int32_t* __restrict__ perm,
+                                                        const double* __restrict__ S, double* __restrict__ pval,
+                                                        int32_t* __restrict__ pidx) {
+    __shared__ double sv[kB];
+    __shared__ int32_t si[kB];
+    const int t = threadIdx.x;
+    double bv = -HUGE_VAL;
+    int32_t bi = INT_MAX;
+    for (int64_t i = (int64_t)k + (int64_t)blockIdx.x * kB + t; i < m; i += (int64_t)gridDim.x * kB) {
+        const double v = S[perm[i]];
+        if (better(v, (int32_t)i, bv, bi)) { bv = v; bi = (int32_t)i; }
+    }
+    sv[t] = bv;
+    si[t] = bi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w && better(sv[t + w], si[t + w], sv[t], si[t])) { sv[t] = sv[t + w]; si[t] = si[t + w]; }
+        __syncthreads();
+    }
+    if (t == 0) { pval[blockIdx.x] = sv[0]; pidx[blockIdx.x] = si[0]; }
+}
+
+__global__ __launch_bounds__(256) void k_pc_pick(int32_t nb, int32_t k, const double* __restrict__ pval,
+                                                 const int32_t* __restrict__ pidx, int32_t* __restrict__ perm,
+                                                 int32_t* __restrict__ pos, int32_t* __restrict__ col_out) {
+    __shared__ double sv[kB];
+    __shared__ int32_t si[kB];
+    const int t = threadIdx.x;
+    double bv = -HUGE_VAL;
+    int32_t bi = INT_MAX;
+    for (int b = t; b < nb; b += kB)
+        if (better(pval[b], pidx[b], bv, bi)) { bv = pval[b]; bi = pidx[b]; }
+    sv[t] = bv;
+    si[t] = bi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w && better(sv[t + w], si[t + w], sv[t], si[t])) { sv[t] = sv[t + w]; si[t] = si[t + w]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        int32_t cm = si[0] == INT_MAX ? k : si[0];
+        const int32_t a = perm[k], b = perm[cm];
+        perm[k] = b;
+        perm[cm] = a;
+        pos[a] = cm;
+        pos[b] = k;
+        *col_out = b;
+    }
+}
+
+__global__ void k_pc_scatter(const int32_t* __restrict__ colp, const int64_t* __restrict__ rowptr,
+                             const int32_t* __restrict__ colind, const double* __restrict__ val,
+                             const double* __restrict__ dinv, double* __restrict__ sc, int clear) {
+    const int32_t col = *colp;
+    const int64_t b = rowptr[col], e = rowptr[col + 1];
+    for (int64_t p = b + (int64_t)blockIdx.x * kB + threadIdx.x; p < e; p += (int64_t)gridDim.x * kB) {
+        const int32_t j = colind[p];
+        sc[j] = clear ? 0.0 : val[p] * dinv[j];
+    }
+}
+
+// w_r = D_r L(col, r) for r < k and D_k = g(col) + delta - sum_r w_r L(col, r)
+__global__ __launch_bounds__(256) void k_pc_pivot(int32_t m, int32_t k, const int32_t* __restrict__ colp,
+                                                  const double* __restrict__ g, const double* __restrict__ L,
+                                                  double delta, double* __restrict__ D, double* __restrict__ w) {
+    __shared__ double red[kB];
+    const int32_t col = *colp;
+    double acc = 0.0;
+    for (int32_t r = threadIdx.x; r < k; r += kB) {
+        const double l = L[(int64_t)r * m + col];
+        const double wr = D[r] * l;
+        w[r] = wr;
+        acc += wr * l;
+    }
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) D[k] = (g[col] + delta) - s;
+}
+
+// column k of L over all rows: 1 at the pivot, 0 at earlier pivots, else
+// (g_j - sum_r L(j,r) w_r) / D_k; the Schur diagonal loses D_k L(j,k)^2
+__global__ __launch_bounds__(256) void k_pc_column(int32_t m, int32_t k, const int32_t* __restrict__ colp,
+                                                   const int32_t* __restrict__ pos, const double* __restrict__ g,
+                                                   const double* __restrict__ wg, const double* __restrict__ D,
+                                                   double* __restrict__ L, double* __restrict__ S) {
+    __shared__ double w[kPcgMaxK];
+    for (int r = threadIdx.x; r < k; r += kB) w[r] = wg[r];
+    __syncthreads();
+    const int32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= m) return;
+    const int32_t col = *colp;
+    double* Lk = L + (int64_t)k * m;
+    if (j == col) { Lk[j] = 1.0; return; }
+    if (pos[j] < k) { Lk[j] = 0.0; return; }
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int32_t r = 0;
+    for (; r + 3 < k; r += 4) {
+        a0 += L[(int64_t)r * m + j] * w[r];
+        a1 += L[(int64_t)(r + 1) * m + j] * w[r + 1];
+        a2 += L[(int64_t)(r + 2) * m + j] * w[r + 2];
+        a3 += L[(int64_t)(r + 3) * m + j] * w[r + 3];
+    }
+    for (; r < k; ++r) a0 += L[(int64_t)r * m + j] * w[r];
+    const double u = g[j] - ((a0 + a1) + (a2 + a3));
+    const double dk = D[k];
+    const double l = u / dk;
+    Lk[j] = l;
+    S[j] -= dk * l * l;
+}
+
+// L11 (row-major K x K, rows = pivots in order) and X = L11^-1 by columns:
+// thread c solves L11 x = e_c forward (unit diagonal)
+__global__ void k_pc_l11_gather(int32_t m, int32_t K, const int32_t* __restrict__ perm,
+                                const double* __restrict__ L, double* __restrict__ L11) {
+    const int64_t idx = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (idx >= (int64_t)K * K) return;
+    const int32_t i = (int32_t)(idx / K), c = (int32_t)(idx % K);
+    L11[idx] = (c <= i) ? L[(int64_t)c * m + perm[i]] : 0.0;
+}
+
+__global__ void k_pc_l11_inv(int32_t K, const double* __restrict__ L11, double* __restrict__ X) {
+    const int32_t c = blockIdx.x * kB + threadIdx.x;
+    if (c >= K) return;
+    for (int32_t i = 0; i < c; ++i) X[(int64_t)i * K + c] = 0.0;
+    X[(int64_t)c * K + c] = 1.0;
+    for (int32_t i = c + 1; i < K; ++i) {
+        double s = 0.0;
+        for (int32_t t = c; t < i; ++t) s += L11[(int64_t)i * K + t] * X[(int64_t)t * K + c];
+        X[(int64_t)i * K + c] = -s;
+    }
+}
+
+__global__ void k_vdiv(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
+                       double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < n) out[i] = a[i] / b[i];
+}
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (i < n) y[i] = (float)x[i];
+}
+
+// ------------------------------------------------------ preconditioner apply
+// y1 = L11^-1 r(pivots)
+__global__ void k_pca_y1(int32_t K, const int32_t* __restrict__ perm, const double* __restrict__ X,
+                         const double* __restrict__ r, double* __restrict__ y1) {
+    const int32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= K) return;
+    double s = 0.0;
+    for (int32_t c = 0; c <= i; ++c) s += X[(int64_t)i * K + c] * r[perm[c]];
+    y1[i] = s;
+}
+
+// z_j = (r_j - sum_c L(j,c) y1_c) / S_j off the pivots, 0 on them
+__global__ __launch_bounds__(256) void k_pca_t2(int32_t m, int32_t K, const int32_t* __restrict__ pos,
+                                                const float* __restrict__ L32, const double* __restrict__ y1g,
+                                                const double* __restrict__ S, const double* __restrict__ r,
+                                                double* __restrict__ z) {
+    __shared__ double y1[kPcgMaxK];
+    for (int c = threadIdx.x; c < K; c += kB) y1[c] = y1g[c];
+    __syncthreads();
+    const int32_t j = blockIdx.x * kB + threadIdx.x;
+    if (j >= m) return;
+    if (pos[j] < K) { z[j] = 0.0; return; }
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int32_t c = 0;
+    for (; c + 3 < K; c += 4) {
+        a0 += (double)L32[(int64_t)c * m + j] * y1[c];
+        a1 += (double)L32[(int64_t)(c + 1) * m + j] * y1[c + 1];
+        a2 += (double)L32[(int64_t)(c + 2) * m + j] * y1[c + 2];
+        a3 += (double)L32[(int64_t)(c + 3) * m + j] * y1[c + 3];
+    }
+    for (; c < K; ++c) a0 += (double)L32[(int64_t)c * m + j] * y1[c];
+    z[j] = (r[j] - ((a0 + a1) + (a2 + a3))) / S[j];
+}
+
+// ltpart[c * nb + b] = sum over the kLtRows rows of block b of L(j,c) z_j
+// (z is 0 on the pivots); a wave per column, 32 rows per lane in flight
+__global__ __launch_bounds__(256) void k_pca_lt(int32_t m, int32_t K, const float* __restrict__ L32,
+                                                const double* __restrict__ z, double* __restrict__ ltpart) {
+    __shared__ double zs[kLtRows];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * kLtRows;
+    for (int t = threadIdx.x; t < kLtRows; t += kB) zs[t] = (r0 + t < m) ? z[r0 + t] : 0.0;
+    __syncthreads();
+    const int64_t last = (int64_t)m - 1;
+    for (int c = wave; c < K; c += 4) {
+        const float* Lc = L32 + (int64_t)c * m;
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < kLtRows / 64; ++u) {
+            const int64_t j = r0 + lane + 64 * u;
+            a[u & 3] += (double)Lc[j < m ? j : last] * zs[lane + 64 * u];
+        }
+        const double s = wave_sum((a[0] + a[1]) + (a[2] + a[3]));
+        if (lane == 0) ltpart[(int64_t)c * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// w_c = y1_c / S(pivot c) - sum_b ltpart[c][b]
+__global__ void k_pca_w(int32_t K, int32_t nb, const int32_t* __restrict__ perm, const double* __restrict__ y1,
+                        const double* __restrict__ S, const double* __restrict__ ltpart, double* __restrict__ w) {
+    const int32_t c = blockIdx.x * kB + threadIdx.x;
+    if (c >= K) return;
+    double s = 0.0;
+    for (int32_t b = 0; b < nb; ++b) s += ltpart[(int64_t)c * nb + b];
+    w[c] = y1[c] / S[perm[c]] - s;
+}
+
+// z(pivot i) = sum_{c >= i} X[c][i] w_c   (L11^-T w)
+__global__ void k_pca_final(int32_t K, const int32_t* __restrict__ perm, const double* __restrict__ X,
+                            const double* __restrict__ w, double* __restrict__ z) {
+    const int32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= K) return;
+    double s = 0.0;
+    for (int32_t c = i; c < K; ++c) s += X[(int64_t)c * K + i] * w[c];
+    z[perm[i]] = s;
+}
+
+}  // namespace
+
+void build_row_split(int32_t m, const std::vector<int64_t>& rowptr, std::vector<RowChunk>& chunks,
+                     std::vector<int32_t>& hrow, std::vector<int32_t>& hptr) {
+    chunks.clear();
+    hrow.clear();
+    hptr.assign(1, 0);
+    for (int32_t i = 0; i < m; ++i) {
+        const int64_t b = rowptr[i], e = rowptr[i + 1];
+        if (e - b <= kHeavyRow) continue;
+        hrow.push_back(i);
+        for (int64_t p = b; p < e; p += kRowChunk) chunks.push_back({i, p, std::min(e, p + kRowChunk)});
+        hptr.push_back((int32_t)chunks.size());
+    }
+}
+
+void launch_csr_split(const RowSplitDev& rs, const int64_t* rowptr, const int32_t* colind, const double* val,
+                      const double* x, double* y, bool square, hipStream_t st) {
+    if (rs.m <= 0) return;
+    if (square) {
+        k_csr_light<true><<<grid_for(rs.m), kB, 0, st>>>(rs.m, rowptr, colind, val, x, y);
+        if (rs.nchunk) k_csr_chunk<true><<<rs.nchunk, kB, 0, st>>>(rs.chunks, colind, val, x, rs.part);
+    } else {
+        k_csr_light<false><<<grid_for(rs.m), kB, 0, st>>>(rs.m, rowptr, colind, val, x, y);
+        if (rs.nchunk) k_csr_chunk<false><<<rs.nchunk, kB, 0, st>>>(rs.chunks, colind, val, x, rs.part);
+    }
+    if (rs.nheavy) k_csr_heavy<<<grid_for(rs.nheavy), kB, 0, st>>>(rs.nheavy, rs.hrow, rs.hptr, rs.part, y);
+}
+
+void launch_csc_t_scaled(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
+                         const double* dinv, const double* x, double* t, hipStream_t st) {
+    if (n > 0) k_csc_ts<<<grid_for(n), kB, 0, st>>>(n, colptr, rowind, val, dinv, x, t);
+}
+
+void launch_pc_argmax(int32_t m, int32_t k, const double* S, double* pval, int32_t* pidx, int32_t* perm,
+                      int32_t* pos, int32_t* col_out, hipStream_t st) {
+    const int nb = (int)std::min<int64_t>(kArgmaxBlocks, std::max<int64_t>(1, ((int64_t)m - k + kB - 1) / kB));
+    k_pc_argmax_part<<<nb, kB, 0, st>>>(m, k, perm, S, pval, pidx);
+    k_pc_pick<<<1, kB, 0, st>>>(nb, k, pval, pidx, perm, pos, col_out);
+}
+
+void launch_pc_scatter(const int32_t* col, int32_t max_row_nnz, const int64_t* rowptr, const int32_t* colind,
+                       const double* val, const double* dinv, double* sc, bool clear, hipStream_t st) {
+    const int nb = (int)std::min<int64_t>(2048, std::max<int64_t>(1, ((int64_t)max_row_nnz + kB - 1) / kB));
+    k_pc_scatter<<<nb, kB, 0, st>>>(col, rowptr, colind, val, dinv, sc, clear ? 1 : 0);
+}
+
+void launch_pc_pivot(int32_t m, int32_t k, const int32_t* col, const double* g, const double* L, double delta,
+                     double* D, double* w, hipStream_t st) {
+    k_pc_pivot<<<1, kB, 0, st>>>(m, k, col, g, L, delta, D, w);
+}
+
+void launch_pc_column(int32_t m, int32_t k, const int32_t* col, const int32_t* pos, const double* g,
+                      const double* w, const double* D, double* L, double* S, hipStream_t st) {
+    k_pc_column<<<grid_for(m), kB, 0, st>>>(m, k, col, pos, g, w, D, L, S);
+}
+
+void launch_pc_l11(int32_t m, int32_t K, const int32_t* perm, const double* L, double* L11, double* X,
+                   hipStream_t st) {
+    if (K <= 0) return;
+    k_pc_l11_gather<<<grid_for((int64_t)K * K), kB, 0, st>>>(m, K, perm, L, L11);
+    k_pc_l11_inv<<<grid_for(K), kB, 0, st>>>(K, L11, X);
+}
+
+void launch_vdiv(int64_t n, const double* a, const double* b, double* out, hipStream_t st) {
+    if (n > 0) k_vdiv<<<grid_for(n), kB, 0, st>>>(n, a, b, out);
+}
+
+void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st) {
+    if (n > 0) k_to_f32<<<grid_for(n), kB, 0, st>>>(n, x, y);
+}
+
+void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* pos, const float* L32,
+                     const double* X, const double* S, const double* r, double* y1, double* ltpart, double* z,
+                     hipStream_t st) {
+    // K = 0: z = r / S
+    const int nb = (int)(((int64_t)m + kLtRows - 1) / kLtRows);
+    if (K > 0) k_pca_y1<<<grid_for(K), kB, 0, st>>>(K, perm, X, r, y1);
+    k_pca_t2<<<grid_for(m), kB, 0, st>>>(m, K, pos, L32, y1, S, r, z);
+    if (K > 0) {
+        k_pca_lt<<<nb, kB, 0, st>>>(m, K, L32, z, ltpart);
+        double* w = y1 + kPcgMaxK;
+        k_pca_w<<<grid_for(K), kB, 0, st>>>(K, nb, perm, y1, S, ltpart, w);
+        k_pca_final<<<grid_for(K), kB, 0, st>>>(K, perm, X, w, z);
+    }
+}
+
+}  // namespace help

@@ -178,7 +178,16 @@ int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows,
 #define HELP_K_ADAT          9   /* ADA^T numeric assembly (bytes)          */
 #define HELP_K_RHS          10   /* r = A D^-1 b1 - b2 (bytes)              */
 #define HELP_K_BACKSUB      11   /* x1 = D^-1 (b1 - A^T x2) (bytes)         */
-#define HELP_K_COUNT        12
+#define HELP_K_PCG_BUILD    12   /* PCG mode: partial-Cholesky build per renew */
+#define HELP_K_PCG_SOLVE    13   /* PCG mode: conjugate gradients per solve  */
+#define HELP_K_COUNT        14
+/* PCG mode: the partial-Cholesky preconditioner of the last renew.  K is
+ * the number of pivots; perm[m] the pivot order (pivots first, PCG_PERM),
+ * D[K] the pivots, S[m] the Schur diagonal by row (DIAG_S), L[m*K] the
+ * factor columns, column-major by original row (L[c*m + row]).  Any output
+ * pointer may be NULL. */
+int help_gpu_pcg_factor(help_gpu* h, int32_t* K, int32_t* perm, double* D, double* S, double* L);
+
 int help_gpu_profile(help_gpu* h, int32_t enable);
 int help_gpu_kernel_stats(const help_gpu* h, int32_t kind, double* ms, int64_t* launches,
                           double* work, const char** name);
