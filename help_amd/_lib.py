@@ -25,6 +25,9 @@ MODE_DIRECT_FP64 = 0
 MODE_DIRECT_FP32 = 1
 MODE_PCG = 2
 MODE_PCG_LEARNED = 3
+FAULT_FP32_BREAKDOWN = 1
+FAULT_REFINE_NOCONV = 2
+FAULT_NET_NONFINITE = 4
 
 GEN_RANDOM, GEN_MCF, GEN_STAIRCASE, GEN_ZIPF = 0, 1, 2, 3
 

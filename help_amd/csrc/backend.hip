@@ -304,6 +304,22 @@ void setup_device_common(help_gpu* h) {
     h->d_z.alloc(m); h->d_p.alloc(m); h->d_q.alloc(m);
     h->d_part.alloc(kDotPartials); h->d_scal.alloc(8);
     h->d_err.alloc(1);
+    std::vector<help::RowChunk> chunks;
+    std::vector<int32_t> hrow, hptr;
+    help::build_row_split(m, h->R.rowptr, chunks, hrow, hptr);
+    for (int32_t i = 0; i < m; ++i)
+        h->max_row_nnz = std::max<int32_t>(h->max_row_nnz, (int32_t)(h->R.rowptr[i + 1] - h->R.rowptr[i]));
+    h->d_chunks.upload(chunks, st);
+    h->d_hrow.upload(hrow, st);
+    h->d_hptr.upload(hptr, st);
+    h->d_rpart.alloc(std::max<size_t>(1, chunks.size()));
+    h->rows.m = m;
+    h->rows.nchunk = (int32_t)chunks.size();
+    h->rows.nheavy = (int32_t)hrow.size();
+    h->rows.chunks = h->d_chunks.p;
+    h->rows.hrow = h->d_hrow.p;
+    h->rows.hptr = h->d_hptr.p;
+    h->rows.part = h->d_rpart.p;
 }
 
 // symbolic structures, schedules and factor storage of the direct modes
@@ -342,6 +358,9 @@ void setup_device_factor(help_gpu* h) {
     int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
     int64_t nd = h->dinvoff[S.nsuper];
     if (h->use_fp32) {
+        h->d_slot_row.upload(h->slot_row, st);
+        h->d_slot_col.upload(h->P.colind, st);
+        h->d_scale.alloc(h->m);
         h->d_L32.alloc(std::max<int64_t>(1, nl));
         h->d_U32.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv32.alloc(std::max<int64_t>(1, nd));
@@ -414,7 +433,9 @@ void apply_factor(help_gpu* h, const double* r, double* z) {
     const T* Lf = std::is_same<T, float>::value ? (const T*)h->d_L32.p : (const T*)h->d_L64.p;
     const T* Df = std::is_same<T, float>::value ? (const T*)h->d_Dinv32.p : (const T*)h->d_Dinv64.p;
     launch_scatter_perm(h->m, r, h->d_iperm.p, h->d_rp.p, h->st);
+    if (std::is_same<T, float>::value) launch_scale_perm(h->m, h->d_rp.p, h->d_scale.p, h->d_iperm.p, h->st);
     run_trisolve<T>(h, Lf, Df, h->d_rp.p);
+    if (std::is_same<T, float>::value) launch_scale_perm(h->m, h->d_rp.p, h->d_scale.p, h->d_iperm.p, h->st);
     launch_gather_perm(h->m, h->d_rp.p, h->d_iperm.p, z, h->st);
 }
 
@@ -432,7 +453,9 @@ int pcg_factor(help_gpu* h, double& rel) {
     const T* Lf = std::is_same<T, float>::value ? (const T*)h->d_L32.p : (const T*)h->d_L64.p;
     const T* Df = std::is_same<T, float>::value ? (const T*)h->d_Dinv32.p : (const T*)h->d_Dinv64.p;
     launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, rhs, st);
+    if (std::is_same<T, float>::value) launch_scale_perm(m, h->d_rp.p, h->d_scale.p, h->d_iperm.p, st);
     run_trisolve<T>(h, Lf, Df, h->d_rp.p);
+    if (std::is_same<T, float>::value) launch_scale_perm(m, h->d_rp.p, h->d_scale.p, h->d_iperm.p, st);
     launch_gather_perm(m, h->d_rp.p, h->d_iperm.p, x, st);
     launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
     auto matvec = [&](const double* v, double* out) {
@@ -472,6 +495,56 @@ int pcg_factor(help_gpu* h, double& rel) {
         launch_xpby(m, z, beta, p, st);
     }
     return it;
+}
+
+// FP64 factor storage, allocated on first use in the FP32 mode (after a
+// refinement failure)
+void ensure_fp64_factor(help_gpu* h) {
+    if (h->d_L64.p) return;
+    const Supernodal& S = h->S;
+    h->d_L64.alloc(std::max<int64_t>(1, S.loff[S.nsuper]));
+    h->d_U64.alloc(std::max<int64_t>(1, S.uoff[S.nsuper]));
+    h->d_Dinv64.alloc(std::max<int64_t>(1, h->dinvoff[S.nsuper]));
+}
+
+// ADA^T (+ beta on the diagonal) written into the zeroed factor storage of
+// the requested precision; d_dinv holds 1/d
+void assemble(help_gpu* h, bool f32) {
+    hipStream_t st = h->st;
+    const int32_t m = h->m;
+    const Supernodal& S = h->S;
+    int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
+    int64_t nslot = (int64_t)h->P.colind.size();
+    if (f32) {
+        HIPCHK(hipMemsetAsync(h->d_L32.p, 0, sizeof(float) * nl, st));
+        HIPCHK(hipMemsetAsync(h->d_U32.p, 0, sizeof(float) * std::max<int64_t>(nu, 1), st));
+        // factor S (A D^-1 A^T + beta I) S, S = diag(1/sqrt(diag)): unit
+        // diagonal, entries bounded by 1, no FP32 overflow
+        launch_csr_split(h->rows, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_dinv.p, h->d_t.p, true, st);
+        launch_diag_scale(m, h->d_t.p, h->opts.beta, h->d_scale.p, st);
+        {
+            ProfScope ps(h, HELP_K_ADAT, h->work_adat);
+            launch_adat_scaled_f32(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
+                                   h->d_dinv.p, nslot, h->d_slot_row.p, h->d_slot_col.p, h->d_scale.p,
+                                   h->d_dest.p, h->d_L32.p, st);
+        }
+        launch_add_beta_scaled_f32(h->d_diagpos.p, m, h->opts.beta, h->d_scale.p, h->d_L32.p, st);
+    } else {
+        HIPCHK(hipMemsetAsync(h->d_L64.p, 0, sizeof(double) * nl, st));
+        HIPCHK(hipMemsetAsync(h->d_U64.p, 0, sizeof(double) * std::max<int64_t>(nu, 1), st));
+        {
+            ProfScope ps(h, HELP_K_ADAT, h->work_adat);
+            launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
+                                 h->d_dinv.p, nslot, h->d_dest.p, h->d_L64.p, nullptr, st);
+        }
+        launch_add_beta(h->d_diagpos.p, m, h->opts.beta, h->d_L64.p, nullptr, st);
+    }
+}
+
+void factor(help_gpu* h, bool f32) {
+    if (f32) run_factor<float>(h, h->d_L32.p, h->d_U32.p, h->d_Dinv32.p);
+    else run_factor<double>(h, h->d_L64.p, h->d_U64.p, h->d_Dinv64.p);
+    h->factor_fp32 = f32;
 }
 
 float elapsed_ms(hipEvent_t a, hipEvent_t b) {
@@ -563,6 +636,11 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
             h->S = analyze(m, adjptr, adj, {}, rp);
             build_assembly_maps(h);
             build_schedules(h);
+            if (h->use_fp32) {
+                h->slot_row.resize(h->P.colind.size());
+                for (int32_t i = 0; i < m; ++i)
+                    for (int64_t t = h->P.rowptr[i]; t < h->P.rowptr[i + 1]; ++t) h->slot_row[t] = i;
+            }
         }
         if (!h->opts.symbolic_only) setup_device(h);
         h->prof = h->opts.profile != 0;
@@ -612,31 +690,11 @@ int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* rs) {
             }
             return HELP_OK;
         }
-        const Supernodal& S = h->S;
-        int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
-        int64_t nslot = (int64_t)h->P.colind.size();
-        if (h->use_fp32) {
-            HIPCHK(hipMemsetAsync(h->d_L32.p, 0, sizeof(float) * nl, st));
-            HIPCHK(hipMemsetAsync(h->d_U32.p, 0, sizeof(float) * std::max<int64_t>(nu, 1), st));
-            {
-                ProfScope ps(h, HELP_K_ADAT, h->work_adat);
-                launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
-                                     h->d_dinv.p, nslot, h->d_dest.p, nullptr, h->d_L32.p, st);
-            }
-            launch_add_beta(h->d_diagpos.p, m, h->opts.beta, nullptr, h->d_L32.p, st);
-        } else {
-            HIPCHK(hipMemsetAsync(h->d_L64.p, 0, sizeof(double) * nl, st));
-            HIPCHK(hipMemsetAsync(h->d_U64.p, 0, sizeof(double) * std::max<int64_t>(nu, 1), st));
-            {
-                ProfScope ps(h, HELP_K_ADAT, h->work_adat);
-                launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
-                                     h->d_dinv.p, nslot, h->d_dest.p, h->d_L64.p, nullptr, st);
-            }
-            launch_add_beta(h->d_diagpos.p, m, h->opts.beta, h->d_L64.p, nullptr, st);
-        }
+        const bool f32 = h->use_fp32 && !h->fp32_off;
+        if (!f32) ensure_fp64_factor(h);
+        assemble(h, f32);
         HIPCHK(hipEventRecord(h->ev[1], st));
-        if (h->use_fp32) run_factor<float>(h, h->d_L32.p, h->d_U32.p, h->d_Dinv32.p);
-        else run_factor<double>(h, h->d_L64.p, h->d_U64.p, h->d_Dinv64.p);
+        factor(h, f32);
         HIPCHK(hipEventRecord(h->ev[2], st));
         int32_t nerr = 0;
         HIPCHK(hipMemcpyAsync(&nerr, h->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -673,7 +731,7 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
             ProfScope ps(h, HELP_K_RHS, h->work_rhs);
             if (h->use_pcg) {
                 launch_vdiv(n, h->d_b1.p, h->d_d.p, h->d_tn.p, st);
-                launch_csr_split(h->pcg_rows(), h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_tn.p,
+                launch_csr_split(h->rows, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_tn.p,
                                  h->d_rp.p, false, st);
                 launch_sub(m, h->d_rp.p, h->d_b2.p, h->d_rp.p, st);
             } else {
@@ -697,7 +755,20 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
             // the factor's solution, refined by FP64 conjugate gradients on
             // (A D^-1 A^T + beta I) preconditioned with the same factor
             double rel = 0;
-            int it = h->use_fp32 ? pcg_factor<float>(h, rel) : pcg_factor<double>(h, rel);
+            int it = h->factor_fp32 ? pcg_factor<float>(h, rel) : pcg_factor<double>(h, rel);
+            if (h->factor_fp32 &&
+                (!(rel <= h->opts.refine_tol) ||
+                 (h->opts.fault_flags & (HELP_FAULT_FP32_BREAKDOWN | HELP_FAULT_REFINE_NOCONV)))) {
+                // the FP32 factor no longer preconditions well enough: factor
+                // in FP64 with the same d and keep FP64 for the rest of the run
+                h->fp32_off = true;
+                ensure_fp64_factor(h);
+                assemble(h, false);
+                factor(h, false);
+                launch_scatter_perm(m, h->d_c.p, h->d_iperm.p, h->d_rp.p, st);   // permuted rhs again
+                it += pcg_factor<double>(h, rel);
+                loc.fallback = 1;
+            }
             loc.refine_steps = it;
             loc.rel_residual = rel;
         }
@@ -822,7 +893,7 @@ int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows, double* vals, 
         int32_t c = S.snrow[s], jj = j - S.sfirst[s];
         int64_t off = S.loff[s] + (int64_t)jj * c;
         int32_t cnt = c - jj;
-        if (h->use_fp32) {
+        if (h->factor_fp32) {
             std::vector<float> tmp(cnt);
             HIPCHK(hipMemcpy(tmp.data(), h->d_L32.p + off + jj, sizeof(float) * cnt, hipMemcpyDeviceToHost));
             for (int32_t t = 0; t < cnt; ++t) vals[t] = tmp[t];

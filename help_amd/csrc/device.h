@@ -60,6 +60,15 @@ void launch_adat_assemble(const int64_t* pairptr, const int32_t* pair_p, const i
                           hipStream_t st);
 void launch_add_beta(const int64_t* diagpos, int32_t m, double beta, double* Ld, float* Lf,
                      hipStream_t st);
+// FP32 factor of S (A D^-1 A^T + beta I) S with S = diag(1/sqrt(diag(A D^-1 A^T) + beta))
+void launch_adat_scaled_f32(const int64_t* pairptr, const int32_t* pair_p, const int32_t* pair_q,
+                            const int32_t* colof, const double* val, const double* dinv, int64_t nslots,
+                            const int32_t* slot_row, const int32_t* slot_col, const double* scale,
+                            const int64_t* dest, float* out_f, hipStream_t st);
+void launch_add_beta_scaled_f32(const int64_t* diagpos, int32_t m, double beta, const double* scale,
+                                float* Lf, hipStream_t st);
+void launch_diag_scale(int32_t m, const double* diag, double beta, double* s, hipStream_t st);
+void launch_scale_perm(int32_t m, double* v, const double* s, const int32_t* iperm, hipStream_t st);
 void launch_spmv_csr(int32_t m, const int64_t* rowptr, const int32_t* colind, const double* val,
                      const double* x, double* y, hipStream_t st);
 void launch_spmv_csc_t(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,

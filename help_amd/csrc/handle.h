@@ -111,7 +111,9 @@ struct help_gpu {
     std::string err;
     bool device = false;
     bool factored = false;
-    bool use_fp32 = false;
+    bool use_fp32 = false;     // FP32 mode requested
+    bool factor_fp32 = false;  // precision of the current factor
+    bool fp32_off = false;     // FP32 abandoned after a refinement failure
     int32_t last_tiny = 0;
     // device state
     hipStream_t st = nullptr;
@@ -124,6 +126,10 @@ struct help_gpu {
     DBuf<double> d_val, d_csrval, d_d, d_dinv, d_b1, d_b2, d_x1, d_x2, d_rp, d_W, d_t, d_tn, d_r,
         d_c, d_z, d_p, d_q, d_part, d_scal, d_slotvals;
     DBuf<double> d_L64, d_U64, d_Dinv64;
+    // FP32 mode: symmetric diagonal scaling of the factored matrix
+    std::vector<int32_t> slot_row;
+    DBuf<int32_t> d_slot_row, d_slot_col;
+    DBuf<double> d_scale;
     DBuf<float> d_L32, d_U32, d_Dinv32;
     DBuf<int8_t> d_big;
     DBuf<EaTask> d_ea;
@@ -138,10 +144,15 @@ struct help_gpu {
     std::vector<hipEvent_t> evpool;
     std::vector<Pending> pending;
     double work_adat = 0, work_rhs = 0, work_backsub = 0;
+    // row partition of A for products with heavy rows (all modes)
+    DBuf<help::RowChunk> d_chunks;
+    DBuf<int32_t> d_hrow, d_hptr;
+    DBuf<double> d_rpart;
+    help::RowSplitDev rows;
+    int32_t max_row_nnz = 0;
     // matrix-free PCG mode (pcg.hip)
     help::PcgState* pcg = nullptr;
     bool use_pcg = false;
-    const help::RowSplitDev& pcg_rows() const { return help::pcg_row_split(pcg); }
 
     ~help_gpu() {
         if (st) { (void)hipStreamSynchronize(st); }

@@ -105,7 +105,7 @@ struct HsdGpu {
     // device: iterates and work vectors
     DBuf<double> x, y, z, xu, yu, zu, xf, D, D_u, r_p, r_d, sle1, sle2, bot1, bot1u, bot2, bot2u, dir1,
         dir1u, dir2, dir2u, pp, ppu, qq, qqu, uu, uuu, vv, vvu, hxz, hxuzu, dx, dxu, dy, dyu, dz, dzu,
-        dx_b, dxu_b, dy_b, dyu_b, dz_b, dzu_b, part, out;
+        dx_b, dxu_b, dy_b, dyu_b, dz_b, dzu_b, part, out, tmpm, tmpn;
     double tau = 1, kappa = 1, rho = 1, inf0 = 1;
     double sum_r_norm = 0, pobj = 0, dobj = 0, rel_gap = 0, pinf = 0, dinf = 0, infe = 0;
     int status = HELP_HSD_OK;
@@ -162,6 +162,8 @@ struct HsdGpu {
                                 &hxuzu, &dxu, &dyu, &dzu, &dxu_b, &dyu_b, &dzu_b})
             v->alloc(std::max(n_ub, 1));
         xf.alloc(std::max(n_fr, 1));
+        tmpm.alloc(std::max(m, 1));
+        tmpn.alloc(std::max(n, 1));
         part.alloc((size_t)kRedBlocks * 8);
         out.alloc(8);
         HIPCHK(hipStreamSynchronize(st));
@@ -213,14 +215,11 @@ struct HsdGpu {
         D_g = kappa / tau + rho * tmp / (tau * tau);
         mu = xz_sum / (1.0 + n_lb + n_ub);
         rfval = (1.0 + n_lb + n_ub) * xz_min / xz_sum;
-        // r_p = tau b - A x  (rows gathered in ascending column order)
+        // r_p = tau b - A x  (A x by the row-split product: heavy rows in chunks)
         double* rpp = r_p.p;
-        const int64_t* rptr = rowptr; const int32_t* rcol = colind; const double* rval = csrval;
-        map(m, [=] __device__(int64_t i) {
-            double acc = tau_ * bp[i];
-            for (int64_t t = rptr[i]; t < rptr[i + 1]; ++t) acc -= rval[t] * xp[rcol[t]];
-            rpp[i] = acc;
-        });
+        const double* ax = tmpm.p;
+        launch_csr_split(h->rows, rowptr, colind, csrval, xp, tmpm.p, false, st);
+        map(m, [=] __device__(int64_t i) { rpp[i] = tau_ * bp[i] - ax[i]; });
         // r_d = z - c tau (+yu) (free: -c tau), then += A^T y
         double* rdp = r_d.p;
         const int64_t* cptr = colptr; const int32_t* crow = rowind; const double* cval = cscval;
@@ -331,15 +330,14 @@ struct HsdGpu {
             hrxuzu[i] /= fmax(xup[i], zup[i]);
             if (xup[i] >= zup[i]) d1u[i] += hrxuzu[i];
         });
-        // dir2 = eta r_p - sum_{j: x_j < z_j} A(:,j) hrxz_j   (row gather, ascending j)
-        map(m, [=] __device__(int64_t r) {
-            double acc = eta * rpp[r];
-            for (int64_t t = rptr[r]; t < rptr[r + 1]; ++t) {
-                int32_t j = rcol[t];
-                if (j < nlb && xp[j] < zp[j]) acc -= rval[t] * hrxz[j];
-            }
-            d2[r] = acc;
-        });
+        // dir2 = eta r_p - sum_{j: x_j < z_j} A(:,j) hrxz_j
+        {
+            double* hv = tmpn.p;
+            const double* ah = tmpm.p;
+            map(n, [=] __device__(int64_t j) { hv[j] = (j < nlb && xp[j] < zp[j]) ? hrxz[j] : 0.0; });
+            launch_csr_split(h->rows, rowptr, colind, csrval, hv, tmpm.p, false, st);
+            map(m, [=] __device__(int64_t r) { d2[r] = eta * rpp[r] - ah[r]; });
+        }
         map(n_ub, [=] __device__(int64_t i) {
             if (xp[i] < zp[i]) d2u[i] -= hrxz[i];
             if (xup[i] < zup[i]) d2u[i] -= hrxuzu[i];
@@ -385,11 +383,11 @@ struct HsdGpu {
             d1[j] = acc;
         });
         // dir2 = eta r_p + b dt - A dx
-        map(m, [=] __device__(int64_t r) {
-            double acc = eta * rpp[r] + bp[r] * dt;
-            for (int64_t t = rptr[r]; t < rptr[r + 1]; ++t) acc -= rval[t] * odx[rcol[t]];
-            d2[r] = acc;
-        });
+        {
+            const double* adx = tmpm.p;
+            launch_csr_split(h->rows, rowptr, colind, csrval, odx, tmpm.p, false, st);
+            map(m, [=] __device__(int64_t r) { d2[r] = (eta * rpp[r] + bp[r] * dt) - adx[r]; });
+        }
         map(n_ub, [=] __device__(int64_t i) {
             d1[i] += odyu[i];
             d1u[i] = eta * (zup[i] + yup[i]) + odyu[i] + odzu[i];

@@ -55,6 +55,48 @@ __global__ void k_add_beta(const int64_t* __restrict__ diagpos, int32_t m, doubl
     else Lf[o] = (float)((double)Lf[o] + beta);
 }
 
+// FP32 factor input: the slot value of the symmetrically scaled matrix
+// S (A D^-1 A^T + beta I) S, S = diag(1/sqrt(diag)), so every entry is at
+// most 1 in magnitude and the FP32 range cannot overflow
+__global__ void k_adat_scaled_f32(const int64_t* __restrict__ pairptr, const int32_t* __restrict__ pp,
+                                  const int32_t* __restrict__ pq, const int32_t* __restrict__ colof,
+                                  const double* __restrict__ val, const double* __restrict__ dinv,
+                                  int64_t nslots, const int32_t* __restrict__ slot_row,
+                                  const int32_t* __restrict__ slot_col, const double* __restrict__ scale,
+                                  const int64_t* __restrict__ dest, float* __restrict__ outf) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    double acc = 0.0;
+    int64_t t1 = pairptr[s + 1];
+    for (int64_t t = pairptr[s]; t < t1; ++t) {
+        int32_t p = pp[t], q = pq[t];
+        double vpdj = val[p] * dinv[colof[p]];
+        acc = acc + vpdj * val[q];
+    }
+    outf[dest[s]] = (float)(acc * scale[slot_row[s]] * scale[slot_col[s]]);
+}
+
+__global__ void k_add_beta_scaled_f32(const int64_t* __restrict__ diagpos, int32_t m, double beta,
+                                      const double* __restrict__ scale, float* __restrict__ Lf) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int64_t o = diagpos[i];
+    Lf[o] = (float)((double)Lf[o] + beta * scale[i] * scale[i]);
+}
+
+// s_i = 1 / sqrt(diag_i + beta)
+__global__ void k_diag_scale(int32_t m, const double* __restrict__ diag, double beta, double* __restrict__ s) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) s[i] = 1.0 / sqrt(diag[i] + beta);
+}
+
+// v[iperm[i]] *= s[i]  (a vector in the factor's numbering scaled by S)
+__global__ void k_scale_perm(int32_t m, double* __restrict__ v, const double* __restrict__ s,
+                             const int32_t* __restrict__ iperm) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) v[iperm[i]] *= s[i];
+}
+
 __global__ void k_spmv_csr(int32_t m, const int64_t* __restrict__ rowptr,
                            const int32_t* __restrict__ colind, const double* __restrict__ val,
                            const double* __restrict__ x, double* __restrict__ y) {
@@ -220,6 +262,29 @@ void launch_adat_assemble(const int64_t* pairptr, const int32_t* pair_p, const i
     if (nslots > 0)
         k_adat<<<grid_for(nslots), kBlock, 0, st>>>(pairptr, pair_p, pair_q, colof, val, dinv,
                                                     nslots, dest, out_d, out_f);
+}
+
+void launch_adat_scaled_f32(const int64_t* pairptr, const int32_t* pair_p, const int32_t* pair_q,
+                            const int32_t* colof, const double* val, const double* dinv, int64_t nslots,
+                            const int32_t* slot_row, const int32_t* slot_col, const double* scale,
+                            const int64_t* dest, float* out_f, hipStream_t st) {
+    if (nslots > 0)
+        k_adat_scaled_f32<<<grid_for(nslots), kBlock, 0, st>>>(pairptr, pair_p, pair_q, colof,
+                                                               val, dinv, nslots, slot_row, slot_col, scale,
+                                                               dest, out_f);
+}
+
+void launch_add_beta_scaled_f32(const int64_t* diagpos, int32_t m, double beta, const double* scale,
+                                float* Lf, hipStream_t st) {
+    if (m > 0) k_add_beta_scaled_f32<<<grid_for(m), kBlock, 0, st>>>(diagpos, m, beta, scale, Lf);
+}
+
+void launch_diag_scale(int32_t m, const double* diag, double beta, double* s, hipStream_t st) {
+    if (m > 0) k_diag_scale<<<grid_for(m), kBlock, 0, st>>>(m, diag, beta, s);
+}
+
+void launch_scale_perm(int32_t m, double* v, const double* s, const int32_t* iperm, hipStream_t st) {
+    if (m > 0) k_scale_perm<<<grid_for(m), kBlock, 0, st>>>(m, v, s, iperm);
 }
 
 void launch_add_beta(const int64_t* diagpos, int32_t m, double beta, double* Ld, float* Lf,

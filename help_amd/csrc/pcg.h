@@ -68,7 +68,6 @@ void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* p
                      double* z, hipStream_t st);
 
 struct PcgState;
-const RowSplitDev& pcg_row_split(const PcgState* p);
 // host side (pcg.hip): allocation at create, the partial-Cholesky build at
 // renew, the conjugate-gradient solve (rhs in, x2 out, device vectors)
 void pcg_free(PcgState* p);

@@ -14,11 +14,6 @@ namespace help {
 
 struct PcgState {
     int32_t K = 0;
-    int32_t max_row_nnz = 0;
-    RowSplitDev rs;
-    DBuf<RowChunk> chunks;
-    DBuf<int32_t> hrow, hptr;
-    DBuf<double> part;
     DBuf<double> L, D, S, w, g, sc, L11, X, y1, ltpart, pval;
     DBuf<float> L32;
     DBuf<int32_t> perm, pos, col, pidx;
@@ -27,7 +22,6 @@ struct PcgState {
 };
 
 void pcg_free(PcgState* p) { delete p; }
-const RowSplitDev& pcg_row_split(const PcgState* p) { return p->rs; }
 
 void pcg_setup(help_gpu* h) {
     auto* P = new PcgState();
@@ -35,22 +29,6 @@ void pcg_setup(help_gpu* h) {
     hipStream_t st = h->st;
     const int32_t m = h->m, n = h->n;
     P->K = std::max(0, std::min<int32_t>(std::min(h->opts.pcg_k, kPcgMaxK), m));
-    std::vector<RowChunk> chunks;
-    std::vector<int32_t> hrow, hptr;
-    build_row_split(m, h->R.rowptr, chunks, hrow, hptr);
-    for (int32_t i = 0; i < m; ++i)
-        P->max_row_nnz = std::max<int32_t>(P->max_row_nnz, (int32_t)(h->R.rowptr[i + 1] - h->R.rowptr[i]));
-    P->chunks.upload(chunks, st);
-    P->hrow.upload(hrow, st);
-    P->hptr.upload(hptr, st);
-    P->part.alloc(std::max<size_t>(1, chunks.size()));
-    P->rs.m = m;
-    P->rs.nchunk = (int32_t)chunks.size();
-    P->rs.nheavy = (int32_t)hrow.size();
-    P->rs.chunks = P->chunks.p;
-    P->rs.hrow = P->hrow.p;
-    P->rs.hptr = P->hptr.p;
-    P->rs.part = P->part.p;
     const int64_t K = std::max<int32_t>(P->K, 1);
     P->L.alloc((size_t)m * K);
     P->L32.alloc((size_t)m * K);
@@ -85,16 +63,16 @@ void pcg_renew(help_gpu* h) {
     const int32_t m = h->m;
     const int32_t K = P->K;
     // Schur diagonal: diag(A D^-1 A^T) (PCG.cpp:40-46)
-    launch_csr_split(P->rs, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_dinv.p, P->S.p, true, st);
+    launch_csr_split(h->rows, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_dinv.p, P->S.p, true, st);
     HIPCHK(hipMemcpyAsync(P->perm.p, P->iota.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(P->pos.p, P->iota.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
     for (int32_t k = 0; k < K; ++k) {
         launch_pc_argmax(m, k, P->S.p, P->pval.p, P->pidx.p, P->perm.p, P->pos.p, P->col.p, st);
         // column of A D^-1 A^T at the pivot row: A (D^-1 a_col^T) (PCG.cpp:63-79)
-        launch_pc_scatter(P->col.p, P->max_row_nnz, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_dinv.p,
+        launch_pc_scatter(P->col.p, h->max_row_nnz, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_dinv.p,
                           P->sc.p, false, st);
-        launch_csr_split(P->rs, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, P->sc.p, P->g.p, false, st);
-        launch_pc_scatter(P->col.p, P->max_row_nnz, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_dinv.p,
+        launch_csr_split(h->rows, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, P->sc.p, P->g.p, false, st);
+        launch_pc_scatter(P->col.p, h->max_row_nnz, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_dinv.p,
                           P->sc.p, true, st);
         // left-looking update against the previous columns (PCG.cpp:85-102)
         launch_pc_pivot(m, k, P->col.p, P->g.p, P->L.p, 0.0, P->D.p, P->w.p, st);
@@ -140,7 +118,7 @@ int pcg_solve(help_gpu* h, const double* rhs, const double* x0, double* x2, doub
     double *x = P->x.p, *r = P->r.p, *z = P->z.p, *p = P->p.p, *q = P->q.p, *t = P->t.p;
     auto matvec = [&](const double* v, double* out) {   // out = A D^-1 A^T v
         launch_csc_t_scaled(n, h->d_colptr.p, h->d_rowind.p, h->d_val.p, h->d_dinv.p, v, t, st);
-        launch_csr_split(P->rs, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, t, out, false, st);
+        launch_csr_split(h->rows, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, t, out, false, st);
     };
     auto precond = [&](const double* v, double* out) {
         launch_pc_apply(m, P->K, P->perm.p, P->pos.p, P->L32.p, P->X.p, P->S.p, v, P->y1.p, P->ltpart.p, out, st);

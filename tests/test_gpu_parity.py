@@ -130,6 +130,41 @@ def test_solve_matches_oracle(cfg0, mode):
     be.destruct()
 
 
+@pytest.mark.parametrize("flag", [L.FAULT_REFINE_NOCONV, L.FAULT_FP32_BREAKDOWN])
+def test_fp32_fallback_to_fp64(cfg0, flag):
+    """A refinement failure of the FP32 factor (forced from the host) makes
+    the solve refactor in FP64 with the same d and keep FP64 afterwards."""
+    be = LinearEquation(mode=L.MODE_DIRECT_FP32, fault_flags=flag)
+    be.construct(cfg0)
+    rng = np.random.default_rng(8)
+    d = log_uniform(rng, cfg0.n, -2, 2)
+    b1 = rng.standard_normal(cfg0.n)
+    b2 = rng.standard_normal(cfg0.m)
+    _flush(f"fp32 fallback cfg0 flag={flag}")
+    be.renew(d)
+    x1, x2 = be.solve(d, b1, b2)
+    assert be.solve_stats.fallback == 1
+    perm, _, _ = be.symbolic_arrays()
+    Lp, Li, Lx = _oracle_factor(cfg0, perm, d)
+    o1, o2 = O.solve_kkt(cfg0, perm, Lp, Li, Lx, d, b1, b2)
+    assert np.linalg.norm(x2 - o2) <= 1e-10 * np.linalg.norm(o2)
+    assert np.linalg.norm(x1 - o1) <= 1e-10 * np.linalg.norm(o1)
+    # the next renew factors in FP64: columns equal the FP64 oracle's
+    be.renew(d)
+    worst = 0.0
+    for j in range(0, cfg0.m, 97):
+        rows, vals = be.factor_column(j)
+        ref = dict(zip(Li[Lp[j]:Lp[j + 1]], Lx[Lp[j]:Lp[j + 1]]))
+        scale = max(abs(v) for v in ref.values())
+        got = dict(zip(rows, vals))
+        worst = max([worst] + [abs(got.get(r, 0.0) - v) / scale for r, v in ref.items()])
+    assert worst <= 1e-12, worst
+    x1b, x2b = be.solve(d, b1, b2)
+    assert be.solve_stats.fallback == 0
+    assert np.linalg.norm(x2b - o2) <= 1e-10 * np.linalg.norm(o2)
+    be.destruct()
+
+
 def test_solve_mcf_large_fronts():
     inst = instances.mcf(side=60, commodities=4, pairs=10, seed=7)
     be = LinearEquation()
