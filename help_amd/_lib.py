@@ -138,6 +138,17 @@ SIGNATURES = {
                                                 C.POINTER(C.c_int64), _D]),
     "help_gpu_backend_solve_stats": (C.c_int, [C.POINTER(Backend), C.POINTER(C.c_int64),
                                                C.POINTER(C.c_int64), _D]),
+    "help_mpnn_num_params": (C.c_int, []),
+    "help_mpnn_new": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "help_mpnn_free": (None, [_P]),
+    "help_mpnn_get_params": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "help_mpnn_set_params": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "help_mpnn_predict": (C.c_int, [_P, _P, _D, _D, _D]),
+    "help_mpnn_loss_grad": (C.c_int, [_P, _P, _D, _D, _D, _D, C.POINTER(C.c_float)]),
+    "help_mpnn_adam_step": (C.c_int, [_P, C.c_double]),
+    "help_gpu_set_warmstart": (C.c_int, [_P, _P]),
+    "help_gpu_backend_set_warmstart": (C.c_int, [C.POINTER(Backend), _P]),
+    "help_gpu_backend_pcg_iterations": (C.c_int64, [C.POINTER(Backend)]),
     "help_gpu_pcg_factor": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D, _D, _D]),
     "help_gpu_profile": (C.c_int, [_P, C.c_int32]),
     "help_gpu_kernel_stats": (C.c_int, [_P, C.c_int32, _D, C.POINTER(C.c_int64), _D,

@@ -134,6 +134,11 @@ class LinearEquation:
                                                 L.i32ptr(tl)), "supernodes")
         return sf, nr, lv, tl
 
+    def set_warmstart(self, net):
+        """HELP_MODE_PCG_LEARNED: start PCG from ``net``'s guess (a Mpnn)."""
+        self._check(L.lib().help_gpu_set_warmstart(self.h, net.ptr if net is not None else None), "set_warmstart")
+        self._net = net
+
     def pcg_factor(self):
         """PCG mode: (perm, D, S, L) of the partial-Cholesky preconditioner,
         L as an m x K array by original row."""

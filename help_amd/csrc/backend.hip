@@ -609,9 +609,8 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
         if (nnz > INT32_MAX) throw std::runtime_error("nnz exceeds the int32 index range");
         h->m = m; h->n = n; h->nnz = nnz;
         h->use_fp32 = h->opts.mode == HELP_MODE_DIRECT_FP32;
-        h->use_pcg = h->opts.mode == HELP_MODE_PCG;
-        if (h->opts.mode == HELP_MODE_PCG_LEARNED)
-            throw std::runtime_error("the learned-warm-start mode is not available in this build");
+        h->use_pcg = h->opts.mode == HELP_MODE_PCG || h->opts.mode == HELP_MODE_PCG_LEARNED;
+        h->learned = h->opts.mode == HELP_MODE_PCG_LEARNED;
         if (h->opts.mode < HELP_MODE_DIRECT_FP64 || h->opts.mode > HELP_MODE_PCG_LEARNED)
             throw std::runtime_error("unknown mode");
         h->A.m = m; h->A.n = n;
@@ -744,9 +743,25 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
             // matrix-free PCG on A D^-1 A^T (UsePCG_CPU.cpp:91-178), original row order
             double rel = 0;
             int it;
+            const double* x0 = nullptr;
+            if (h->learned) {
+                // the network's guess (PCG then runs to the same FP64 target)
+                bool ok = h->net != nullptr && !(h->opts.fault_flags & HELP_FAULT_NET_NONFINITE);
+                if (ok) {
+                    help::mpnn_predict_device(h->net, h, h->d_d.p, h->d_rp.p, h->d_q.p);
+                    HIPCHK(hipMemsetAsync(h->d_err.p, 0, sizeof(int32_t), st));
+                    help::nn_check_finite(m, h->d_q.p, h->d_err.p, st);
+                    int32_t bad = 0;
+                    HIPCHK(hipMemcpyAsync(&bad, h->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipStreamSynchronize(st));
+                    ok = bad == 0;
+                }
+                if (ok) x0 = h->d_q.p;
+                else loc.fallback = 1;
+            }
             {
                 ProfScope ps(h, HELP_K_PCG_SOLVE, 0.0);
-                it = help::pcg_solve(h, h->d_rp.p, nullptr, h->d_x2.p, rel);
+                it = help::pcg_solve(h, h->d_rp.p, x0, h->d_x2.p, rel);
                 ps.set_work(help::pcg_iteration_bytes(h) * (it + 1));
             }
             loc.pcg_iters = it;
@@ -848,6 +863,13 @@ int help_gpu_adat_values(help_gpu* h, const double* d, double* vals) {
     } catch (const std::exception& e) {
         return fail(h, e);
     }
+}
+
+int help_gpu_set_warmstart(help_gpu* h, help_mpnn* net) {
+    if (!h) return HELP_ERR_ARG;
+    if (!h->learned) { h->err = "set_warmstart: handle is not in the learned PCG mode"; return HELP_ERR_STATE; }
+    h->net = net;
+    return HELP_OK;
 }
 
 int help_gpu_pcg_factor(help_gpu* h, int32_t* K, int32_t* perm, double* D, double* S, double* L) {
@@ -983,6 +1005,8 @@ struct GpuBackendCtx {
     help_gpu::KStat kstat[HELP_K_COUNT];
     int64_t n_solve = 0, refine_steps = 0;
     double max_rel_residual = 0;
+    int64_t pcg_iters = 0;
+    help_mpnn* net = nullptr;   // warm start for the learned PCG mode (not owned)
 };
 
 static void gb_release(GpuBackendCtx* c) {
@@ -999,7 +1023,9 @@ static void gb_release(GpuBackendCtx* c) {
 static int gb_construct(void* ctx, const help_lp* lp) {
     auto* c = (GpuBackendCtx*)ctx;
     gb_release(c);
-    return help_gpu_create(lp->m, lp->n, lp->nnz, lp->colptr, lp->rowind, lp->val, &c->opts, &c->h);
+    int st = help_gpu_create(lp->m, lp->n, lp->nnz, lp->colptr, lp->rowind, lp->val, &c->opts, &c->h);
+    if (st == HELP_OK && c->opts.mode == HELP_MODE_PCG_LEARNED) st = help_gpu_set_warmstart(c->h, c->net);
+    return st;
 }
 static int gb_destruct(void* ctx) {
     gb_release((GpuBackendCtx*)ctx);
@@ -1017,6 +1043,7 @@ static int gb_solve(void* ctx, const double* d, const double* b1, const double* 
     if (st == HELP_OK) {
         c->n_solve += 1;
         c->refine_steps += ss.refine_steps;
+        c->pcg_iters += ss.pcg_iters;
         c->max_rel_residual = std::max(c->max_rel_residual, ss.rel_residual);
     }
     return st;
@@ -1047,6 +1074,17 @@ void help_gpu_backend_free(help_backend* be) {
 help_gpu* help_gpu_backend_handle(const help_backend* be) {
     if (!be || !be->ctx) return nullptr;
     return ((GpuBackendCtx*)be->ctx)->h;
+}
+
+int help_gpu_backend_set_warmstart(help_backend* be, help_mpnn* net) {
+    if (!be || !be->ctx) return HELP_ERR_ARG;
+    ((GpuBackendCtx*)be->ctx)->net = net;
+    return HELP_OK;
+}
+
+int64_t help_gpu_backend_pcg_iterations(const help_backend* be) {
+    if (!be || !be->ctx) return -1;
+    return ((GpuBackendCtx*)be->ctx)->pcg_iters;
 }
 
 int help_gpu_backend_solve_stats(const help_backend* be, int64_t* n_solve, int64_t* refine_steps,

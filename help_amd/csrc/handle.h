@@ -10,6 +10,7 @@
 
 #include "device.h"
 #include "help_mi355x.h"
+#include "mpnn.h"
 #include "pcg.h"
 #include "symbolic.h"
 
@@ -153,6 +154,8 @@ struct help_gpu {
     // matrix-free PCG mode (pcg.hip)
     help::PcgState* pcg = nullptr;
     bool use_pcg = false;
+    bool learned = false;          // PCG started from the network's guess
+    help_mpnn* net = nullptr;      // not owned
 
     ~help_gpu() {
         if (st) { (void)hipStreamSynchronize(st); }
@@ -173,4 +176,6 @@ int pcg_solve(help_gpu* h, const double* rhs, const double* x0, double* x2, doub
 double pcg_build_bytes(const help_gpu* h);
 double pcg_iteration_bytes(const help_gpu* h);
 void pcg_download(help_gpu* h, int32_t* K, int32_t* perm, double* D, double* S, double* L);
+// warm-start network forward on device vectors (mpnn.hip)
+void mpnn_predict_device(help_mpnn* net, help_gpu* h, const double* d, const double* r, double* x0);
 }  // namespace help

@@ -133,6 +133,11 @@ int pcg_solve(help_gpu* h, const double* rhs, const double* x0, double* x2, doub
         HIPCHK(hipMemcpyAsync(r, rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
     }
     double rr = dot(h, r, r, m);
+    if (x0 && !(rr <= bb)) {   // a guess worse than zero (or not finite): start from zero
+        HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * m, st));
+        HIPCHK(hipMemcpyAsync(r, rhs, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+        rr = bb;
+    }
     // the reference's stop is relative to ||r0|| (PCG.cpp:328); with a
     // warm start the relative target is taken against ||rhs|| so that a
     // good guess cannot loosen it

@@ -20,6 +20,14 @@ class GpuBackend:
         if st != L.HELP_OK:
             raise HelpError(st, "help_gpu_backend_new failed")
 
+    def set_warmstart(self, net):
+        """HELP_MODE_PCG_LEARNED: start every PCG solve from ``net``'s guess."""
+        self._net = net
+        L.lib().help_gpu_backend_set_warmstart(C.byref(self.table), net.ptr if net is not None else None)
+
+    def pcg_iterations(self):
+        return L.lib().help_gpu_backend_pcg_iterations(C.byref(self.table))
+
     def solve_stats(self):
         """(solves, FP64 refinement iterations, largest final relative residual)"""
         n, it, rel = C.c_int64(), C.c_int64(), C.c_double()

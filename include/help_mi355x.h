@@ -181,6 +181,33 @@ int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows,
 #define HELP_K_PCG_BUILD    12   /* PCG mode: partial-Cholesky build per renew */
 #define HELP_K_PCG_SOLVE    13   /* PCG mode: conjugate gradients per solve  */
 #define HELP_K_COUNT        14
+/* ------------------------------------------------------------------ */
+/* learned warm start for the PCG mode (no reference counterpart: the  */
+/* initial guess of PCG.cpp:309-310 is zero)                           */
+/* ------------------------------------------------------------------ */
+/* A bipartite message-passing network on A's row/column graph (4 layers,
+ * width 64) predicts x0 ~ (A D^-1 A^T)^-1 r from d and r.  bf16 = 1 keeps
+ * activations and the weights the kernels use in BF16 (FP32 accumulation);
+ * bf16 = 0 keeps them in FP32.  Training: loss_grad (forward + backward on
+ * the handle's matrix, target dy) then adam_step. */
+typedef struct help_mpnn help_mpnn;
+int  help_mpnn_num_params(void);
+int  help_mpnn_new(int32_t bf16, uint64_t seed, help_mpnn** out);
+void help_mpnn_free(help_mpnn* net);
+int  help_mpnn_get_params(const help_mpnn* net, float* out);   /* FP32 master weights */
+int  help_mpnn_set_params(help_mpnn* net, const float* in);
+/* x0[m] = net(A, d, r) on the handle's matrix; host or device pointers */
+int  help_mpnn_predict(help_mpnn* net, help_gpu* h, const double* d, const double* r, double* x0);
+/* loss sum_i (y_i - t_i)^2 / sum_i t_i^2 (t_i = dy_i g_i / rms(r)) and its
+ * gradient (num_params floats, may be NULL) */
+int  help_mpnn_loss_grad(help_mpnn* net, help_gpu* h, const double* d, const double* r,
+                         const double* dy, double* loss, float* grad);
+int  help_mpnn_adam_step(help_mpnn* net, double lr);
+/* HELP_MODE_PCG_LEARNED: the network whose guess starts PCG (the handle
+ * does not own it; NULL or a non-finite guess starts from zero and the
+ * solve reports fallback = 1) */
+int  help_gpu_set_warmstart(help_gpu* h, help_mpnn* net);
+
 /* PCG mode: the partial-Cholesky preconditioner of the last renew.  K is
  * the number of pivots; perm[m] the pivot order (pivots first, PCG_PERM),
  * D[K] the pivots, S[m] the Schur diagonal by row (DIAG_S), L[m*K] the
@@ -317,6 +344,10 @@ int help_gpu_backend_kernel_stats(const help_backend* be, int32_t kind, double* 
  * residual ||r - (A D^-1 A^T + beta I) x2|| / ||r||. */
 int help_gpu_backend_solve_stats(const help_backend* be, int64_t* n_solve, int64_t* refine_steps,
                                  double* max_rel_residual);
+/* PCG iterations summed over the backend's solves (PCG modes) */
+int64_t help_gpu_backend_pcg_iterations(const help_backend* be);
+/* HELP_MODE_PCG_LEARNED: the network every handle of this backend starts from */
+int help_gpu_backend_set_warmstart(help_backend* be, help_mpnn* net);
 
 #ifdef __cplusplus
 }
