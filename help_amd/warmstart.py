@@ -183,3 +183,115 @@ def pcg_iterations(be_plain, be_learned, d, r):
     return dict(it_zero=it_zero, it_net=it_net, t_zero=t_zero, t_net=t_net,
                 fallback=be_learned.solve_stats.fallback,
                 agree=float(np.linalg.norm(x_net - x_zero) / max(np.linalg.norm(x_zero), 1e-300)))
+
+
+def _instances(count, seed0, side, ks):
+    from . import instances
+    rng = np.random.default_rng(seed0)
+    out = []
+    for i in range(count):
+        s = int(rng.integers(side[0], side[1] + 1))
+        k = int(rng.integers(ks[0], ks[1] + 1))
+        out.append(((s, k, seed0 + i), instances.mcf(side=s, commodities=k, pairs=20, seed=seed0 + i)))
+    return out
+
+
+def main(argv=None):
+    """Harvest, train and evaluate on the multicommodity-flow family:
+
+        python -m help_amd.warmstart --train 24 --test 8 --out DIR
+
+    writes DIR/warmstart_net.npz and DIR/warmstart_eval.json."""
+    import argparse
+    import json
+    import os
+    from .hsd import GpuBackend, solve_lp
+    ap = argparse.ArgumentParser(prog="python -m help_amd.warmstart")
+    ap.add_argument("--train", type=int, default=24)
+    ap.add_argument("--test", type=int, default=8)
+    ap.add_argument("--side", type=int, nargs=2, default=[20, 40])
+    ap.add_argument("--k", type=int, nargs=2, default=[2, 4])
+    ap.add_argument("--every", type=int, default=3)
+    ap.add_argument("--tuples", type=int, default=12)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--pcg-k", type=int, default=50)
+    ap.add_argument("--eval-per-instance", type=int, default=3)
+    ap.add_argument("--out", default=".")
+    args = ap.parse_args(argv)
+    os.makedirs(args.out, exist_ok=True)
+    log = lambda s: print(s, flush=True)
+    t0 = time.time()
+    train_inst = _instances(args.train, 1000, args.side, args.k)
+    test_inst = _instances(args.test, 2000, args.side, args.k)
+    data, handles = [], []
+    for key, inst in train_inst:
+        tuples, res = harvest(inst, every=args.every, max_tuples=args.tuples)
+        be = graph_handle(inst)
+        handles.append(be)
+        data += [(be, t) for t in tuples]
+        log(f"train {key}: m={inst.m} n={inst.n} hsd status {res.status} iters {res.iterations} "
+            f"tuples {len(tuples)}")
+    log(f"{len(data)} training tuples ({time.time() - t0:.1f}s)")
+    net = Mpnn(bf16=True, seed=0)
+    t1 = time.time()
+    hist = train(net, data, epochs=args.epochs, lr=args.lr, log=log)
+    t_train = time.time() - t1
+    net.save(os.path.join(args.out, "warmstart_net.npz"))
+    for be in handles:
+        be.destruct()
+    rows = []
+    for key, inst in test_inst:
+        tuples, res = harvest(inst, every=args.every, max_tuples=args.tuples)
+        g = graph_handle(inst)
+        losses = [net.loss_grad(g, *t) for t in tuples]
+        g.destruct()
+        plain = graph_handle(inst, pcg_k=args.pcg_k, pcg_tol=1e-10)
+        learned = graph_handle(inst, pcg_k=args.pcg_k, pcg_tol=1e-10, mode=L.MODE_PCG_LEARNED)
+        learned.set_warmstart(net)
+        pick = np.linspace(0, len(tuples) - 1, min(args.eval_per_instance, len(tuples))).astype(int)
+        for i in pick:
+            out = pcg_iterations(plain, learned, tuples[i][0], tuples[i][1])
+            out.update(instance=key, tuple=int(i), loss=float(losses[i]), m=inst.m, n=inst.n)
+            rows.append(out)
+            log(f"test {key} tuple {i}: loss {losses[i]:.3f} pcg iterations zero {out['it_zero']} "
+                f"net {out['it_net']} ({out['t_zero']:.3f}s / {out['t_net']:.3f}s) agree {out['agree']:.1e}")
+        plain.destruct()
+        learned.destruct()
+    it0 = np.array([r["it_zero"] for r in rows], float)
+    it1 = np.array([r["it_net"] for r in rows], float)
+    tz = np.array([r["t_zero"] for r in rows], float)
+    tn = np.array([r["t_net"] for r in rows], float)
+    # end to end on the first held-out instance: PCG vs learned PCG
+    key, inst = test_inst[0]
+    e2e = {}
+    for name, mode in (("pcg", L.MODE_PCG), ("pcg_learned", L.MODE_PCG_LEARNED)):
+        gb = GpuBackend(mode=mode, pcg_k=args.pcg_k)
+        if mode == L.MODE_PCG_LEARNED:
+            gb.set_warmstart(net)
+        t = time.time()
+        res, _ = solve_lp(inst, gb.table, device=True)
+        e2e[name] = dict(status=L.HSD_STATUS.get(res.status, str(res.status)), iterations=res.iterations,
+                         rel_gap=res.rel_gap, primal_obj=res.primal_obj, seconds=time.time() - t,
+                         pcg_iterations=int(gb.pcg_iterations()))
+        gb.close()
+        log(f"end to end {name}: {e2e[name]}")
+    summary = dict(
+        family="multicommodity flow (the configs[1] generator)", side=args.side, commodities=args.k,
+        n_train=args.train, n_test=args.test, training_tuples=len(data), epochs=args.epochs,
+        params=net.nparams, precision="bf16 activations and weights, fp32 accumulation",
+        loss_history=hist, train_seconds=t_train, pcg_k=args.pcg_k, pcg_tol=1e-10,
+        pcg_iterations_zero_mean=float(it0.mean()), pcg_iterations_net_mean=float(it1.mean()),
+        iteration_ratio_mean=float((it1 / np.maximum(it0, 1)).mean()),
+        time_zero_mean=float(tz.mean()), time_net_mean=float(tn.mean()),
+        end_to_end=e2e, per_solve=rows)
+    with open(os.path.join(args.out, "warmstart_eval.json"), "w") as f:
+        json.dump(summary, f, indent=1)
+    log(json.dumps({k: v for k, v in summary.items() if k not in ("per_solve", "loss_history")}))
+    net.close()
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())
