@@ -39,7 +39,7 @@ class Opts(C.Structure):
                 ("refine_tol", C.c_double), ("pcg_k", C.c_int32), ("pcg_max_iter", C.c_int32),
                 ("pcg_tol", C.c_double), ("pcg_fp32_precond", C.c_int32),
                 ("fault_flags", C.c_int32), ("verbose", C.c_int32), ("symbolic_only", C.c_int32),
-                ("front_path", C.c_int32), ("profile", C.c_int32)]
+                ("front_path", C.c_int32), ("profile", C.c_int32), ("refine_fp64", C.c_int32)]
 
 
 class SolveStats(C.Structure):

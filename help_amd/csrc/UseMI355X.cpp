@@ -6,7 +6,12 @@
 // LP.h:99-109) and linked against libhelp_mi355x.so.
 //
 // Mode selection: environment variable HELP_MI355X_MODE = fp64 (default),
-// fp32 (FP32 factor + FP64 refinement).
+// fp32 (FP32 factor + FP64 refinement), pcg (matrix-free PCG with the
+// partial-Cholesky preconditioner) or pcg_learned (PCG started from the
+// warm-start network whose FP32 weights, help_mpnn_num_params() floats in
+// the library's parameter order, are read from the raw file named by
+// HELP_MI355X_NET; HELP_MI355X_NET_BF16=0 runs the network in FP32).
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -15,6 +20,7 @@
 #include "help_mi355x.h"
 
 static help_gpu* g_help_handle = nullptr;
+static help_mpnn* g_help_net = nullptr;
 static char g_help_msg[512];
 
 static void help_check(int status, const char* what) {
@@ -44,9 +50,24 @@ int LinearEquation_Construct() {
     o.beta = Cholesky_Diagonal_Add;   // LP.h:139
     const char* mode = std::getenv("HELP_MI355X_MODE");
     if (mode && !std::strcmp(mode, "fp32")) o.mode = HELP_MODE_DIRECT_FP32;
+    else if (mode && !std::strcmp(mode, "pcg")) o.mode = HELP_MODE_PCG;
+    else if (mode && !std::strcmp(mode, "pcg_learned")) o.mode = HELP_MODE_PCG_LEARNED;
+    else if (mode && std::strcmp(mode, "fp64")) CheckError(1, (char*)"HELP_MI355X_MODE: fp64, fp32, pcg or pcg_learned");
     help_check(help_gpu_create(n_Row, n_Col, (int64_t)rowind.size(), colptr.data(), rowind.data(),
                                val.data(), &o, &g_help_handle),
                "construct");
+    if (o.mode == HELP_MODE_PCG_LEARNED) {
+        const char* path = std::getenv("HELP_MI355X_NET");
+        std::vector<float> w(help_mpnn_num_params());
+        FILE* f = path ? std::fopen(path, "rb") : nullptr;
+        const bool ok = f && std::fread(w.data(), sizeof(float), w.size(), f) == w.size();
+        if (f) std::fclose(f);
+        if (!ok) CheckError(1, (char*)"HELP_MI355X_NET: cannot read the network weights");
+        const char* bf = std::getenv("HELP_MI355X_NET_BF16");
+        help_check(help_mpnn_new(!(bf && !std::strcmp(bf, "0")), 0, &g_help_net), "network");
+        help_check(help_mpnn_set_params(g_help_net, w.data()), "network weights");
+        help_check(help_gpu_set_warmstart(g_help_handle, g_help_net), "warm start");
+    }
     return 0;
 }
 
@@ -54,6 +75,8 @@ int LinearEquation_Construct() {
 int LinearEquation_Destruct() {
     help_gpu_destroy(g_help_handle);
     g_help_handle = nullptr;
+    help_mpnn_free(g_help_net);
+    g_help_net = nullptr;
     return 0;
 }
 

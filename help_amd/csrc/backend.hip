@@ -442,11 +442,12 @@ void apply_factor(help_gpu* h, const double* r, double* z) {
 // Preconditioned CG in FP64 on M = A D^-1 A^T + beta I with the factor of
 // type T as preconditioner, started from the factor's solution.  On entry
 // d_rp holds the permuted right-hand side; on exit d_x2 holds the solution
-// (original order).  Stops at ||r|| <= refine_tol ||rhs||, after
-// refine_max iterations, or (FP64 factor) when an iteration fails to halve
-// the residual: the rounding floor; returns iterations.
+// (original order).  Stops at ||r|| <= refine_tol ||rhs||, after max_it
+// iterations, or (FP64 factor) when an iteration fails to halve the
+// residual: the rounding floor; returns iterations.  The final relative
+// residual is measured in FP64 also when max_it is 0.
 template <typename T>
-int pcg_factor(help_gpu* h, double& rel) {
+int pcg_factor(help_gpu* h, double& rel, int max_it) {
     hipStream_t st = h->st;
     const int32_t m = h->m, n = h->n;
     double *rhs = h->d_c.p, *r = h->d_r.p, *x = h->d_x2.p, *z = h->d_z.p, *p = h->d_p.p, *q = h->d_q.p;
@@ -468,12 +469,12 @@ int pcg_factor(help_gpu* h, double& rel) {
     if (rr0 == 0.0) { rel = 0; return 0; }
     double rr = dot_dev(h, r, r, m);
     rel = std::sqrt(rr / rr0);
-    if (rel <= h->opts.refine_tol || h->opts.refine_max <= 0) return 0;
+    if (rel <= h->opts.refine_tol || max_it <= 0) return 0;
     apply_factor<T>(h, r, z);
     HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
     double rz = dot_dev(h, r, z, m);
     int it = 0;
-    while (it < h->opts.refine_max && rz > 0) {
+    while (it < max_it && rz > 0) {
         ++it;
         matvec(p, q);
         double pq = dot_dev(h, p, q, m);
@@ -768,9 +769,12 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
             loc.rel_residual = rel;
         } else {
             // the factor's solution, refined by FP64 conjugate gradients on
-            // (A D^-1 A^T + beta I) preconditioned with the same factor
+            // (A D^-1 A^T + beta I) preconditioned with the same factor; the
+            // FP64 mode takes the plain factor solve unless refine_fp64 > 0
             double rel = 0;
-            int it = h->factor_fp32 ? pcg_factor<float>(h, rel) : pcg_factor<double>(h, rel);
+            int it = h->factor_fp32 ? pcg_factor<float>(h, rel, h->opts.refine_max)
+                                    : pcg_factor<double>(h, rel, h->fp32_off ? h->opts.refine_max
+                                                                             : h->opts.refine_fp64);
             if (h->factor_fp32 &&
                 (!(rel <= h->opts.refine_tol) ||
                  (h->opts.fault_flags & (HELP_FAULT_FP32_BREAKDOWN | HELP_FAULT_REFINE_NOCONV)))) {
@@ -781,7 +785,7 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
                 assemble(h, false);
                 factor(h, false);
                 launch_scatter_perm(m, h->d_c.p, h->d_iperm.p, h->d_rp.p, st);   // permuted rhs again
-                it += pcg_factor<double>(h, rel);
+                it += pcg_factor<double>(h, rel, h->opts.refine_max);
                 loc.fallback = 1;
             }
             loc.refine_steps = it;

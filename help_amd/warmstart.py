@@ -277,7 +277,7 @@ def main(argv=None):
         gb.close()
         log(f"end to end {name}: {e2e[name]}")
     summary = dict(
-        family="multicommodity flow (the configs[1] generator)", side=args.side, commodities=args.k,
+        family="multicommodity flow on a 4-connected grid (instances.mcf)", side=args.side, commodities=args.k,
         n_train=args.train, n_test=args.test, training_tuples=len(data), epochs=args.epochs,
         params=net.nparams, precision="bf16 activations and weights, fp32 accumulation",
         loss_history=hist, train_seconds=t_train, pcg_k=args.pcg_k, pcg_tol=1e-10,

@@ -68,10 +68,13 @@ typedef struct help_opts {
     int32_t symbolic_only;   /* 1: analysis only, no device state (host tests)  */
     int32_t front_path;      /* 0 auto; 1 every front through the tiled path   */
     int32_t profile;         /* 1: kernel-class profiling from creation on      */
+    int32_t refine_fp64;     /* FP64 mode: refinement steps on the FP64 factor
+                                (0: one factor solve, as UseCholmod.cpp does)  */
 } help_opts;
 
-/* Defaults: FP64 direct, beta=1e-5, refine 20 / 1e-12, pcg_k 500,
- * pcg_max_iter 5000, pcg_tol 1e-10, FP32 preconditioner. */
+/* Defaults: FP64 direct, beta=1e-5, refine 20 / 1e-12 (FP32 mode; FP64 mode
+ * refine_fp64 0), pcg_k 500, pcg_max_iter 5000, pcg_tol 1e-10, FP32
+ * preconditioner. */
 void help_opts_default(help_opts* o);
 
 typedef struct help_solve_stats {

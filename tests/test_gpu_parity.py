@@ -220,6 +220,30 @@ def test_hsd_end_to_end(cfg0, mode, device):
     assert res.rel_gap <= 1e-8
 
 
+@pytest.mark.parametrize("side,k", [(30, 8), (40, 6)])
+def test_hsd_end_to_end_mcf(side, k):
+    """Multicommodity-flow instances of the benchmark's family: the plain FP64
+    path (one factor solve per system, as UseCholmod.cpp) reaches the
+    oracle's status, iteration count within +-1 and objective."""
+    from help_amd.hsd import GpuBackend, solve_lp
+    inst = instances.mcf(side=side, commodities=k, pairs=20, seed=50 + side)
+    _flush(f"hsd mcf side={side} K={k} m={inst.m} n={inst.n}")
+    gb = GpuBackend(mode=L.MODE_DIRECT_FP64)
+    res, _ = solve_lp(inst, gb.table, device=True)
+    gb.close()
+    be = LinearEquation(symbolic_only=True)
+    be.construct(inst)
+    perm, _, _ = be.symbolic_arrays()
+    ob = O.OracleBackend(1e-5, perm)
+    ref, _ = solve_lp(inst, ob.table)
+    ob.close()
+    _flush(f"  gpu {res.status} {res.iterations} {res.primal_obj!r} | oracle {ref.status} "
+           f"{ref.iterations} {ref.primal_obj!r}")
+    assert res.status == ref.status == 0
+    assert abs(res.iterations - ref.iterations) <= 1
+    assert abs(res.primal_obj - ref.primal_obj) / (1 + abs(ref.primal_obj)) <= 1e-8
+
+
 def test_reference_driver_with_mi355x_backend(cfg0, tmp_path):
     exe = os.path.join(O.REF_DIR, "help_ref_mi355x")
     cpu = os.path.join(O.REF_DIR, "help_ref_cpu")
