@@ -387,13 +387,16 @@ __device__ inline void mfma_tile(const T* As, const T* Bs, int kc, int lane, int
 constexpr int kKc = 16;   // inner-dimension chunk staged in LDS
 
 // C[row0:+64, col0:+64] -= P[rows, k0:k1] * P[cols, k0:k1]^T, lower part only,
-// with P the L panel of the front.
+// with P the L panel of the front.  The k chunks are double-buffered in LDS
+// and the next chunk's global loads are in flight while the MFMAs consume
+// the current one (one barrier per chunk).
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, DevSym sym,
                                               T* __restrict__ L, T* __restrict__ U) {
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ T As[kKc * kTile];
-    __shared__ T Bs[kKc * kTile];
+    constexpr int kPer = kKc / 4;
+    __shared__ T As[2][kKc * kTile];
+    __shared__ T Bs[2][kKc * kTile];
     const GemmTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s;
     const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
@@ -409,19 +412,32 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
     const int li = tid & 63, lk = tid >> 6;
-    for (int32_t kb = tk.k0; kb < tk.k1; kb += kKc) {
+    const int32_t ia = tk.row0 + li, ib = tk.col0 + li;
+    const bool ain = ia < c, bin = ib < c;
+    const T* pa = Lp + min(ia, c - 1);
+    const T* pb = Lp + min(ib, c - 1);
+    T ra[kPer], rb[kPer];
+    auto load = [&](int32_t kb) {
 #pragma unroll
-        for (int r = 0; r < kKc / 4; ++r) {
-            int32_t k = lk + 4 * r;
-            int32_t kg = kb + k;
-            int32_t ia = tk.row0 + li, ib = tk.col0 + li;
-            bool kin = kg < tk.k1;
-            As[k * kTile + li] = (kin && ia < c) ? Lp[(int64_t)kg * c + ia] : T(0);
-            Bs[k * kTile + li] = (kin && ib < c) ? Lp[(int64_t)kg * c + ib] : T(0);
+        for (int r = 0; r < kPer; ++r) {
+            const int32_t kg = min(kb + lk + 4 * r, tk.k1 - 1);
+            ra[r] = pa[(int64_t)kg * c];
+            rb[r] = pb[(int64_t)kg * c];
+        }
+    };
+    load(tk.k0);
+    int buf = 0;
+    for (int32_t kb = tk.k0; kb < tk.k1; kb += kKc, buf ^= 1) {
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const int k = lk + 4 * r;
+            const bool kin = kb + k < tk.k1;
+            As[buf][k * kTile + li] = (kin && ain) ? ra[r] : T(0);
+            Bs[buf][k * kTile + li] = (kin && bin) ? rb[r] : T(0);
         }
         __syncthreads();
-        mfma_tile<T>(As, Bs, kKc, lane, wr, wc, acc);
-        __syncthreads();
+        if (kb + kKc < tk.k1) load(kb + kKc);
+        mfma_tile<T>(As[buf], Bs[buf], kKc, lane, wr, wc, acc);
     }
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
@@ -454,14 +470,31 @@ __global__ __launch_bounds__(256) void k_trsm_gemm(const PanelTask* __restrict__
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    // As[k][i] = X[i][k] = P[r0+i][b+k];  Bs[k][j] = Dinv[j][k]
-    for (int32_t idx = tid; idx < kTile * kTile; idx += 256) {
-        int32_t k = idx >> 6, i = idx & 63;
-        As[idx] = (k < nb && r0 + i < c) ? Lp[(int64_t)(b + k) * c + r0 + i] : T(0);
-        Bs[idx] = (k < nb && i < nb) ? Db[k * nb + i] : T(0);   // column k of Dinv = Dinv[.][k]
+    // As[k][i] = X[i][k] = P[r0+i][b+k];  Bs[k][j] = Dinv[j][k] (Dinv column-major:
+    // Db[k*nb + j] = Dinv[j][k]).  Every load is issued before the first LDS
+    // store: out-of-range elements read a clamped in-range address and are
+    // zeroed afterwards, so no branch separates the loads.
+    constexpr int kPer = kTile * kTile / 256;
+    T va[kPer], vb[kPer];
+    {
+        const int i = tid & 63;
+        const int32_t ri = min(r0 + i, c - 1);
+        const int ib = min(i, nb - 1);
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int k = (tid >> 6) + 4 * t;
+            const int kc = min(k, nb - 1);
+            va[t] = Lp[(int64_t)(b + kc) * c + ri];
+            vb[t] = Db[kc * nb + ib];
+        }
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int k = (tid >> 6) + 4 * t;
+            As[k * kTile + i] = (k < nb && r0 + i < c) ? va[t] : T(0);
+            Bs[k * kTile + i] = (k < nb && i < nb) ? vb[t] : T(0);
+        }
     }
     __syncthreads();
-    // Bs currently holds Bs[k][j] = Dinv[j][k]? Dinv column-major: Db[k*nb + j] = Dinv[j][k]. Yes.
     acc_t acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int q = 0; q < 2; ++q) acc[a][q] = acc_t{0, 0, 0, 0};
