@@ -197,6 +197,8 @@ def main():
         "config": {"workload": names[args.config], "mode": args.mode, "solver": "HELP HSD driver",
                    "status": L.HSD_STATUS.get(res.status, str(res.status)), "iterations": res.iterations,
                    "rel_gap": res.rel_gap, "primal_obj": res.primal_obj,
+                   "primal_infeas": res.primal_infeas, "dual_infeas": res.dual_infeas,
+                   "ms_per_iteration": round(1e3 * value / max(res.iterations, 1), 3),
                    "renew_s": round(res.seconds_renew, 3), "solve_s": round(res.seconds_solve, 3),
                    "driver": args.driver, "driver_s": round(host_driver, 3),
                    "construct_s": round(res.seconds_construct, 3),

@@ -15,7 +15,7 @@
 // extend-add, the dense partial factorization in LDS, the diagonal-block
 // inverses and the update SYRK.
 // Large fronts: extend-add by column chunks, then for each 64-column panel a
-// one-wave diagonal-block Cholesky + inverse, the panel rows times the
+// two-wave diagonal-block Cholesky + inverse, the panel rows times the
 // inverse on MFMA, an MFMA GEMM on the trailing panel columns, and finally
 // one MFMA SYRK for U with the full inner dimension n.
 #include "device.h"
@@ -251,93 +251,175 @@ __device__ void factor_invert_block(const T* __restrict__ src, int64_t lds, int3
     }
 }
 
-// Register-resident form of factor_invert_block with 512 threads: thread
-// (r, p) holds A[r][k] and X[r][k] for the 8 columns k = 8p + q.  Step j
-// broadcasts column j of A and row j of X through double-buffered LDS (one
-// barrier per step); every thread then applies the same predicated update
-// to its 8 columns, so the step is a short branch-free sequence.  Entries
-// above the diagonal of A are updated too but never read.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Diagonal block of a panel: LDL^T of the nb x nb block (identity padding to
+// 64) with L11 = L_unit diag(sqrt(d)) written in place and its inverse
+// Dinv = diag(1/sqrt(d)) L_unit^-1 written to the block's Dinv slot.
+//
+// Two waves with 2D register blocking: lane (bi, bk) = (lane >> 3, lane & 7)
+// holds the 8x8 block of rows 8bi.. and columns 8bk.. .
+//  * Wave 0 factors.  Step j publishes column j of the Schur complement to
+//    LDS (the lanes of column block j/8 write it), every lane reads its 8
+//    row values and 8 column values and applies a -= (raw_r / d_j) raw_k to
+//    its whole block.  That is exact on the trailing matrix (r, k > j); the
+//    other entries it touches are never read again (finished columns are
+//    already published, rows above j only own upper-triangle entries), so
+//    the step needs no masks and no branches.  L11 is formed at the end from
+//    the published columns.
+//  * Wave 1 forms X = L_unit^-1 in the same layout: step j waits for wave 0
+//    to publish step j (LDS flag; wave 0 never waits), takes l_rj for its
+//    rows from the published column, and subtracts l_rj X[j][:] (row j of X
+//    is final at step j and published by its owners).
+// 8 steps are unrolled so every register index is static; the publishing
+// stores are branch-free (non-owner lanes write a dump row), and the output
+// stores are unconditional (elements outside nb x nb go to an LDS slot
+// through the generic address space).
 template <typename T>
-__global__ __launch_bounds__(512) void k_potrf_diag(const PanelTask* __restrict__ tasks, DevSym sym,
-                                                    T* __restrict__ L, T* __restrict__ Dinv,
-                                                    int32_t* __restrict__ err) {
-    constexpr int Q = 8;
-    __shared__ T colb[2][kTile];
-    __shared__ T xrow[2][kTile];
-    __shared__ T dvec[kTile];
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_potrf_diag(const PanelTask* __restrict__ tasks, DevSym sym, T* __restrict__ L, T* __restrict__ Dinv,
+                  int32_t* __restrict__ err) {
+    constexpr int kS = kTile + 1;
+    __shared__ T S[kTile * kS];            // input staging, then X for the Dinv output
+    __shared__ T cbt[kTile * kTile];       // column j of the Schur complement at step j
+    __shared__ T rps[kTile], pivs[kTile], sdv[kTile];
+    __shared__ T xrow[kTile];
+    __shared__ T dump[2][kTile * 8];
+    __shared__ int flag;
     const PanelTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s, b = tk.b;
     const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
     const int32_t c = sym.snrow[s];
-    const int32_t nb = min(kTile, n - b);
-    T* Lp = L + sym.loff[s] + (int64_t)b * c + b;
+    const int nb = min(kTile, n - b);
+    T* P = L + sym.loff[s] + (int64_t)b * c + b;
     T* Db = Dinv + dinv_block_offset(sym, s, b);
-    const int tid = threadIdx.x;
-    const int r = tid & 63, p = tid >> 6;
-    const int k0 = Q * p;
-    T a[Q], x[Q];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int bi = lane >> 3, bk = lane & 7;
+    {
+        // lower triangle, identity outside nb, zero above; clamped loads
+        const int r = lane, rr = min(r, nb - 1);
+        T v[kTile / 2];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int k = k0 + q;
-        T v = (r == k) ? T(1) : T(0);
-        if (r < nb && k < nb && r >= k) v = Lp[(int64_t)k * c + r];
-        a[q] = v;
-        x[q] = (r == k) ? T(1) : T(0);
+        for (int t = 0; t < kTile / 2; ++t) {
+            const int k = w + 2 * t;
+            v[t] = P[(int64_t)min(k, nb - 1) * c + rr];
+        }
+#pragma unroll
+        for (int t = 0; t < kTile / 2; ++t) {
+            const int k = w + 2 * t;
+            S[k * kS + r] = (r < nb && k < nb) ? ((r >= k) ? v[t] : T(0)) : ((r == k) ? T(1) : T(0));
+        }
+        if (tid == 0) flag = 0;
     }
-    int nbad = 0;
-    for (int j = 0; j < kTile; ++j) {
-        const int buf = j & 1;
-        const int jq = j - k0;               // register of column j in this wave (0..Q-1 if owned)
-        if (jq >= 0 && jq < Q) {
-            T v = a[0];
-#pragma unroll
-            for (int q = 1; q < Q; ++q) v = (q == jq) ? a[q] : v;
-            colb[buf][r] = v;
-        }
-        if (r == j) {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) xrow[buf][k0 + q] = x[q];
-        }
-        __syncthreads();
-        T piv = colb[buf][j];
-        if (!(piv > T(0))) { if (j < nb) ++nbad; piv = Huge<T>::v; }
-        const T rpiv = T(1) / piv;
-        const T cr = colb[buf][r];
-        const T m = (r > j) ? cr * rpiv : T(0);
-        T ca[Q], cx[Q];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) { ca[q] = colb[buf][k0 + q]; cx[q] = xrow[buf][k0 + q]; }
-        // columns right of j update A, the others X (both operands finite)
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const bool right = k0 + q > j;
-            const T ma = right ? m : T(0), mx = right ? T(0) : m;
-            a[q] -= ma * ca[q];
-            x[q] -= mx * cx[q];
-        }
-        if (jq >= 0 && jq < Q) {
-            const T nv = (r > j) ? m : piv;
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (q == jq && r >= j) a[q] = nv;
-        }
-    }
-    if (tid == 0 && nbad) atomicAdd(err, nbad);
-    // diagonal d_k lives in thread (k, k/Q), register k%Q
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-        if (r == k0 + q) dvec[r] = a[q];
     __syncthreads();
-    const T sdr = sqrt(dvec[r]);
+    if (w == 0) {
+        T a[8][8];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int k = k0 + q;
-        if (r >= nb || k >= nb) continue;
-        if (r >= k) {
-            Lp[(int64_t)k * c + r] = (r == k) ? sdr : a[q] * sqrt(dvec[k]);
-            Db[k * nb + r] = x[q] / sdr;
-        } else {
-            Db[k * nb + r] = T(0);
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int v = 0; v < 8; ++v) a[u][v] = S[(8 * bk + v) * kS + 8 * bi + u];
+        int nbad = 0;
+        for (int jb = 0; jb < 8; ++jb) {
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) {
+                const int j = 8 * jb + jj;
+                T* col = cbt + j * kTile;
+                T* dst = (bk == jb) ? col + 8 * bi : dump[0] + 8 * lane;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) dst[u] = a[u][jj];
+                wave_sync();
+                T piv = col[j];
+                T raw[8], cc[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) raw[u] = col[8 * bi + u];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) cc[v] = col[8 * bk + v];
+                const bool bad = !(piv > T(0));
+                nbad += (bad && j < nb) ? 1 : 0;
+                piv = bad ? Huge<T>::v : piv;
+                const T rp = T(1) / piv;
+                T m[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) m[u] = raw[u] * rp;
+                // the next column first: it is published at the next step
+                if (jj < 7) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u][jj + 1] -= m[u] * cc[jj + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int v = 0; v < 8; ++v)
+                        if (v != jj + 1) a[u][v] -= m[u] * cc[v];
+                rps[j] = rp;
+                pivs[j] = piv;
+                __hip_atomic_store(&flag, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        if (lane == 0 && nbad) atomicAdd(err, nbad);
+        // L11[r][k] = l_rk sqrt(d_k) = raw_r(step k) / d_k * sqrt(d_k)
+        const int r = lane;
+        sdv[r] = sqrt(pivs[r]);
+        wave_sync();
+        T* trash = dump[0] + lane;
+#pragma unroll
+        for (int k = 0; k < kTile; ++k) {
+            const T sk = sdv[k];
+            const T f = (r > k) ? rps[k] * sk : T(0);
+            T v = cbt[k * kTile + r] * f;
+            v = (r == k) ? sk : v;
+            T* dst = (r < nb && k < nb) ? P + (int64_t)k * c + r : trash;
+            *dst = v;
+        }
+    } else {
+        T x[8][8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int v = 0; v < 8; ++v) x[u][v] = (8 * bi + u == 8 * bk + v) ? T(1) : T(0);
+        for (int jb = 0; jb < 8; ++jb) {
+            const bool lo = bi > jb, hi = bi >= jb;    // row 8bi+u > j for u <= jj / u > jj
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) {
+                const int j = 8 * jb + jj;
+                T* dx = (bi == jb) ? xrow + 8 * bk : dump[1] + 8 * lane;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) dx[v] = x[jj][v];
+                while (__hip_atomic_load(&flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) <= j)
+                    __builtin_amdgcn_s_sleep(1);
+                const T rp = rps[j];
+                T m[8], xr[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const T mv = cbt[j * kTile + 8 * bi + u] * rp;
+                    m[u] = ((u > jj) ? hi : lo) ? mv : T(0);
+                }
+#pragma unroll
+                for (int v = 0; v < 8; ++v) xr[v] = xrow[8 * bk + v];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) x[u][v] -= m[u] * xr[v];
+                wave_sync();
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int v = 0; v < 8; ++v) S[(8 * bk + v) * kS + 8 * bi + u] = x[u][v];
+        wave_sync();
+        const int r = lane;
+        const T isd = T(1) / sqrt(pivs[r]);
+        T* trash = dump[1] + lane;
+#pragma unroll
+        for (int k = 0; k < kTile; ++k) {
+            const T v = S[k * kS + r] * ((r >= k) ? isd : T(0));
+            T* dst = (r < nb && k < nb) ? Db + k * nb + r : trash;
+            *dst = v;
         }
     }
 }
@@ -545,7 +627,7 @@ void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, h
 template <typename T>
 void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv, int32_t* err,
                        hipStream_t st) {
-    if (count > 0) k_potrf_diag<T><<<count, 512, 0, st>>>(tasks, sym, L, Dinv, err);
+    if (count > 0) k_potrf_diag<T><<<count, 128, 0, st>>>(tasks, sym, L, Dinv, err);
 }
 template <typename T>
 void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, const T* Dinv,
