@@ -618,7 +618,17 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
         h->A.colptr.assign(colptr, colptr + n + 1);
         h->A.rowind.assign(rowind, rowind + nnz);
         h->A.val.assign(val, val + nnz);
+        // phase timing for verbose runs (stderr)
+        auto t_last = std::chrono::steady_clock::now();
+        auto phase = [&](const char* what) {
+            if (!h->opts.verbose) return;
+            const auto now = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "[help] create: %-18s %8.3f s\n", what,
+                         std::chrono::duration<double>(now - t_last).count());
+            t_last = now;
+        };
         h->R = csc_to_csr(h->A);
+        phase("csr");
         if (h->use_pcg) {
             // matrix-free: no ADA^T pattern, ordering or factor
             h->csrcol = h->R.colind;
@@ -629,13 +639,18 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
             h->work_backsub = 12.0 * nnz_d + 8.0 * (nd + 1) + 8.0 * md + 24.0 * nd;
         } else {
             h->P = adat_pattern(h->A, h->R, true);
+            phase("adat pattern");
             std::vector<int64_t> adjptr;
             std::vector<int32_t> adj;
             adat_adjacency(m, h->P, adjptr, adj);
+            phase("adjacency");
             RelaxParams rp;
             h->S = analyze(m, adjptr, adj, {}, rp);
+            phase("analyze");
             build_assembly_maps(h);
+            phase("assembly maps");
             build_schedules(h);
+            phase("schedules");
             if (h->use_fp32) {
                 h->slot_row.resize(h->P.colind.size());
                 for (int32_t i = 0; i < m; ++i)
@@ -643,6 +658,7 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
             }
         }
         if (!h->opts.symbolic_only) setup_device(h);
+        phase("device setup");
         h->prof = h->opts.profile != 0;
     } catch (const std::exception& e) {
         g_create_error = e.what();

@@ -344,9 +344,60 @@ std::vector<int32_t> postorder(const std::vector<int32_t>& parent) {
     return post;
 }
 
+// Column counts of L for a postordered elimination tree (node k of the
+// permuted matrix has postorder number k), by the row-subtree leaf method of
+// Gilbert, Ng and Peyton (1994) in O(nnz(M) alpha):
+//   cc[j] = #{rows i : j in T_i}, T_i = the row subtree of row i, and
+//   1[j in T_i] = sum over x in subtree(j) of delta_i(x), with delta_i = +1 at
+//   every leaf of T_i, -1 at the least common ancestor of consecutive leaves
+//   (postorder), -1 at parent(i), and +1 at i when row i has no entry left
+//   of the diagonal.
+// A neighbour j < i of row i (taken in increasing j) is a leaf of T_i exactly
+// when first[j] (the smallest node of subtree(j)) exceeds the previous
+// neighbour of row i.  The least common ancestor of the previous leaf and j
+// is found in a disjoint-set forest where every finished node is linked to
+// its parent.
 std::vector<int32_t> colcounts(int32_t m, const std::vector<int64_t>& adjptr,
                                const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
                                const std::vector<int32_t>& iperm, const std::vector<int32_t>& parent) {
+    for (int32_t j = 0; j < m; ++j)
+        if (parent[j] != -1 && parent[j] <= j) throw std::runtime_error("colcounts: tree not postordered");
+    std::vector<int32_t> first(m, -1);
+    for (int32_t k = 0; k < m; ++k)
+        for (int32_t x = k; x != -1 && first[x] == -1; x = parent[x]) first[x] = k;
+    std::vector<int32_t> delta(m, 0), prevnbr(m, -1), prevleaf(m, -1), uf(m);
+    for (int32_t x = 0; x < m; ++x) uf[x] = x;
+    auto find = [&](int32_t x) {
+        int32_t r = x;
+        while (uf[r] != r) r = uf[r];
+        while (uf[x] != r) { int32_t nx = uf[x]; uf[x] = r; x = nx; }
+        return r;
+    };
+    for (int32_t j = 0; j < m; ++j) {
+        const int32_t old = perm[j];
+        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
+            const int32_t i = iperm[adj[t]];
+            if (i <= j || prevnbr[i] >= first[j]) { if (i > j) prevnbr[i] = j; continue; }
+            delta[j] += 1;
+            if (prevleaf[i] != -1) delta[find(prevleaf[i])] -= 1;
+            prevleaf[i] = j;
+            prevnbr[i] = j;
+        }
+        if (parent[j] != -1) uf[j] = parent[j];
+    }
+    for (int32_t i = 0; i < m; ++i) {
+        if (parent[i] != -1) delta[parent[i]] -= 1;
+        if (prevleaf[i] == -1) delta[i] += 1;
+    }
+    for (int32_t j = 0; j < m; ++j)
+        if (parent[j] != -1) delta[parent[j]] += delta[j];
+    return delta;
+}
+
+// Reference form (row-subtree walk, O(nnz(L))), kept for the tests.
+std::vector<int32_t> colcounts_walk(int32_t m, const std::vector<int64_t>& adjptr,
+                                    const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
+                                    const std::vector<int32_t>& iperm, const std::vector<int32_t>& parent) {
     std::vector<int32_t> cc(m, 1), mark(m, -1);
     for (int32_t i = 0; i < m; ++i) {   // row subtree of row i of L
         mark[i] = i;

@@ -84,6 +84,10 @@ std::vector<int32_t> postorder(const std::vector<int32_t>& parent);
 std::vector<int32_t> colcounts(int32_t m, const std::vector<int64_t>& adjptr,
                                const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
                                const std::vector<int32_t>& iperm, const std::vector<int32_t>& parent);
+// the same counts by walking every row subtree (O(nnz(L))); test reference
+std::vector<int32_t> colcounts_walk(int32_t m, const std::vector<int64_t>& adjptr,
+                               const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
+                               const std::vector<int32_t>& iperm, const std::vector<int32_t>& parent);
 
 struct RelaxParams {          // merge child into parent when the merged supernode
     int32_t nrelax[3] = {4, 16, 48};       // has fewer columns than nrelax[k] and
