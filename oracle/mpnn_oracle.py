@@ -4,7 +4,9 @@ The same function as the backend's HIP kernels (help_amd/csrc/mpnn.h states
 the math) written densely in torch float64 on the CPU; gradients come from
 autograd, so they check the hand-written backward kernels independently.
 Only tests import this module.  There is no reference counterpart: HELP
-starts PCG from zero (PCG.cpp:309-310).
+starts PCG from zero (PCG.cpp:309-310).  Pinned by tests/test_mpnn_oracle.py
+(closed forms of the zero network and of both aggregations, permutation
+equivariance, scale invariances, finite-difference gradients).
 """
 import numpy as np
 import torch
