@@ -39,7 +39,8 @@ class Opts(C.Structure):
                 ("refine_tol", C.c_double), ("pcg_k", C.c_int32), ("pcg_max_iter", C.c_int32),
                 ("pcg_tol", C.c_double), ("pcg_fp32_precond", C.c_int32),
                 ("fault_flags", C.c_int32), ("verbose", C.c_int32), ("symbolic_only", C.c_int32),
-                ("front_path", C.c_int32), ("profile", C.c_int32), ("refine_fp64", C.c_int32)]
+                ("front_path", C.c_int32), ("profile", C.c_int32), ("refine_fp64", C.c_int32),
+                ("inverse_solve", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -158,10 +159,10 @@ SIGNATURES = {
 
 KERNEL_NAMES = ["k_small_fronts", "k_extend_add", "k_potrf_diag", "k_trsm_rows", "k_gemm", "k_fwd",
                 "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub", "pcg_build",
-                "pcg_solve"]
+                "pcg_solve", "k_xinv", "k_fwd_xinv", "k_bwd_xinv"]
 # work unit of each kernel class: flops for the factor kernels, HBM bytes otherwise
 KERNEL_BOUND = ["mfma", "hbm", "mfma", "mfma", "mfma", "hbm", "hbm", "hbm", "hbm", "hbm", "hbm", "hbm",
-                "hbm", "hbm"]
+                "hbm", "hbm", "mfma", "hbm", "hbm"]
 
 _lib = None
 

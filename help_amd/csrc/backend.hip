@@ -132,6 +132,15 @@ void build_schedules(help_gpu* h) {
     const Supernodal& S = h->S;
     h->sbig.assign(S.nsuper, 0);
     auto ncol = [&](int32_t s) { return S.sfirst[s + 1] - S.sfirst[s]; };
+    // explicit inverses of L11 for supernodes wider than one 64-column block
+    h->use_xinv = h->opts.inverse_solve > 0 || (h->opts.inverse_solve == 0 && h->use_fp32);
+    auto nblk_of = [&](int32_t s) { return (ncol(s) + kTile - 1) / kTile; };
+    auto wide = [&](int32_t s) { return h->use_xinv && nblk_of(s) > 1; };
+    h->xoff.assign(S.nsuper + 1, 0);
+    for (int32_t s = 0; s < S.nsuper; ++s) {
+        const int64_t nb = nblk_of(s);
+        h->xoff[s + 1] = h->xoff[s] + (wide(s) ? nb * (nb - 1) / 2 * kTile * kTile : 0);
+    }
     // diagonal-block inverses: 64x64 blocks, the last one nb x nb
     h->dinvoff.assign(S.nsuper + 1, 0);
     for (int32_t s = 0; s < S.nsuper; ++s) {
@@ -153,6 +162,26 @@ void build_schedules(help_gpu* h) {
         }
         return b;
     };
+    // inverse blocks of a level's wide supernodes (small and large fronts),
+    // one launch per block distance i - j
+    auto push_xinv = [&](int32_t l) {
+        int32_t maxnb = 0;
+        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t)
+            if (wide(S.levelsn[t])) maxnb = std::max(maxnb, nblk_of(S.levelsn[t]));
+        for (int32_t dlt = 1; dlt < maxnb; ++dlt) {
+            int32_t xo = (int32_t)h->xtask.size();
+            double wx = 0;
+            for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
+                int32_t s = S.levelsn[t];
+                if (!wide(s)) continue;
+                for (int32_t i = dlt; i < nblk_of(s); ++i) {
+                    h->xtask.push_back({s, i, i - dlt});
+                    wx += 2.0 * kTile * kTile * kTile * (dlt + 1);
+                }
+            }
+            if ((int32_t)h->xtask.size() > xo) h->fops.push_back({OP_XINV, xo, (int32_t)h->xtask.size() - xo, wx});
+        }
+    };
     for (int32_t l = 0; l < S.nlevels; ++l) {
         std::vector<int32_t> large;
         int32_t off = (int32_t)h->list_small.size();
@@ -172,7 +201,7 @@ void build_schedules(help_gpu* h) {
         }
         if ((int32_t)h->list_small.size() > off)
             h->fops.push_back({OP_SMALL, off, (int32_t)h->list_small.size() - off, w, lds});
-        if (large.empty()) continue;
+        if (large.empty()) { push_xinv(l); continue; }
         // extend-add by 32-column chunks of each front
         off = (int32_t)h->ea.size();
         w = 0;
@@ -221,6 +250,7 @@ void build_schedules(help_gpu* h) {
                 }
         }
         if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
+        push_xinv(l);
     }
     // triangular solves
     // algorithmic bytes of the solves: each stored factor entry is read once
@@ -243,32 +273,47 @@ void build_schedules(help_gpu* h) {
             }
         return ntiny;
     };
+    // bytes of the inverse-based triangle of a wide supernode: the stored
+    // off-diagonal inverse blocks plus the diagonal-block inverses
+    auto xinv_bytes = [&](int32_t s) {
+        double nb = nblk_of(s), n = ncol(s);
+        return wsz * (nb * (nb - 1) / 2 * kTile * kTile + n * kTile);
+    };
     for (int32_t l = 0; l < S.nlevels; ++l) {
-        int32_t off = (int32_t)h->list_fwd.size(), to = (int32_t)h->fgemv.size();
-        double w = 0, wg = 0;
+        int32_t off = (int32_t)h->list_fwd.size(), to = (int32_t)h->fgemv.size(), xo = (int32_t)h->fxs.size();
+        double w = 0, wg = 0, wx = 0;
         int32_t ntiny = level_list(l, h->list_fwd);
         for (int32_t t = off; t < (int32_t)h->list_fwd.size(); ++t) {
             int32_t s = h->list_fwd[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-            bool big = is_big_solve(c, nc);
-            w += tri_bytes(s, !big);
+            bool big = is_big_solve(c, nc) || wide(s);
+            w += wide(s) ? 0.0 : tri_bytes(s, !big);
+            if (wide(s)) {
+                wx += xinv_bytes(s);
+                for (int32_t i = 0; i < nblk_of(s); ++i) h->fxs.push_back({s, i});
+            }
             if (big) {
-                h->sbig[s] = 1;
+                h->sbig[s] = wide(s) ? 2 : 1;
                 wg += wsz * (double)(c - nc) * nc;
                 for (int32_t b = 0; nc + b * 64 < c; ++b) h->fgemv.push_back({s, b});
             }
         }
         h->sops_fwd.push_back({OP_FWD, off, (int32_t)h->list_fwd.size() - off, w, 0, ntiny});
+        if ((int32_t)h->fxs.size() > xo) h->sops_fwd.push_back({OP_FWD_XINV, xo, (int32_t)h->fxs.size() - xo, wx});
         if ((int32_t)h->fgemv.size() > to) h->sops_fwd.push_back({OP_FWD_GEMV, to, (int32_t)h->fgemv.size() - to, wg});
     }
     for (int32_t l = S.nlevels - 1; l >= 0; --l) {
-        int32_t off = (int32_t)h->list_bwd.size(), to = (int32_t)h->bgemv.size();
-        double w = 0, wg = 0;
+        int32_t off = (int32_t)h->list_bwd.size(), to = (int32_t)h->bgemv.size(), xo = (int32_t)h->bxs.size();
+        double w = 0, wg = 0, wx = 0;
         int32_t ntiny = level_list(l, h->list_bwd);
         for (int32_t t = off; t < (int32_t)h->list_bwd.size(); ++t) {
             int32_t s = h->list_bwd[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-            w += tri_bytes(s, !h->sbig[s]);
+            w += wide(s) ? 0.0 : tri_bytes(s, !h->sbig[s]);
+            if (wide(s)) {
+                wx += xinv_bytes(s);
+                for (int32_t j = 0; j < nblk_of(s); ++j) h->bxs.push_back({s, j});
+            }
             if (h->sbig[s]) {
                 wg += wsz * (double)(c - nc) * nc;
                 for (int32_t b = 0; b * 16 < nc; ++b) h->bgemv.push_back({s, b});
@@ -276,6 +321,7 @@ void build_schedules(help_gpu* h) {
         }
         if ((int32_t)h->bgemv.size() > to) h->sops_bwd.push_back({OP_BWD_GEMV, to, (int32_t)h->bgemv.size() - to, wg});
         h->sops_bwd.push_back({OP_BWD, off, (int32_t)h->list_bwd.size() - off, w, 0, ntiny});
+        if ((int32_t)h->bxs.size() > xo) h->sops_bwd.push_back({OP_BWD_XINV, xo, (int32_t)h->bxs.size() - xo, wx});
     }
     // SpMV-class kernels: 12 bytes per nonzero (value + index) plus vectors
     const double nnz = (double)h->nnz, m = h->m, n = h->n;
@@ -354,6 +400,11 @@ void setup_device_factor(help_gpu* h) {
     h->d_gemm.upload(h->gemm, st);
     h->d_fgemv.upload(h->fgemv, st);
     h->d_bgemv.upload(h->bgemv, st);
+    h->d_fxs.upload(h->fxs, st);
+    h->d_bxs.upload(h->bxs, st);
+    h->d_xtask.upload(h->xtask, st);
+    h->d_xoff.upload(h->xoff, st);
+    if (h->use_xinv) h->d_y.alloc(h->m);
     h->d_W.alloc(std::max<int64_t>(1, S.reloff[S.nsuper]));
     int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
     int64_t nd = h->dinvoff[S.nsuper];
@@ -364,10 +415,12 @@ void setup_device_factor(help_gpu* h) {
         h->d_L32.alloc(std::max<int64_t>(1, nl));
         h->d_U32.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv32.alloc(std::max<int64_t>(1, nd));
+        if (h->use_xinv) h->d_X32.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
     } else {
         h->d_L64.alloc(std::max<int64_t>(1, nl));
         h->d_U64.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv64.alloc(std::max<int64_t>(1, nd));
+        if (h->use_xinv) h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
     }
     DevSym& y = h->sym;
     y.nsuper = S.nsuper;
@@ -375,6 +428,7 @@ void setup_device_factor(help_gpu* h) {
     y.rowoff = h->d_rowoff.p; y.rowidx = h->d_rowidx.p; y.reloff = h->d_reloff.p;
     y.relmap = h->d_relmap.p; y.childptr = h->d_childptr.p; y.child = h->d_child.p;
     y.dinvoff = h->d_dinvoff.p;
+    y.xoff = h->d_xoff.p;
 }
 
 void setup_device(help_gpu* h) {
@@ -383,6 +437,13 @@ void setup_device(help_gpu* h) {
     else setup_device_factor(h);
     HIPCHK(hipStreamSynchronize(h->st));
     h->device = true;
+}
+
+// inverse-block storage of the factor precision T
+template <typename T>
+T* xinv_store(help_gpu* h) {
+    if (std::is_same<T, float>::value) return (T*)h->d_X32.p;
+    return (T*)h->d_X64.p;
 }
 
 template <typename T>
@@ -396,6 +457,7 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
             case OP_POTRF: launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, Dinv, h->d_err.p, st); break;
             case OP_TRSM: launch_trsm_rows<T>(h->d_trsm.p + o.off, o.count, h->sym, L, Dinv, st); break;
             case OP_GEMM: launch_gemm<T>(h->d_gemm.p + o.off, o.count, h->sym, L, U, st); break;
+            case OP_XINV: launch_xinv<T>(h->d_xtask.p + o.off, o.count, h->sym, L, Dinv, xinv_store<T>(h), st); break;
             default: break;
         }
     }
@@ -406,14 +468,17 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
 template <typename T>
 void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x) {
     hipStream_t st = h->st;
+    const T* X = xinv_store<T>(h);
     for (const Op& o : h->sops_fwd) {
         ProfScope ps(h, o.kind, o.work);
-        if (o.kind == OP_FWD) launch_fwd<T>(h->d_list_fwd.p + o.off, o.count, o.ntiny, h->sym, h->d_big.p, L, Dinv, x, h->d_W.p, st);
+        if (o.kind == OP_FWD) launch_fwd<T>(h->d_list_fwd.p + o.off, o.count, o.ntiny, h->sym, h->d_big.p, L, Dinv, x, h->d_W.p, h->d_y.p, st);
+        else if (o.kind == OP_FWD_XINV) launch_fwd_xinv<T>(h->d_fxs.p + o.off, o.count, h->sym, Dinv, X, h->d_y.p, x, st);
         else launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, x, h->d_W.p, st);
     }
     for (const Op& o : h->sops_bwd) {
         ProfScope ps(h, o.kind, o.work);
-        if (o.kind == OP_BWD) launch_bwd<T>(h->d_list_bwd.p + o.off, o.count, o.ntiny, h->sym, h->d_big.p, L, Dinv, x, h->d_t.p, st);
+        if (o.kind == OP_BWD) launch_bwd<T>(h->d_list_bwd.p + o.off, o.count, o.ntiny, h->sym, h->d_big.p, L, Dinv, x, h->d_t.p, h->d_y.p, st);
+        else if (o.kind == OP_BWD_XINV) launch_bwd_xinv<T>(h->d_bxs.p + o.off, o.count, h->sym, Dinv, X, h->d_y.p, x, st);
         else launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, st);
     }
     HIPCHK(hipGetLastError());
@@ -506,6 +571,7 @@ void ensure_fp64_factor(help_gpu* h) {
     h->d_L64.alloc(std::max<int64_t>(1, S.loff[S.nsuper]));
     h->d_U64.alloc(std::max<int64_t>(1, S.uoff[S.nsuper]));
     h->d_Dinv64.alloc(std::max<int64_t>(1, h->dinvoff[S.nsuper]));
+    if (h->use_xinv) h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
 }
 
 // ADA^T (+ beta on the diagonal) written into the zeroed factor storage of

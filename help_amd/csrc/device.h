@@ -20,7 +20,18 @@ struct DevSym {
     const int32_t* childptr;  // nsuper+1
     const int32_t* child;
     const int64_t* dinvoff;   // nsuper+1 offsets into the diagonal-block inverses
+    const int64_t* xoff;      // nsuper+1 offsets into the off-diagonal inverse blocks
 };
+
+// Off-diagonal 64x64 block X_ij (i > j) of inv(L11) of a supernode wider than
+// one block, at xoff[s] + (i(i-1)/2 + j) * 4096, column-major, zero outside
+// the supernode's columns.  The diagonal blocks are the Dinv blocks.
+struct XTask {
+    int32_t s, i, j;
+};
+__host__ __device__ inline int64_t xblock_offset(const int64_t* xoff, int32_t s, int32_t i, int32_t j) {
+    return xoff[s] + ((int64_t)i * (i - 1) / 2 + j) * 4096;
+}
 
 // C -= A B^T task on one 64x64 tile of a front.  Front coordinates: rows and
 // columns 0..nrow-1 of supernode s; columns < ncol live in the L panel, the
@@ -117,6 +128,12 @@ template <typename T>
 void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st);
 template <typename T>
 void launch_mfma_probe(const T* a, const T* b, T* out, hipStream_t st);
+// off-diagonal inverse blocks X_ij = -Dinv_i sum_{k=j}^{i-1} L_ik X_kj (X_jj =
+// Dinv_j); the tasks of one launch share i - j, earlier launches hold smaller
+// differences
+template <typename T>
+void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, const T* Dinv, T* X,
+                 hipStream_t st);
 
 // ---- supernodal triangular solves (kernels_solve.hip) ----
 constexpr int kMaxSnCols = 2048;   // supernodes are split to at most this many columns
@@ -129,11 +146,22 @@ struct SolveTask {
 constexpr int kTinyRows = 64;      // supernodes with at most this many rows take one wave
 // Forward: for every supernode of the level gather the children's update
 // vectors and solve with L11; small supernodes also apply L21 (big[s] == 0).
+// big[s] == 2: the supernode is wider than one block and solved with the
+// explicit inverse: the gathered right-hand side goes to y instead and
+// launch_fwd_xinv / launch_bwd_xinv finish it.
 // The first `ntiny` entries of `list` have snrow <= kTinyRows and are
 // solved one per wave, four per workgroup; the rest one per workgroup.
 template <typename T>
 void launch_fwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big,
-                const T* L, const T* Dinv, double* x, double* W, hipStream_t st);
+                const T* L, const T* Dinv, double* x, double* W, double* y, hipStream_t st);
+// Forward with the inverse: x_s(64-row block i) = sum_{j<=i} X_ij y_s(block j).
+template <typename T>
+void launch_fwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
+                     const double* y, double* x, hipStream_t st);
+// Backward with the inverse: x_s(64-column block j) = sum_{i>=j} X_ij^T y_s(block i).
+template <typename T>
+void launch_bwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
+                     const double* y, double* x, hipStream_t st);
 // Forward, big supernodes: W_s += L21 x_s by 64-row blocks.
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
@@ -146,6 +174,6 @@ void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T*
 // `list` and `ntiny` as in launch_fwd.
 template <typename T>
 void launch_bwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big,
-                const T* L, const T* Dinv, double* x, const double* t, hipStream_t st);
+                const T* L, const T* Dinv, double* x, const double* t, double* y, hipStream_t st);
 
 }  // namespace help

@@ -56,7 +56,8 @@ enum OpKind : int32_t {
     OP_SMALL = HELP_K_SMALL_FRONTS, OP_EA = HELP_K_EXTEND_ADD, OP_POTRF = HELP_K_POTRF,
     OP_TRSM = HELP_K_TRSM, OP_GEMM = HELP_K_GEMM,
     OP_FWD = HELP_K_FWD, OP_FWD_GEMV = HELP_K_FWD_GEMV, OP_BWD_GEMV = HELP_K_BWD_GEMV,
-    OP_BWD = HELP_K_BWD
+    OP_BWD = HELP_K_BWD, OP_XINV = HELP_K_XINV, OP_FWD_XINV = HELP_K_FWD_XINV,
+    OP_BWD_XINV = HELP_K_BWD_XINV
 };
 struct Op {
     OpKind kind;
@@ -69,7 +70,7 @@ struct Op {
 inline const char* const kKindName[HELP_K_COUNT] = {
     "k_small_fronts", "k_extend_add", "k_potrf_diag", "k_trsm_rows", "k_gemm",
     "k_fwd", "k_fwd_gemv", "k_bwd_gemv", "k_bwd", "k_adat", "k_rhs", "k_backsub",
-    "pcg_build", "pcg_solve"};
+    "pcg_build", "pcg_solve", "k_xinv", "k_fwd_xinv", "k_bwd_xinv"};
 
 inline double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -87,6 +88,7 @@ using help::GemmTask;
 using help::HipError;
 using help::Op;
 using help::PanelTask;
+using help::XTask;
 using help::SolveTask;
 using help::Supernodal;
 
@@ -109,6 +111,11 @@ struct help_gpu {
     std::vector<PanelTask> potrf, trsm;
     std::vector<GemmTask> gemm;
     std::vector<SolveTask> fgemv, bgemv;
+    // explicit inverse of L11 for supernodes wider than one 64-column block
+    bool use_xinv = false;
+    std::vector<int64_t> xoff;             // nsuper+1 offsets of the inverse blocks
+    std::vector<XTask> xtask;
+    std::vector<SolveTask> fxs, bxs;       // row blocks (forward) / column blocks (backward)
     std::string err;
     bool device = false;
     bool factored = false;
@@ -136,7 +143,11 @@ struct help_gpu {
     DBuf<EaTask> d_ea;
     DBuf<PanelTask> d_potrf, d_trsm;
     DBuf<GemmTask> d_gemm;
-    DBuf<SolveTask> d_fgemv, d_bgemv;
+    DBuf<SolveTask> d_fgemv, d_bgemv, d_fxs, d_bxs;
+    DBuf<XTask> d_xtask;
+    DBuf<int64_t> d_xoff;
+    DBuf<float> d_X32;
+    DBuf<double> d_X64, d_y;
     DevSym sym{};
     // profiling
     bool prof = false;

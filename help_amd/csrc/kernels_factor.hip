@@ -591,6 +591,89 @@ __global__ __launch_bounds__(256) void k_trsm_gemm(const PanelTask* __restrict__
             }
 }
 
+// Off-diagonal inverse block of a wide supernode (i > j):
+//   T = sum_{k=j}^{i-1} L_ik X_kj  (X_jj = Dinv_j),  X_ij = -Dinv_i T,
+// both products on MFMA through LDS.  Rows of block i past the supernode's
+// columns are zero; every block with index below i is a full 64x64 block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_xinv(const XTask* __restrict__ tasks, DevSym sym,
+                                              const T* __restrict__ L, const T* __restrict__ Dinv,
+                                              T* __restrict__ X) {
+    typedef typename Mfma<T>::acc_t acc_t;
+    constexpr int kPer = kTile * kTile / 256;
+    __shared__ T As[kTile * kTile];
+    __shared__ T Bs[kTile * kTile];
+    const XTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s, i = tk.i, j = tk.j;
+    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
+    const int32_t c = sym.snrow[s];
+    const int32_t nbi = min(kTile, n - kTile * i);
+    const T* Lp = L + sym.loff[s];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int li = tid & 63;
+    acc_t acc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < 2; ++q) acc[a][q] = acc_t{0, 0, 0, 0};
+    const int32_t row = kTile * i + min(li, nbi - 1);
+    for (int32_t k = j; k < i; ++k) {
+        // As[kk][r] = L_ik[r][kk], Bs[kk][q] = X_kj[kk][q] (column-major blocks)
+        const T* Xk = (k == j) ? Dinv + sym.dinvoff[s] + (int64_t)j * 4096
+                               : X + xblock_offset(sym.xoff, s, k, j);
+        T va[kPer], vb[kPer];
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int kk = (tid >> 6) + 4 * t;
+            va[t] = Lp[(int64_t)(kTile * k + kk) * c + row];
+            vb[t] = Xk[li * kTile + kk];          // X_kj[kk][q = li]
+        }
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int kk = (tid >> 6) + 4 * t;
+            As[kk * kTile + li] = (li < nbi) ? va[t] : T(0);
+            Bs[kk * kTile + li] = vb[t];
+        }
+        __syncthreads();
+        mfma_tile<T>(As, Bs, kTile, lane, wr, wc, acc);
+        __syncthreads();
+    }
+    // Bs[kk][q] = T[kk][q]; As[kk][r] = Dinv_i[r][kk] (ld nbi, zero outside)
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < 2; ++q)
+            for (int r = 0; r < 4; ++r) {
+                const int rr = wr + a * 16 + Mfma<T>::row(lane, r);
+                const int cc = wc + q * 16 + (lane & 15);
+                Bs[rr * kTile + cc] = acc[a][q][r];
+                acc[a][q][r] = T(0);
+            }
+    {
+        const T* Di = Dinv + sym.dinvoff[s] + (int64_t)i * 4096;
+        T va[kPer];
+        const int rr = min(li, nbi - 1);
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int kk = (tid >> 6) + 4 * t;
+            va[t] = Di[min(kk, nbi - 1) * nbi + rr];
+        }
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int kk = (tid >> 6) + 4 * t;
+            As[kk * kTile + li] = (li < nbi && kk < nbi) ? va[t] : T(0);
+        }
+    }
+    __syncthreads();
+    mfma_tile<T>(As, Bs, kTile, lane, wr, wc, acc);
+    T* Xo = X + xblock_offset(sym.xoff, s, i, j);
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < 2; ++q)
+            for (int r = 0; r < 4; ++r) {
+                const int rr = wr + a * 16 + Mfma<T>::row(lane, r);
+                const int cc = wc + q * 16 + (lane & 15);
+                Xo[cc * kTile + rr] = -acc[a][q][r];
+            }
+}
+
 // one wave: lane l feeds a[l], b[l]; writes its 4 result elements
 template <typename T>
 __global__ void k_mfma_probe(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out) {
@@ -634,6 +717,11 @@ void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, c
                       hipStream_t st) {
     if (count > 0) k_trsm_gemm<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv);
 }
+template <typename T>
+void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, const T* Dinv, T* X,
+                 hipStream_t st) {
+    if (count > 0) k_xinv<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv, X);
+}
 
 #define HELP_INST(T)                                                                                \
     template void launch_small_fronts<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,      \
@@ -642,7 +730,8 @@ void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, c
     template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, hipStream_t);           \
     template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, T*, int32_t*,        \
                                        hipStream_t);                                                \
-    template void launch_trsm_rows<T>(const PanelTask*, int32_t, DevSym, T*, const T*, hipStream_t);
+    template void launch_trsm_rows<T>(const PanelTask*, int32_t, DevSym, T*, const T*, hipStream_t);     \
+    template void launch_xinv<T>(const XTask*, int32_t, DevSym, const T*, const T*, T*, hipStream_t);
 HELP_INST(double)
 HELP_INST(float)
 #undef HELP_INST

@@ -167,7 +167,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_fwd(const int32_t* __restrict__ list, DevSym sym,
                                              const int8_t* __restrict__ big,
                                              const T* __restrict__ L, const T* __restrict__ Dinv,
-                                             double* __restrict__ x, double* __restrict__ W) {
+                                             double* __restrict__ x, double* __restrict__ W,
+                                             double* __restrict__ Y) {
     __shared__ double xs[kMaxLdsCols];
     __shared__ double red[4][64];
     const int32_t s = list[blockIdx.x];
@@ -193,6 +194,10 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t* __restrict__ list, D
             else Ws[rr - n] += v;
         }
         __syncthreads();
+    }
+    if (big[s] == 2) {   // solved with the explicit inverse by k_fwd_xinv
+        for (int32_t t = tid; t < n; t += nt) Y[f + t] = xs[t];
+        return;
     }
     const int32_t rend = big[s] ? n : c;
     for (int32_t jb = 0; jb < n; jb += 64) {
@@ -310,7 +315,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, DevSym sym,
                                              const int8_t* __restrict__ big,
                                              const T* __restrict__ L, const T* __restrict__ Dinv,
-                                             double* __restrict__ x, const double* __restrict__ tin) {
+                                             double* __restrict__ x, const double* __restrict__ tin,
+                                             double* __restrict__ Y) {
     __shared__ double xs[kMaxLdsCols];
     __shared__ double xr[kMaxLdsRows];
     __shared__ double red[4][64];
@@ -323,6 +329,10 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, D
     const int32_t* rows = sym.rowidx + sym.rowoff[s];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
+    if (big[s] == 2) {   // solved with the explicit inverse by k_bwd_xinv
+        for (int32_t t = tid; t < n; t += nt) Y[f + t] = x[f + t] - tin[f + t];
+        return;
+    }
     if (big[s]) {
         for (int32_t t = tid; t < n; t += nt) xs[t] = x[f + t] - tin[f + t];
     } else {
@@ -393,13 +403,113 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, D
     for (int32_t t = tid; t < n; t += nt) x[f + t] = xs[t];
 }
 
+// ------------------------------------------ wide supernodes, explicit inverse
+// Forward, 64-row block i: x[f+64i+r] = sum_{j<=i} sum_q X_ij[r][q] y[f+64j+q]
+// with X_ii = Dinv_i.  Lanes over r, wave w takes q = 16w..16w+15 of every
+// block, the four partial sums combine in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_xinv(const SolveTask* __restrict__ tasks, DevSym sym,
+                                                  const T* __restrict__ Dinv, const T* __restrict__ X,
+                                                  const double* __restrict__ Y, double* __restrict__ x) {
+    __shared__ double ys[kMaxLdsCols];
+    __shared__ double red[4][64];
+    const SolveTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s, i = tk.blk;
+    const int32_t f = sym.sfirst[s];
+    const int32_t n = sym.sfirst[s + 1] - f;
+    const int tid = threadIdx.x, r = tid & 63, w = tid >> 6;
+    const int32_t hi = min(n, 64 * (i + 1));
+    for (int32_t t = tid; t < hi; t += 256) ys[t] = Y[f + t];
+    __syncthreads();
+    double a0 = 0.0, a1 = 0.0;
+    for (int32_t j = 0; j < i; ++j) {
+        const T* Xb = X + xblock_offset(sym.xoff, s, i, j) + r;
+        const double* yj = ys + 64 * j + 16 * w;
+#pragma unroll
+        for (int u = 0; u < 16; u += 2) {
+            a0 += (double)Xb[64 * (16 * w + u)] * yj[u];
+            a1 += (double)Xb[64 * (16 * w + u + 1)] * yj[u + 1];
+        }
+    }
+    {
+        const int32_t nb = min(64, n - 64 * i);
+        const T* Db = Dinv + sym.dinvoff[s] + (int64_t)i * 4096;
+        const int rr = min(r, nb - 1);
+        const double* yi = ys + 64 * i;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = 16 * w + u;
+            const double dv = (double)Db[min(q, nb - 1) * nb + rr];
+            a0 += (r < nb && q < nb) ? dv * yi[min(q, nb - 1)] : 0.0;
+        }
+    }
+    red[w][r] = a0 + a1;
+    __syncthreads();
+    if (w == 0 && 64 * i + r < n) x[f + 64 * i + r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+}
+
+// Backward, 64-column block j: x[f+64j+q] = sum_{i>=j} sum_r X_ij[r][q] y[f+64i+r].
+// Wave w takes the 16 columns q = 16w.., lanes over r; the sum over blocks
+// and lanes has a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_xinv(const SolveTask* __restrict__ tasks, DevSym sym,
+                                                  const T* __restrict__ Dinv, const T* __restrict__ X,
+                                                  const double* __restrict__ Y, double* __restrict__ x) {
+    __shared__ double ys[kMaxLdsCols];
+    const SolveTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s, j = tk.blk;
+    const int32_t f = sym.sfirst[s];
+    const int32_t n = sym.sfirst[s + 1] - f;
+    const int32_t nblk = (n + 63) / 64;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int32_t t = 64 * j + tid; t < n; t += 256) ys[t] = Y[f + t];
+    __syncthreads();
+    double acc[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+    {
+        const int32_t nb = min(64, n - 64 * j);
+        const T* Db = Dinv + sym.dinvoff[s] + (int64_t)j * 4096;
+        const int rr = min(lane, nb - 1);
+        const double yv = lane < nb ? ys[64 * j + rr] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = min(16 * w + u, nb - 1);
+            acc[u] += (double)Db[q * nb + rr] * yv;
+        }
+    }
+    for (int32_t i = j + 1; i < nblk; ++i) {
+        const T* Xb = X + xblock_offset(sym.xoff, s, i, j) + lane + 64 * 16 * w;
+        const int32_t nbi = min(64, n - 64 * i);
+        const double yv = lane < nbi ? ys[64 * i + min(lane, nbi - 1)] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc[u] += (double)Xb[64 * u] * yv;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const double v = wave_sum(acc[u]);
+        const int32_t q = 64 * j + 16 * w + u;
+        if (lane == u && q < n) x[f + q] = v;
+    }
+}
+
 }  // namespace
 
 template <typename T>
 void launch_fwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big, const T* L,
-                const T* Dinv, double* x, double* W, hipStream_t st) {
+                const T* Dinv, double* x, double* W, double* y, hipStream_t st) {
     if (ntiny > 0) k_fwd_tiny<T><<<(ntiny + 3) / 4, 256, 0, st>>>(list, ntiny, sym, L, Dinv, x, W);
-    if (count > ntiny) k_fwd<T><<<count - ntiny, 256, 0, st>>>(list + ntiny, sym, big, L, Dinv, x, W);
+    if (count > ntiny) k_fwd<T><<<count - ntiny, 256, 0, st>>>(list + ntiny, sym, big, L, Dinv, x, W, y);
+}
+template <typename T>
+void launch_fwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
+                     const double* y, double* x, hipStream_t st) {
+    if (count > 0) k_fwd_xinv<T><<<count, 256, 0, st>>>(tasks, sym, Dinv, X, y, x);
+}
+template <typename T>
+void launch_bwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
+                     const double* y, double* x, hipStream_t st) {
+    if (count > 0) k_bwd_xinv<T><<<count, 256, 0, st>>>(tasks, sym, Dinv, X, y, x);
 }
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* x,
@@ -413,20 +523,24 @@ void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T*
 }
 template <typename T>
 void launch_bwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big, const T* L,
-                const T* Dinv, double* x, const double* t, hipStream_t st) {
-    if (count > ntiny) k_bwd<T><<<count - ntiny, 256, 0, st>>>(list + ntiny, sym, big, L, Dinv, x, t);
+                const T* Dinv, double* x, const double* t, double* y, hipStream_t st) {
+    if (count > ntiny) k_bwd<T><<<count - ntiny, 256, 0, st>>>(list + ntiny, sym, big, L, Dinv, x, t, y);
     if (ntiny > 0) k_bwd_tiny<T><<<(ntiny + 3) / 4, 256, 0, st>>>(list, ntiny, sym, L, Dinv, x);
 }
 
 #define HELP_INST(T)                                                                                  \
     template void launch_fwd<T>(const int32_t*, int32_t, int32_t, DevSym, const int8_t*, const T*,   \
-                                const T*, double*, double*, hipStream_t);                             \
+                                const T*, double*, double*, double*, hipStream_t);                    \
+    template void launch_fwd_xinv<T>(const SolveTask*, int32_t, DevSym, const T*, const T*,           \
+                                     const double*, double*, hipStream_t);                            \
+    template void launch_bwd_xinv<T>(const SolveTask*, int32_t, DevSym, const T*, const T*,           \
+                                     const double*, double*, hipStream_t);                            \
     template void launch_fwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,      \
                                      double*, hipStream_t);                                           \
     template void launch_bwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,      \
                                      double*, hipStream_t);                                           \
     template void launch_bwd<T>(const int32_t*, int32_t, int32_t, DevSym, const int8_t*, const T*,    \
-                                const T*, double*, const double*, hipStream_t);
+                                const T*, double*, const double*, double*, hipStream_t);
 HELP_INST(double)
 HELP_INST(float)
 #undef HELP_INST

@@ -70,6 +70,9 @@ typedef struct help_opts {
     int32_t profile;         /* 1: kernel-class profiling from creation on      */
     int32_t refine_fp64;     /* FP64 mode: refinement steps on the FP64 factor
                                 (0: one factor solve, as UseCholmod.cpp does)  */
+    int32_t inverse_solve;   /* supernodes wider than 64 columns solve their
+                                dense triangle with an explicit inverse of L11:
+                                0 in the FP32 mode only, 1 always, -1 never    */
 } help_opts;
 
 /* Defaults: FP64 direct, beta=1e-5, refine 20 / 1e-12 (FP32 mode; FP64 mode
@@ -183,7 +186,10 @@ int help_gpu_factor_column(help_gpu* h, int32_t j, int32_t* rows,
 #define HELP_K_BACKSUB      11   /* x1 = D^-1 (b1 - A^T x2) (bytes)         */
 #define HELP_K_PCG_BUILD    12   /* PCG mode: partial-Cholesky build per renew */
 #define HELP_K_PCG_SOLVE    13   /* PCG mode: conjugate gradients per solve  */
-#define HELP_K_COUNT        14
+#define HELP_K_XINV         14   /* inverse blocks of wide supernodes (flops) */
+#define HELP_K_FWD_XINV     15   /* forward solve with the inverse (bytes)   */
+#define HELP_K_BWD_XINV     16   /* backward solve with the inverse (bytes)  */
+#define HELP_K_COUNT        17
 /* ------------------------------------------------------------------ */
 /* learned warm start for the PCG mode (no reference counterpart: the  */
 /* initial guess of PCG.cpp:309-310 is zero)                           */

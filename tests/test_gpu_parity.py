@@ -197,6 +197,34 @@ def test_solve_mcf_large_fronts():
     be.destruct()
 
 
+@pytest.mark.parametrize("mode", [L.MODE_DIRECT_FP64, L.MODE_DIRECT_FP32])
+def test_solve_with_explicit_inverse(mode):
+    """Supernodes wider than one 64-column block solved through the explicit
+    inverse of L11 (always on in the FP32 mode, forced here in FP64 too):
+    solutions within 1e-10 of the oracle on well-conditioned d."""
+    inst = instances.mcf(side=60, commodities=4, pairs=10, seed=7)
+    be = LinearEquation(mode=mode, inverse_solve=1)
+    be.construct(inst)
+    sf, _, _, _ = be.supernodes()
+    wide = int((np.diff(sf) > 64).sum())
+    _flush(f"explicit inverse mode={mode}: {wide} supernodes wider than 64 columns")
+    assert wide > 0
+    rng = np.random.default_rng(12)
+    d = log_uniform(rng, inst.n, -2, 2)
+    b1 = rng.standard_normal(inst.n)
+    b2 = rng.standard_normal(inst.m)
+    be.renew(d)
+    x1, x2 = be.solve(d, b1, b2)
+    perm, _, _ = be.symbolic_arrays()
+    Lp, Li, Lx = _oracle_factor(inst, perm, d)
+    o1, o2 = O.solve_kkt(inst, perm, Lp, Li, Lx, d, b1, b2)
+    e2 = np.linalg.norm(x2 - o2) / np.linalg.norm(o2)
+    e1 = np.linalg.norm(x1 - o1) / np.linalg.norm(o1)
+    _flush(f"  dy err {e2:.3e}  x1 err {e1:.3e}  refine {be.solve_stats.refine_steps}")
+    assert e2 <= 1e-10 and e1 <= 1e-10
+    be.destruct()
+
+
 @pytest.mark.parametrize("mode,device", [(L.MODE_DIRECT_FP64, False), (L.MODE_DIRECT_FP32, False),
                                          (L.MODE_DIRECT_FP64, True), (L.MODE_DIRECT_FP32, True)])
 def test_hsd_end_to_end(cfg0, mode, device):
