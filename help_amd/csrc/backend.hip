@@ -163,20 +163,26 @@ void build_schedules(help_gpu* h) {
         return b;
     };
     // inverse blocks of a level's wide supernodes (small and large fronts),
-    // one launch per block distance i - j
+    // right-looking: the launch for step t finalises X_kj with k = j + t and
+    // adds its terms to the partial sums of the blocks below
     auto push_xinv = [&](int32_t l) {
         int32_t maxnb = 0;
         for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t)
             if (wide(S.levelsn[t])) maxnb = std::max(maxnb, nblk_of(S.levelsn[t]));
-        for (int32_t dlt = 1; dlt < maxnb; ++dlt) {
+        for (int32_t step = 0; step < maxnb; ++step) {
             int32_t xo = (int32_t)h->xtask.size();
             double wx = 0;
             for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
                 int32_t s = S.levelsn[t];
                 if (!wide(s)) continue;
-                for (int32_t i = dlt; i < nblk_of(s); ++i) {
-                    h->xtask.push_back({s, i, i - dlt});
-                    wx += 2.0 * kTile * kTile * kTile * (dlt + 1);
+                const int32_t nb = nblk_of(s);
+                for (int32_t j = 0; j + step < nb; ++j) {
+                    const int32_t k = j + step;
+                    for (int32_t i = k; i < nb; ++i) {
+                        if (step == 0 && i == j) continue;
+                        h->xtask.push_back({s, i, j, k});
+                        wx += 2.0 * kTile * kTile * kTile * ((step > 0 ? 1 : 0) + (i > k ? 1 : 0));
+                    }
                 }
             }
             if ((int32_t)h->xtask.size() > xo) h->fops.push_back({OP_XINV, xo, (int32_t)h->xtask.size() - xo, wx});
@@ -415,12 +421,18 @@ void setup_device_factor(help_gpu* h) {
         h->d_L32.alloc(std::max<int64_t>(1, nl));
         h->d_U32.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv32.alloc(std::max<int64_t>(1, nd));
-        if (h->use_xinv) h->d_X32.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
+        if (h->use_xinv) {
+            h->d_X32.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
+            h->d_S32.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
+        }
     } else {
         h->d_L64.alloc(std::max<int64_t>(1, nl));
         h->d_U64.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv64.alloc(std::max<int64_t>(1, nd));
-        if (h->use_xinv) h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
+        if (h->use_xinv) {
+            h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
+            h->d_S64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
+        }
     }
     DevSym& y = h->sym;
     y.nsuper = S.nsuper;
@@ -439,11 +451,16 @@ void setup_device(help_gpu* h) {
     h->device = true;
 }
 
-// inverse-block storage of the factor precision T
+// inverse-block storage of the factor precision T, and its partial sums
 template <typename T>
 T* xinv_store(help_gpu* h) {
     if (std::is_same<T, float>::value) return (T*)h->d_X32.p;
     return (T*)h->d_X64.p;
+}
+template <typename T>
+T* xinv_partial(help_gpu* h) {
+    if (std::is_same<T, float>::value) return (T*)h->d_S32.p;
+    return (T*)h->d_S64.p;
 }
 
 template <typename T>
@@ -457,7 +474,7 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
             case OP_POTRF: launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, Dinv, h->d_err.p, st); break;
             case OP_TRSM: launch_trsm_rows<T>(h->d_trsm.p + o.off, o.count, h->sym, L, Dinv, st); break;
             case OP_GEMM: launch_gemm<T>(h->d_gemm.p + o.off, o.count, h->sym, L, U, st); break;
-            case OP_XINV: launch_xinv<T>(h->d_xtask.p + o.off, o.count, h->sym, L, Dinv, xinv_store<T>(h), st); break;
+            case OP_XINV: launch_xinv<T>(h->d_xtask.p + o.off, o.count, h->sym, L, Dinv, xinv_store<T>(h), xinv_partial<T>(h), st); break;
             default: break;
         }
     }
@@ -571,7 +588,10 @@ void ensure_fp64_factor(help_gpu* h) {
     h->d_L64.alloc(std::max<int64_t>(1, S.loff[S.nsuper]));
     h->d_U64.alloc(std::max<int64_t>(1, S.uoff[S.nsuper]));
     h->d_Dinv64.alloc(std::max<int64_t>(1, h->dinvoff[S.nsuper]));
-    if (h->use_xinv) h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
+    if (h->use_xinv) {
+        h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
+        h->d_S64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
+    }
 }
 
 // ADA^T (+ beta on the diagonal) written into the zeroed factor storage of

@@ -27,7 +27,7 @@ struct DevSym {
 // one block, at xoff[s] + (i(i-1)/2 + j) * 4096, column-major, zero outside
 // the supernode's columns.  The diagonal blocks are the Dinv blocks.
 struct XTask {
-    int32_t s, i, j;
+    int32_t s, i, j, k;   // block (i, j) at the step that finalises X_kj (j <= k <= i)
 };
 __host__ __device__ inline int64_t xblock_offset(const int64_t* xoff, int32_t s, int32_t i, int32_t j) {
     return xoff[s] + ((int64_t)i * (i - 1) / 2 + j) * 4096;
@@ -128,11 +128,12 @@ template <typename T>
 void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st);
 template <typename T>
 void launch_mfma_probe(const T* a, const T* b, T* out, hipStream_t st);
-// off-diagonal inverse blocks X_ij = -Dinv_i sum_{k=j}^{i-1} L_ik X_kj (X_jj =
-// Dinv_j); the tasks of one launch share i - j, earlier launches hold smaller
-// differences
+// off-diagonal inverse blocks X_ij = -Dinv_i S_ij, S_ij = sum_{k=j}^{i-1} L_ik X_kj
+// (X_jj = Dinv_j), right-looking: the launch for step t finalises X_kj,
+// k = j + t, and adds L_ik X_kj to the partial sums S_ij of every i > k (S is
+// written at t = 0 and accumulated in k order afterwards)
 template <typename T>
-void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, const T* Dinv, T* X,
+void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, const T* Dinv, T* X, T* S,
                  hipStream_t st);
 
 // ---- supernodal triangular solves (kernels_solve.hip) ----

@@ -146,8 +146,8 @@ struct help_gpu {
     DBuf<SolveTask> d_fgemv, d_bgemv, d_fxs, d_bxs;
     DBuf<XTask> d_xtask;
     DBuf<int64_t> d_xoff;
-    DBuf<float> d_X32;
-    DBuf<double> d_X64, d_y;
+    DBuf<float> d_X32, d_S32;              // inverse blocks and their partial sums
+    DBuf<double> d_X64, d_S64, d_y;
     DevSym sym{};
     // profiling
     bool prof = false;

@@ -591,86 +591,115 @@ __global__ __launch_bounds__(256) void k_trsm_gemm(const PanelTask* __restrict__
             }
 }
 
-// Off-diagonal inverse block of a wide supernode (i > j):
-//   T = sum_{k=j}^{i-1} L_ik X_kj  (X_jj = Dinv_j),  X_ij = -Dinv_i T,
-// both products on MFMA through LDS.  Rows of block i past the supernode's
-// columns are zero; every block with index below i is a full 64x64 block.
+// acc += As[k][rows] * Bs[k][cols]^T as mfma_tile, with a padded Bs (leading
+// dimension ldb) so that column-major blocks can be staged with coalesced loads
+template <typename T>
+__device__ inline void mfma_tile_ld(const T* As, const T* Bs, int ldb, int lane, int wr, int wc,
+                                    typename Mfma<T>::acc_t (&acc)[2][2]) {
+    for (int kk = 0; kk < kTile; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        T a0 = As[kr * kTile + wr + (lane & 15)];
+        T a1 = As[kr * kTile + wr + 16 + (lane & 15)];
+        T b0 = Bs[kr * ldb + wc + (lane & 15)];
+        T b1 = Bs[kr * ldb + wc + 16 + (lane & 15)];
+        acc[0][0] = Mfma<T>::fma(a0, b0, acc[0][0]);
+        acc[0][1] = Mfma<T>::fma(a0, b1, acc[0][1]);
+        acc[1][0] = Mfma<T>::fma(a1, b0, acc[1][0]);
+        acc[1][1] = Mfma<T>::fma(a1, b1, acc[1][1]);
+    }
+}
+
+// One step of the right-looking inverse of a wide supernode's L11, task
+// (i, j, k = j + t):
+//   t = 0:  Xk = Dinv_j                       and S_ij  = L_ij Xk   (i > j)
+//   t > 0:  Xk = -Dinv_k S_kj  (= X_kj);  i == k: store X_kj,
+//                                         i >  k: S_ij += L_ik Xk
+// Every task recomputes Xk from S_kj, which no task of the launch writes, so
+// the tasks are independent; S_ij gains its terms in k order.  Blocks are
+// column-major 64x64; rows and columns past the supernode are zero.
 template <typename T>
 __global__ __launch_bounds__(256) void k_xinv(const XTask* __restrict__ tasks, DevSym sym,
                                               const T* __restrict__ L, const T* __restrict__ Dinv,
-                                              T* __restrict__ X) {
+                                              T* __restrict__ X, T* __restrict__ Sb) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int kPer = kTile * kTile / 256;
+    constexpr int kLdb = kTile + 1;
     __shared__ T As[kTile * kTile];
-    __shared__ T Bs[kTile * kTile];
+    __shared__ T Bs[kTile * kLdb];
     const XTask tk = tasks[blockIdx.x];
-    const int32_t s = tk.s, i = tk.i, j = tk.j;
+    const int32_t s = tk.s, i = tk.i, j = tk.j, k = tk.k;
     const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
     const int32_t c = sym.snrow[s];
-    const int32_t nbi = min(kTile, n - kTile * i);
-    const T* Lp = L + sym.loff[s];
+    const int32_t nbi = min(kTile, n - kTile * i), nbk = min(kTile, n - kTile * k);
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    const int li = tid & 63;
+    const int li = tid & 63, lk = tid >> 6;
     acc_t acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int q = 0; q < 2; ++q) acc[a][q] = acc_t{0, 0, 0, 0};
-    const int32_t row = kTile * i + min(li, nbi - 1);
-    for (int32_t k = j; k < i; ++k) {
-        // As[kk][r] = L_ik[r][kk], Bs[kk][q] = X_kj[kk][q] (column-major blocks)
-        const T* Xk = (k == j) ? Dinv + sym.dinvoff[s] + (int64_t)j * 4096
-                               : X + xblock_offset(sym.xoff, s, k, j);
+    if (k == j) {
+        // Bs[kk][q] = Dinv_j[kk][q] (a full block: j < i)
+        const T* Dj = Dinv + sym.dinvoff[s] + (int64_t)j * 4096;
+        T vb[kPer];
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) vb[t] = Dj[(lk + 4 * t) * kTile + li];   // element (li, q)
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) Bs[li * kLdb + lk + 4 * t] = vb[t];
+    } else {
+        // Xk = -Dinv_k S_kj: As[kk][r] = Dinv_k[r][kk] (ld nbk), Bs[kk][q] = S_kj[kk][q]
+        const T* Dk = Dinv + sym.dinvoff[s] + (int64_t)k * 4096;
+        const T* Skj = Sb + xblock_offset(sym.xoff, s, k, j);
+        const int rr = min(li, nbk - 1);
         T va[kPer], vb[kPer];
 #pragma unroll
         for (int t = 0; t < kPer; ++t) {
-            const int kk = (tid >> 6) + 4 * t;
-            va[t] = Lp[(int64_t)(kTile * k + kk) * c + row];
-            vb[t] = Xk[li * kTile + kk];          // X_kj[kk][q = li]
+            const int kk = lk + 4 * t;
+            va[t] = Dk[min(kk, nbk - 1) * nbk + rr];
+            vb[t] = Skj[kk * kTile + li];                                    // element (li, kk)
         }
 #pragma unroll
         for (int t = 0; t < kPer; ++t) {
-            const int kk = (tid >> 6) + 4 * t;
-            As[kk * kTile + li] = (li < nbi) ? va[t] : T(0);
-            Bs[kk * kTile + li] = vb[t];
+            const int kk = lk + 4 * t;
+            As[kk * kTile + li] = (li < nbk && kk < nbk) ? va[t] : T(0);
+            Bs[li * kLdb + kk] = vb[t];
         }
         __syncthreads();
-        mfma_tile<T>(As, Bs, kTile, lane, wr, wc, acc);
+        mfma_tile_ld<T>(As, Bs, kLdb, lane, wr, wc, acc);
         __syncthreads();
+        T* Xkj = X + xblock_offset(sym.xoff, s, k, j);
+        for (int a = 0; a < 2; ++a)
+            for (int q = 0; q < 2; ++q)
+                for (int r = 0; r < 4; ++r) {
+                    const int rw = wr + a * 16 + Mfma<T>::row(lane, r);
+                    const int cl = wc + q * 16 + (lane & 15);
+                    const T v = -acc[a][q][r];
+                    Bs[rw * kLdb + cl] = v;       // Xk[rw][cl] feeds the next product
+                    if (i == k) Xkj[cl * kTile + rw] = v;
+                    acc[a][q][r] = T(0);
+                }
+        if (i == k) return;
     }
-    // Bs[kk][q] = T[kk][q]; As[kk][r] = Dinv_i[r][kk] (ld nbi, zero outside)
-    for (int a = 0; a < 2; ++a)
-        for (int q = 0; q < 2; ++q)
-            for (int r = 0; r < 4; ++r) {
-                const int rr = wr + a * 16 + Mfma<T>::row(lane, r);
-                const int cc = wc + q * 16 + (lane & 15);
-                Bs[rr * kTile + cc] = acc[a][q][r];
-                acc[a][q][r] = T(0);
-            }
+    // S_ij (+)= L_ik Xk: As[kk][r] = L_ik[r][kk]
     {
-        const T* Di = Dinv + sym.dinvoff[s] + (int64_t)i * 4096;
+        const T* Lp = L + sym.loff[s];
+        const int32_t row = kTile * i + min(li, nbi - 1);
         T va[kPer];
-        const int rr = min(li, nbi - 1);
 #pragma unroll
-        for (int t = 0; t < kPer; ++t) {
-            const int kk = (tid >> 6) + 4 * t;
-            va[t] = Di[min(kk, nbi - 1) * nbi + rr];
-        }
+        for (int t = 0; t < kPer; ++t) va[t] = Lp[(int64_t)(kTile * k + lk + 4 * t) * c + row];
 #pragma unroll
-        for (int t = 0; t < kPer; ++t) {
-            const int kk = (tid >> 6) + 4 * t;
-            As[kk * kTile + li] = (li < nbi && kk < nbi) ? va[t] : T(0);
-        }
+        for (int t = 0; t < kPer; ++t) As[(lk + 4 * t) * kTile + li] = (li < nbi) ? va[t] : T(0);
     }
     __syncthreads();
-    mfma_tile<T>(As, Bs, kTile, lane, wr, wc, acc);
-    T* Xo = X + xblock_offset(sym.xoff, s, i, j);
+    mfma_tile_ld<T>(As, Bs, kLdb, lane, wr, wc, acc);
+    T* Sij = Sb + xblock_offset(sym.xoff, s, i, j);
     for (int a = 0; a < 2; ++a)
         for (int q = 0; q < 2; ++q)
             for (int r = 0; r < 4; ++r) {
-                const int rr = wr + a * 16 + Mfma<T>::row(lane, r);
-                const int cc = wc + q * 16 + (lane & 15);
-                Xo[cc * kTile + rr] = -acc[a][q][r];
+                const int rw = wr + a * 16 + Mfma<T>::row(lane, r);
+                const int cl = wc + q * 16 + (lane & 15);
+                const T prev = (k == j) ? T(0) : Sij[cl * kTile + rw];
+                Sij[cl * kTile + rw] = prev + acc[a][q][r];
             }
 }
 
@@ -718,9 +747,9 @@ void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, c
     if (count > 0) k_trsm_gemm<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv);
 }
 template <typename T>
-void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, const T* Dinv, T* X,
+void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, const T* Dinv, T* X, T* S,
                  hipStream_t st) {
-    if (count > 0) k_xinv<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv, X);
+    if (count > 0) k_xinv<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv, X, S);
 }
 
 #define HELP_INST(T)                                                                                \
@@ -731,7 +760,7 @@ void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, cons
     template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, T*, int32_t*,        \
                                        hipStream_t);                                                \
     template void launch_trsm_rows<T>(const PanelTask*, int32_t, DevSym, T*, const T*, hipStream_t);     \
-    template void launch_xinv<T>(const XTask*, int32_t, DevSym, const T*, const T*, T*, hipStream_t);
+    template void launch_xinv<T>(const XTask*, int32_t, DevSym, const T*, const T*, T*, T*, hipStream_t);
 HELP_INST(double)
 HELP_INST(float)
 #undef HELP_INST
