@@ -18,6 +18,25 @@
 #include "handle.h"
 
 namespace help {
+
+// acc + sum_p A_pj v[row_p] over column j in ascending p, with the loads of
+// four terms in flight (indices past the column clamped, terms dropped)
+__device__ inline double col_dot_acc(const int64_t* cptr, const int32_t* crow, const double* cval,
+                                     const double* v, int64_t j, double acc) {
+    const int64_t b = cptr[j], e = cptr[j + 1];
+    for (int64_t p = b; p < e; p += 4) {
+        double t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t q = p + u < e ? p + u : e - 1;
+            t[u] = cval[q] * v[crow[q]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = (p + u < e) ? acc + t[u] : acc;
+    }
+    return acc;
+}
+
 namespace {
 
 constexpr int kRedBlocks = 512;
@@ -229,8 +248,7 @@ struct HsdGpu {
             else if (j < nlb + nfr) acc = -cp[j] * tau_;
             else acc = 0.0;
             if (j < nub) acc += yup[j];
-            for (int64_t p = cptr[j]; p < cptr[j + 1]; ++p) acc += cval[p] * yp[crow[p]];
-            rdp[j] = acc;
+            rdp[j] = col_dot_acc(cptr, crow, cval, yp, j, acc);
         });
         double cTx = dot(n_lb + n_fr, cp, xp);
         double bTy = dot(m, bp, yp) + dot(n_ub, upp, yup);
@@ -379,8 +397,7 @@ struct HsdGpu {
                 int64_t i = j - nlb;
                 acc += -(cp[i] - rt * xfp[i] * xp[j]) * dt - rho_ * xfp[i] * odx[j];
             }
-            for (int64_t p = cptr[j]; p < cptr[j + 1]; ++p) acc += cval[p] * ody[crow[p]];
-            d1[j] = acc;
+            d1[j] = col_dot_acc(cptr, crow, cval, ody, j, acc);
         });
         // dir2 = eta r_p + b dt - A dx
         {

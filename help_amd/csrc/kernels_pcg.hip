@@ -52,10 +52,18 @@ __global__ __launch_bounds__(256) void k_csr_light(int32_t m, const int64_t* __r
     if (i >= m) return;
     const int64_t b = rowptr[i], e = rowptr[i + 1];
     if (e - b > kHeavyRow) return;
+    // 8 terms' loads in flight (indices past the row clamped, terms dropped)
     double acc = 0.0;
-    for (int64_t p = b; p < e; ++p) {
-        const double v = val[p];
-        acc += (SQ ? v * v : v) * x[colind[p]];
+    for (int64_t p = b; p < e; p += 8) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int64_t q = p + u < e ? p + u : e - 1;
+            const double v = val[q];
+            t[u] = (SQ ? v * v : v) * x[colind[q]];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = (p + u < e) ? acc + t[u] : acc;
     }
     y[i] = acc;
 }
@@ -90,8 +98,18 @@ __global__ void k_csc_ts(int32_t n, const int64_t* __restrict__ colptr, const in
                          const double* __restrict__ x, double* __restrict__ t) {
     const int32_t j = blockIdx.x * kB + threadIdx.x;
     if (j >= n) return;
+    const int64_t b = colptr[j], e = colptr[j + 1];
     double acc = 0.0;
-    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) acc += val[p] * x[rowind[p]];
+    for (int64_t p = b; p < e; p += 4) {
+        double tt[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t q = p + u < e ? p + u : e - 1;
+            tt[u] = val[q] * x[rowind[q]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = (p + u < e) ? acc + tt[u] : acc;
+    }
     t[j] = acc * dinv[j];
 }
 

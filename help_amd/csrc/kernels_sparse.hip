@@ -18,6 +18,25 @@ constexpr int kBlock = 256;
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// acc <- acc (+/-) term(t) for t = b..e-1, in that order.  The loads of U
+// consecutive terms are issued together: indices past the end are clamped
+// to e-1 (an in-range duplicate) and their terms dropped by a select, so no
+// branch separates the loads and the rounding is that of the plain loop.
+template <int U, bool SUB, typename F>
+__device__ inline double seq_acc(int64_t b, int64_t e, double acc, F term) {
+    for (int64_t t = b; t < e; t += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = term(t + u < e ? t + u : e - 1);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double nx = SUB ? acc - v[u] : acc + v[u];
+            acc = (t + u < e) ? nx : acc;
+        }
+    }
+    return acc;
+}
+
 __global__ void k_dinv(const double* __restrict__ d, double* __restrict__ dinv, int32_t n) {
     int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n) dinv[j] = 1.0 / d[j];   // UsePardiso.cpp:172-173
@@ -34,13 +53,11 @@ __global__ void k_adat(const int64_t* __restrict__ pairptr, const int32_t* __res
                        float* __restrict__ outf) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
-    double acc = 0.0;
-    int64_t t1 = pairptr[s + 1];
-    for (int64_t t = pairptr[s]; t < t1; ++t) {
-        int32_t p = pp[t], q = pq[t];
-        double vpdj = val[p] * dinv[colof[p]];
-        acc = acc + vpdj * val[q];
-    }
+    const double acc = seq_acc<4, false>(pairptr[s], pairptr[s + 1], 0.0, [&](int64_t t) {
+        const int32_t p = pp[t], q = pq[t];
+        const double vpdj = val[p] * dinv[colof[p]];
+        return vpdj * val[q];
+    });
     int64_t o = dest ? dest[s] : s;
     if (outd) outd[o] = acc;
     else outf[o] = (float)acc;
@@ -66,13 +83,11 @@ __global__ void k_adat_scaled_f32(const int64_t* __restrict__ pairptr, const int
                                   const int64_t* __restrict__ dest, float* __restrict__ outf) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
-    double acc = 0.0;
-    int64_t t1 = pairptr[s + 1];
-    for (int64_t t = pairptr[s]; t < t1; ++t) {
-        int32_t p = pp[t], q = pq[t];
-        double vpdj = val[p] * dinv[colof[p]];
-        acc = acc + vpdj * val[q];
-    }
+    const double acc = seq_acc<4, false>(pairptr[s], pairptr[s + 1], 0.0, [&](int64_t t) {
+        const int32_t p = pp[t], q = pq[t];
+        const double vpdj = val[p] * dinv[colof[p]];
+        return vpdj * val[q];
+    });
     outf[dest[s]] = (float)(acc * scale[slot_row[s]] * scale[slot_col[s]]);
 }
 
@@ -102,9 +117,8 @@ __global__ void k_spmv_csr(int32_t m, const int64_t* __restrict__ rowptr,
                            const double* __restrict__ x, double* __restrict__ y) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    double acc = 0.0;
-    for (int64_t t = rowptr[i]; t < rowptr[i + 1]; ++t) acc = acc + val[t] * x[colind[t]];
-    y[i] = acc;
+    y[i] = seq_acc<8, false>(rowptr[i], rowptr[i + 1], 0.0,
+                             [&](int64_t t) { return val[t] * x[colind[t]]; });
 }
 
 __global__ void k_spmv_csc_t(int32_t n, const int64_t* __restrict__ colptr,
@@ -112,9 +126,8 @@ __global__ void k_spmv_csc_t(int32_t n, const int64_t* __restrict__ colptr,
                              const double* __restrict__ x, double* __restrict__ y) {
     int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    double acc = 0.0;
-    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) acc = acc + val[p] * x[rowind[p]];
-    y[j] = acc;
+    y[j] = seq_acc<4, false>(colptr[j], colptr[j + 1], 0.0,
+                             [&](int64_t p) { return val[p] * x[rowind[p]]; });
 }
 
 // r_i = -b2_i + sum_j A_ij (b1_j / d_j), columns ascending (cholmod_sdmult
@@ -127,12 +140,11 @@ __global__ void k_rhs(int32_t m, const int64_t* __restrict__ rowptr,
                       double* __restrict__ rp) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    double acc = -b2[i];
-    for (int64_t t = rowptr[i]; t < rowptr[i + 1]; ++t) {
-        int32_t j = colind[t];
-        double tj = b1[j] / d[j];
-        acc = acc + csrval[t] * tj;
-    }
+    const double acc = seq_acc<8, false>(rowptr[i], rowptr[i + 1], -b2[i], [&](int64_t t) {
+        const int32_t j = colind[t];
+        const double tj = b1[j] / d[j];
+        return csrval[t] * tj;
+    });
     rp[iperm[i]] = acc;
 }
 
@@ -144,8 +156,8 @@ __global__ void k_backsub(int32_t n, const int64_t* __restrict__ colptr,
                           const double* __restrict__ x2, double* __restrict__ x1) {
     int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    double acc = b1[j];
-    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) acc = acc - val[p] * x2[rowind[p]];
+    const double acc = seq_acc<4, true>(colptr[j], colptr[j + 1], b1[j],
+                                        [&](int64_t p) { return val[p] * x2[rowind[p]]; });
     x1[j] = acc / d[j];
 }
 
@@ -167,8 +179,8 @@ __global__ void k_csc_t_scaled(int32_t n, const int64_t* __restrict__ colptr,
                                double* __restrict__ y) {
     int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    double acc = 0.0;
-    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) acc = acc + val[p] * x[rowind[p]];
+    const double acc = seq_acc<4, false>(colptr[j], colptr[j + 1], 0.0,
+                                         [&](int64_t p) { return val[p] * x[rowind[p]]; });
     y[j] = acc * dinv[j];
 }
 
@@ -178,8 +190,8 @@ __global__ void k_csr_plus_shift(int32_t m, const int64_t* __restrict__ rowptr,
                                  const double* __restrict__ x, double* __restrict__ y) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    double acc = 0.0;
-    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) acc = acc + val[k] * t[colind[k]];
+    const double acc = seq_acc<8, false>(rowptr[i], rowptr[i + 1], 0.0,
+                                         [&](int64_t k) { return val[k] * t[colind[k]]; });
     y[i] = acc + beta * x[i];
 }
 
