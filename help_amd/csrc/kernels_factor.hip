@@ -173,26 +173,50 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
         // rel is ascending: the child columns that land in [c0, c1)
         int32_t lo = 0, hi = chc;
         while (lo < hi) { int32_t mid = (lo + hi) >> 1; if (rel[mid] < tk.c0) lo = mid + 1; else hi = mid; }
-        for (int32_t jj = lo + wave; jj < chc; jj += 4) {
-            const int32_t cc = rel[jj];
-            if (cc >= tk.c1) break;
-            const T* col = Uc + (int64_t)jj * chc;
-            T* dst = (cc < n) ? Lp + (int64_t)cc * c : Up + (int64_t)(cc - n) * cu - n;
-            for (int32_t i0 = jj + lane; i0 < chc; i0 += 256) {
-                int32_t rr[4];
-                T v[4];
+        // count of child columns that land in [c0, c1)
+        int32_t hi2 = lo;
+        {
+            int32_t a = lo, b = chc;
+            while (a < b) { int32_t mid = (a + b) >> 1; if (rel[mid] < tk.c1) a = mid + 1; else b = mid; }
+            hi2 = a;
+        }
+        // each wave takes two child columns at a time, 4 rows of each per lane
+        // in flight; all loads are issued before the read-modify-writes
+        for (int32_t j0 = lo + wave; j0 < hi2; j0 += 8) {
+            const int32_t jA = j0, jB = min(j0 + 4, hi2 - 1);
+            const bool hasB = j0 + 4 < hi2;
+            const int32_t ccA = rel[jA], ccB = rel[jB];
+            const T* colA = Uc + (int64_t)jA * chc;
+            const T* colB = Uc + (int64_t)jB * chc;
+            T* dstA = (ccA < n) ? Lp + (int64_t)ccA * c : Up + (int64_t)(ccA - n) * cu - n;
+            T* dstB = (ccB < n) ? Lp + (int64_t)ccB * c : Up + (int64_t)(ccB - n) * cu - n;
+            const int32_t endA = chc, endB = hasB ? chc : jB;   // B empty when absent
+            for (int32_t i0 = lane; jA + i0 < endA || jB + i0 < endB; i0 += 256) {
+                int32_t ra[4], rb[4];
+                T va[4], vb[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int32_t ii = i0 + 64 * u;
-                    rr[u] = (ii < chc) ? rel[ii] : -1;
-                    v[u] = (ii < chc) ? col[ii] : T(0);
+                    const int32_t ia = jA + i0 + 64 * u, ib = jB + i0 + 64 * u;
+                    const bool oka = ia < endA, okb = ib < endB;
+                    const int32_t ca = oka ? ia : jA, cb = okb ? ib : jB;
+                    const int32_t r1 = rel[ca], r2 = rel[cb];
+                    const T x1 = colA[ca], x2 = colB[cb];
+                    ra[u] = oka ? r1 : -1;
+                    rb[u] = okb ? r2 : -1;
+                    va[u] = x1;
+                    vb[u] = x2;
                 }
-                T d[4];
+                T da[4], db[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) d[u] = (rr[u] >= 0) ? dst[rr[u]] : T(0);
+                for (int u = 0; u < 4; ++u) {
+                    da[u] = dstA[ra[u] >= 0 ? ra[u] : ccA];
+                    db[u] = dstB[rb[u] >= 0 ? rb[u] : ccB];
+                }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (rr[u] >= 0) dst[rr[u]] = d[u] + v[u];
+                for (int u = 0; u < 4; ++u) {
+                    if (ra[u] >= 0) dstA[ra[u]] = da[u] + va[u];
+                    if (rb[u] >= 0) dstB[rb[u]] = db[u] + vb[u];
+                }
             }
         }
         __syncthreads();
