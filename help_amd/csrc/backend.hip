@@ -215,7 +215,19 @@ void build_schedules(help_gpu* h) {
             int32_t c = S.snrow[s];
             if (S.childptr[s + 1] == S.childptr[s]) continue;
             w += ea_bytes(s);
-            for (int32_t c0 = 0; c0 < c; c0 += 32) h->ea.push_back({s, c0, std::min(c, c0 + 32)});
+            for (int32_t c0 = 0; c0 < c; c0 += 32) {
+                const int32_t c1 = std::min(c, c0 + 32);
+                h->ea.push_back({s, c0, c1, (int32_t)h->earange.size()});
+                // per child: the update columns that land in [c0, c1) (the
+                // relative map of a child is ascending)
+                for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
+                    const int32_t ch = S.child[t];
+                    const int32_t chc = S.snrow[ch] - (S.sfirst[ch + 1] - S.sfirst[ch]);
+                    const int32_t* rel = S.relmap.data() + S.reloff[ch];
+                    h->earange.push_back((int32_t)(std::lower_bound(rel, rel + chc, c0) - rel));
+                    h->earange.push_back((int32_t)(std::lower_bound(rel, rel + chc, c1) - rel));
+                }
+            }
         }
         if ((int32_t)h->ea.size() > off) h->fops.push_back({OP_EA, off, (int32_t)h->ea.size() - off, w});
         int32_t maxn = 0;
@@ -401,6 +413,7 @@ void setup_device_factor(help_gpu* h) {
     h->d_list_bwd.upload(h->list_bwd, st);
     h->d_big.upload(h->sbig, st);
     h->d_ea.upload(h->ea, st);
+    h->d_earange.upload(h->earange, st);
     h->d_potrf.upload(h->potrf, st);
     h->d_trsm.upload(h->trsm, st);
     h->d_gemm.upload(h->gemm, st);
@@ -470,7 +483,7 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
         ProfScope ps(h, o.kind, o.work);
         switch (o.kind) {
             case OP_SMALL: launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st); break;
-            case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->sym, L, U, st); break;
+            case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->d_earange.p, h->sym, L, U, st); break;
             case OP_POTRF: launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, Dinv, h->d_err.p, st); break;
             case OP_TRSM: launch_trsm_rows<T>(h->d_trsm.p + o.off, o.count, h->sym, L, Dinv, st); break;
             case OP_GEMM: launch_gemm<T>(h->d_gemm.p + o.off, o.count, h->sym, L, U, st); break;

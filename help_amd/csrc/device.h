@@ -56,6 +56,7 @@ struct PanelTask {
 struct EaTask {
     int32_t s;
     int32_t c0, c1;
+    int32_t r0;   // ranges[r0 + 2t, r0 + 2t + 1]: the columns of child t of s that land in [c0, c1)
 };
 
 constexpr int kTile = 64;      // GEMM tile and panel width
@@ -114,7 +115,7 @@ template <typename T>
 void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, T* Dinv,
                          int32_t* err, int32_t lds_elems, hipStream_t st);
 template <typename T>
-void launch_extend_add(const EaTask* tasks, int32_t count, DevSym sym, T* L, T* U,
+void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges, DevSym sym, T* L, T* U,
                        hipStream_t st);
 // factor the 64x64 diagonal block at column b and store its inverse
 template <typename T>

@@ -108,6 +108,7 @@ struct help_gpu {
     std::vector<Op> fops, sops_fwd, sops_bwd;
     std::vector<int32_t> list_small, list_fwd, list_bwd;
     std::vector<EaTask> ea;
+    std::vector<int32_t> earange;
     std::vector<PanelTask> potrf, trsm;
     std::vector<GemmTask> gemm;
     std::vector<SolveTask> fgemv, bgemv;
@@ -141,6 +142,7 @@ struct help_gpu {
     DBuf<float> d_L32, d_U32, d_Dinv32;
     DBuf<int8_t> d_big;
     DBuf<EaTask> d_ea;
+    DBuf<int32_t> d_earange;
     DBuf<PanelTask> d_potrf, d_trsm;
     DBuf<GemmTask> d_gemm;
     DBuf<SolveTask> d_fgemv, d_bgemv, d_fxs, d_bxs;

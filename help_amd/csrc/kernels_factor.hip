@@ -154,7 +154,8 @@ __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict_
 // child columns that land in the range are spread over the 4 waves (a
 // column per wave), the lanes take the rows, 4 rows per lane in flight.
 template <typename T>
-__global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks, DevSym sym,
+__global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks,
+                                                    const int32_t* __restrict__ ranges, DevSym sym,
                                                     T* __restrict__ L, T* __restrict__ U) {
     const EaTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s;
@@ -170,16 +171,9 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
         const int32_t chc = sym.snrow[ch] - chn;
         const int32_t* rel = sym.relmap + sym.reloff[ch];
         const T* Uc = U + sym.uoff[ch];
-        // rel is ascending: the child columns that land in [c0, c1)
-        int32_t lo = 0, hi = chc;
-        while (lo < hi) { int32_t mid = (lo + hi) >> 1; if (rel[mid] < tk.c0) lo = mid + 1; else hi = mid; }
-        // count of child columns that land in [c0, c1)
-        int32_t hi2 = lo;
-        {
-            int32_t a = lo, b = chc;
-            while (a < b) { int32_t mid = (a + b) >> 1; if (rel[mid] < tk.c1) a = mid + 1; else b = mid; }
-            hi2 = a;
-        }
+        // the child columns that land in [c0, c1), precomputed on the host
+        const int32_t lo = ranges[tk.r0 + 2 * (t - sym.childptr[s])];
+        const int32_t hi2 = ranges[tk.r0 + 2 * (t - sym.childptr[s]) + 1];
         // each wave takes two child columns at a time, 4 rows of each per lane
         // in flight; all loads are issued before the read-modify-writes
         for (int32_t j0 = lo + wave; j0 < hi2; j0 += 8) {
@@ -753,8 +747,9 @@ void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T
         k_small_fronts<T><<<count, 256, (size_t)lds_elems * sizeof(T), st>>>(list, sym, L, U, Dinv, err);
 }
 template <typename T>
-void launch_extend_add(const EaTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st) {
-    if (count > 0) k_extend_add<T><<<count, 256, 0, st>>>(tasks, sym, L, U);
+void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges, DevSym sym, T* L, T* U,
+                       hipStream_t st) {
+    if (count > 0) k_extend_add<T><<<count, 256, 0, st>>>(tasks, ranges, sym, L, U);
 }
 template <typename T>
 void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st) {
@@ -779,7 +774,7 @@ void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, cons
 #define HELP_INST(T)                                                                                \
     template void launch_small_fronts<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,      \
                                          int32_t, hipStream_t);                                     \
-    template void launch_extend_add<T>(const EaTask*, int32_t, DevSym, T*, T*, hipStream_t);       \
+    template void launch_extend_add<T>(const EaTask*, int32_t, const int32_t*, DevSym, T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, hipStream_t);           \
     template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, T*, int32_t*,        \
                                        hipStream_t);                                                \
