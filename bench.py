@@ -112,6 +112,9 @@ def main():
                     help="gpu: device-resident HSD vectors; host: HELP-order host driver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--inverse-solve", type=int, default=None, choices=[-1, 0, 1],
+                    help="explicit L11 inverse for supernodes wider than 64 columns: "
+                         "0 FP32 mode only (default), 1 always, -1 never")
     ap.add_argument("--refine-max", type=int, default=None,
                     help="FP64 refinement iterations per solve (default: the library's, "
                          "0 in the FP64 mode as in UseCholmod.cpp, 20 in the FP32 mode)")
@@ -137,6 +140,8 @@ def main():
     extra = {}
     if args.refine_max is not None:
         extra = {"refine_fp64" if args.mode == "fp64" else "refine_max": args.refine_max}
+    if args.inverse_solve is not None:
+        extra["inverse_solve"] = args.inverse_solve
     for w in range(args.warmup):
         gb = GpuBackend(mode=mode, **extra)
         res, _ = solve_lp(inst, gb.table, device=args.driver == "gpu")
@@ -207,6 +212,7 @@ def main():
                    "driver_step_s": round(res.seconds_step, 3),
                    "n_renew": res.n_renew, "n_solve": res.n_solve,
                    "refine_iterations": n_ref, "max_rel_residual": max_rel,
+                   "inverse_solve": args.inverse_solve if args.inverse_solve is not None else 0,
                    "profiled_solve_s": None if prof_s is None else round(prof_s, 3)},
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
     }
