@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--inverse-solve", type=int, default=None, choices=[-1, 0, 1],
                     help="explicit L11 inverse for supernodes wider than 64 columns: "
-                         "0 FP32 mode only (default), 1 always, -1 never")
+                         "0 or 1 on (default), -1 off (block substitution)")
     ap.add_argument("--refine-max", type=int, default=None,
                     help="FP64 refinement iterations per solve (default: the library's, "
                          "0 in the FP64 mode as in UseCholmod.cpp, 20 in the FP32 mode)")
@@ -212,7 +212,7 @@ def main():
                    "driver_step_s": round(res.seconds_step, 3),
                    "n_renew": res.n_renew, "n_solve": res.n_solve,
                    "refine_iterations": n_ref, "max_rel_residual": max_rel,
-                   "inverse_solve": args.inverse_solve if args.inverse_solve is not None else 0,
+                   "inverse_solve": "off" if args.inverse_solve == -1 else "on",
                    "profiled_solve_s": None if prof_s is None else round(prof_s, 3)},
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
     }

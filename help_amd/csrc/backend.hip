@@ -84,16 +84,23 @@ struct ProfScope {
         }
         hipEvent_t a = h->evpool[used];
         b = h->evpool[used + 1];
-        HIPCHK(hipEventRecord(a, h->st));
         idx = h->pending.size();
         h->pending.push_back({kind, work, a, b});
+        // the launches of the region take a and b as launch events
+        help::g_region_events = {a, b, false};
     }
     // work known only after the launches (iterative kernels)
     void set_work(double w) {
         if (h->prof && b) h->pending[idx].work = w;
     }
     ~ProfScope() {
-        if (h->prof && b) (void)hipEventRecord(b, h->st);
+        if (!h->prof || !b) return;
+        if (!help::g_region_events.used) {
+            // nothing was launched: an empty interval
+            (void)hipEventRecord(h->pending[idx].a, h->st);
+            (void)hipEventRecord(b, h->st);
+        }
+        help::g_region_events = {};
     }
 };
 
@@ -133,7 +140,7 @@ void build_schedules(help_gpu* h) {
     h->sbig.assign(S.nsuper, 0);
     auto ncol = [&](int32_t s) { return S.sfirst[s + 1] - S.sfirst[s]; };
     // explicit inverses of L11 for supernodes wider than one 64-column block
-    h->use_xinv = h->opts.inverse_solve > 0 || (h->opts.inverse_solve == 0 && h->use_fp32);
+    h->use_xinv = h->opts.inverse_solve >= 0;
     auto nblk_of = [&](int32_t s) { return (ncol(s) + kTile - 1) / kTile; };
     auto wide = [&](int32_t s) { return h->use_xinv && nblk_of(s) > 1; };
     h->xoff.assign(S.nsuper + 1, 0);
@@ -354,7 +361,10 @@ void setup_device_common(help_gpu* h) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         throw HipError("no HIP device visible", HELP_ERR_NOGPU);
     HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&h->st_side, hipStreamNonBlocking));
     for (auto& e : h->ev) HIPCHK(hipEventCreate(&e));
+    for (auto& e : h->ev_side) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipHostMalloc((void**)&h->h_pin, 8 * sizeof(double), hipHostMallocDefault));
     hipStream_t st = h->st;
     h->d_colptr.upload(h->A.colptr, st);
     h->d_rowind.upload(h->A.rowind, st);
@@ -479,7 +489,19 @@ T* xinv_partial(help_gpu* h) {
 template <typename T>
 void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
     hipStream_t st = h->st;
+    // The inverse blocks of a level need only that level's finished L11 and
+    // Dinv blocks, and nothing on the main stream reads them before the
+    // solves: each run of XINV launches goes to the side stream behind a
+    // fork event, and the main stream joins the side stream at the end.
+    bool forked = false, side_used = false;
     for (const Op& o : h->fops) {
+        const bool side = o.kind == OP_XINV;
+        if (side && !forked) {
+            HIPCHK(hipEventRecord(h->ev_side[0], st));
+            HIPCHK(hipStreamWaitEvent(h->st_side, h->ev_side[0], 0));
+            side_used = true;
+        }
+        forked = side;
         ProfScope ps(h, o.kind, o.work);
         switch (o.kind) {
             case OP_SMALL: launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st); break;
@@ -487,9 +509,13 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
             case OP_POTRF: launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, Dinv, h->d_err.p, st); break;
             case OP_TRSM: launch_trsm_rows<T>(h->d_trsm.p + o.off, o.count, h->sym, L, Dinv, st); break;
             case OP_GEMM: launch_gemm<T>(h->d_gemm.p + o.off, o.count, h->sym, L, U, st); break;
-            case OP_XINV: launch_xinv<T>(h->d_xtask.p + o.off, o.count, h->sym, L, Dinv, xinv_store<T>(h), xinv_partial<T>(h), st); break;
+            case OP_XINV: launch_xinv<T>(h->d_xtask.p + o.off, o.count, h->sym, L, Dinv, xinv_store<T>(h), xinv_partial<T>(h), h->st_side); break;
             default: break;
         }
+    }
+    if (side_used) {
+        HIPCHK(hipEventRecord(h->ev_side[1], h->st_side));
+        HIPCHK(hipStreamWaitEvent(st, h->ev_side[1], 0));
     }
     HIPCHK(hipGetLastError());
 }
@@ -560,6 +586,17 @@ int pcg_factor(help_gpu* h, double& rel, int max_it) {
     };
     matvec(x, q);
     launch_sub(m, rhs, q, r, st);
+    if (max_it <= 0) {
+        // no refinement: the residual is only reported, so its two norms are
+        // copied to pinned memory and read after the solve's final
+        // synchronization (no host round trip inside the solve)
+        launch_dot(rhs, rhs, m, h->d_part.p, h->d_scal.p + 2, st);
+        launch_dot(r, r, m, h->d_part.p, h->d_scal.p + 3, st);
+        HIPCHK(hipMemcpyAsync(h->h_pin, h->d_scal.p + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        h->rel_deferred = true;
+        rel = 0;
+        return 0;
+    }
     const double rr0 = dot_dev(h, rhs, rhs, m);
     if (rr0 == 0.0) { rel = 0; return 0; }
     double rr = dot_dev(h, r, r, m);
@@ -917,6 +954,11 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
         HIPCHK(hipMemcpyAsync(x1, h->d_x1.p, sizeof(double) * n, hipMemcpyDefault, st));
         HIPCHK(hipMemcpyAsync(x2, h->d_x2.p, sizeof(double) * m, hipMemcpyDefault, st));
         HIPCHK(hipStreamSynchronize(st));
+        if (h->rel_deferred) {
+            const double rr0 = h->h_pin[0], rr = h->h_pin[1];
+            loc.rel_residual = rr0 == 0.0 ? 0.0 : std::sqrt(rr / rr0);
+            h->rel_deferred = false;
+        }
         if (h->prof) collect_prof(h);
         loc.ms_rhs = elapsed_ms(h->ev[0], h->ev[1]);
         loc.ms_solve = elapsed_ms(h->ev[1], h->ev[2]);

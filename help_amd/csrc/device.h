@@ -4,6 +4,8 @@
 
 #include <cstdint>
 
+#include "launch.h"
+
 namespace help {
 
 // Supernodal structure as uploaded to the device (all arrays device-resident).

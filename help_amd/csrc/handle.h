@@ -151,6 +151,15 @@ struct help_gpu {
     DBuf<float> d_X32, d_S32;              // inverse blocks and their partial sums
     DBuf<double> d_X64, d_S64, d_y;
     DevSym sym{};
+    // side stream of the factorization: the inverse blocks of a finished
+    // level are formed there while the main stream factors the next levels
+    // (fork: main -> side, join: side -> main)
+    hipStream_t st_side = nullptr;
+    hipEvent_t ev_side[2] = {nullptr, nullptr};
+    // pinned host words for results read after the solve's final
+    // synchronization (the residual norms of an unrefined solve)
+    double* h_pin = nullptr;
+    bool rel_deferred = false;
     // profiling
     bool prof = false;
     struct KStat { double ms = 0, work = 0; int64_t launches = 0; } kstat[HELP_K_COUNT];
@@ -172,6 +181,11 @@ struct help_gpu {
 
     ~help_gpu() {
         if (st) { (void)hipStreamSynchronize(st); }
+        if (st_side) { (void)hipStreamSynchronize(st_side); }
+        for (auto& e : ev_side)
+            if (e) (void)hipEventDestroy(e);
+        if (st_side) (void)hipStreamDestroy(st_side);
+        if (h_pin) (void)hipHostFree(h_pin);
         help::pcg_free(pcg);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);

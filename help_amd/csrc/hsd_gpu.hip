@@ -144,7 +144,7 @@ struct HsdGpu {
 
     template <class F>
     void map(int64_t k, F f) {
-        if (k > 0) k_map<<<(unsigned)((k + 255) / 256), 256, 0, st>>>(k, f);
+        if (k > 0) help::launch_kernel(k_map<F>, (unsigned)((k + 255) / 256), 256, 0, st, k, f);
     }
     // returns {sum_0..sum_{K-1}, min of output 0}
     template <int K, class F>
@@ -154,8 +154,8 @@ struct HsdGpu {
         r[K] = HUGE_VAL;
         if (k <= 0) return r;
         int nb = (int)std::min<int64_t>(kRedBlocks, (k + 255) / 256);
-        k_reduce<K><<<nb, 256, 0, st>>>(k, f, part.p);
-        k_reduce_final<K><<<1, 256, 0, st>>>(part.p, nb, out.p);
+        help::launch_kernel(k_reduce<K, F>, nb, 256, 0, st, k, f, part.p);
+        help::launch_kernel(k_reduce_final<K>, 1, 256, 0, st, part.p, nb, out.p);
         HIPCHK(hipMemcpyAsync(r.data(), out.p, sizeof(double) * (K + 1), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         return r;

@@ -735,7 +735,7 @@ __global__ void k_mfma_probe(const T* __restrict__ a, const T* __restrict__ b, T
 
 template <typename T>
 void launch_mfma_probe(const T* a, const T* b, T* out, hipStream_t st) {
-    k_mfma_probe<T><<<1, 64, 0, st>>>(a, b, out);
+    help::launch_kernel(k_mfma_probe<T>, 1, 64, 0, st, a, b, out);
 }
 template void launch_mfma_probe<double>(const double*, const double*, double*, hipStream_t);
 template void launch_mfma_probe<float>(const float*, const float*, float*, hipStream_t);
@@ -744,31 +744,31 @@ template <typename T>
 void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, T* Dinv,
                          int32_t* err, int32_t lds_elems, hipStream_t st) {
     if (count > 0)
-        k_small_fronts<T><<<count, 256, (size_t)lds_elems * sizeof(T), st>>>(list, sym, L, U, Dinv, err);
+        help::launch_kernel(k_small_fronts<T>, count, 256, (size_t)lds_elems * sizeof(T), st, list, sym, L, U, Dinv, err);
 }
 template <typename T>
 void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges, DevSym sym, T* L, T* U,
                        hipStream_t st) {
-    if (count > 0) k_extend_add<T><<<count, 256, 0, st>>>(tasks, ranges, sym, L, U);
+    if (count > 0) help::launch_kernel(k_extend_add<T>, count, 256, 0, st, tasks, ranges, sym, L, U);
 }
 template <typename T>
 void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st) {
-    if (count > 0) k_gemm<T><<<count, 256, 0, st>>>(tasks, sym, L, U);
+    if (count > 0) help::launch_kernel(k_gemm<T>, count, 256, 0, st, tasks, sym, L, U);
 }
 template <typename T>
 void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv, int32_t* err,
                        hipStream_t st) {
-    if (count > 0) k_potrf_diag<T><<<count, 128, 0, st>>>(tasks, sym, L, Dinv, err);
+    if (count > 0) help::launch_kernel(k_potrf_diag<T>, count, 128, 0, st, tasks, sym, L, Dinv, err);
 }
 template <typename T>
 void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, const T* Dinv,
                       hipStream_t st) {
-    if (count > 0) k_trsm_gemm<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv);
+    if (count > 0) help::launch_kernel(k_trsm_gemm<T>, count, 256, 0, st, tasks, sym, L, Dinv);
 }
 template <typename T>
 void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, const T* Dinv, T* X, T* S,
                  hipStream_t st) {
-    if (count > 0) k_xinv<T><<<count, 256, 0, st>>>(tasks, sym, L, Dinv, X, S);
+    if (count > 0) help::launch_kernel(k_xinv<T>, count, 256, 0, st, tasks, sym, L, Dinv, X, S);
 }
 
 #define HELP_INST(T)                                                                                \

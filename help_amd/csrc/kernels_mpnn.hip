@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "launch.h"
 #include "mpnn.h"
 
 namespace help {
@@ -334,61 +335,61 @@ __global__ void k_finite(int32_t m, const double* __restrict__ x, int32_t* __res
 
 void nn_features(int32_t m, int32_t n, const double* g, const double* r, const double* dinv, double rs, double mu,
                  double nu, float* xr, float* xc, hipStream_t st) {
-    k_feat_row<<<grid_for(m), kB, 0, st>>>(m, g, r, rs, mu, xr);
-    k_feat_col<<<grid_for(n), kB, 0, st>>>(n, dinv, nu, xc);
+    help::launch_kernel(k_feat_row, grid_for(m), kB, 0, st, m, g, r, rs, mu, xr);
+    help::launch_kernel(k_feat_col, grid_for(n), kB, 0, st, n, dinv, nu, xc);
 }
 
 void nn_logsum_partials(int64_t n, const double* x, int mode, double* part, int nblk, hipStream_t st) {
-    k_logsum<<<nblk, kB, 0, st>>>(n, x, mode, part);
+    help::launch_kernel(k_logsum, nblk, kB, 0, st, n, x, mode, part);
 }
 
 template <typename AT, typename IT>
 void nn_dense(int64_t N, const IT* in1, int k1, const AT* in2, int k2, const float* W, const float* b, AT* out,
               hipStream_t st) {
-    if (N > 0) k_dense<AT, IT><<<grid_cap((N + 3) / 4, 2048), kB, 0, st>>>(N, in1, k1, in2, k2, W, b, out);
+    if (N > 0) help::launch_kernel(k_dense<AT, IT>, grid_cap((N + 3) / 4, 2048), kB, 0, st, N, in1, k1, in2, k2, W, b, out);
 }
 
 template <typename AT>
 void nn_mean(int32_t N, const int64_t* ptr, const int32_t* idx, const double* val, const AT* src, AT* out,
              hipStream_t st) {
-    if (N > 0) k_mean<AT><<<(N + 3) / 4, kB, 0, st>>>(N, ptr, idx, val, src, out);
+    if (N > 0) help::launch_kernel(k_mean<AT>, (N + 3) / 4, kB, 0, st, N, ptr, idx, val, src, out);
 }
 
 template <typename AT>
 void nn_output(int32_t m, const AT* z, const float* wo, const float* bo, const double* g, double rs, float* y,
                double* x0, hipStream_t st) {
-    if (m > 0) k_output<AT><<<(m + 3) / 4, kB, 0, st>>>(m, z, wo, bo, g, rs, y, x0);
+    if (m > 0) help::launch_kernel(k_output<AT>, (m + 3) / 4, kB, 0, st, m, z, wo, bo, g, rs, y, x0);
 }
 
 void nn_loss_partials(int32_t m, const float* y, const double* dy, const double* g, double rs, double* part,
                       int nblk, hipStream_t st) {
-    k_loss<<<nblk, kB, 0, st>>>(m, y, dy, g, rs, part);
+    help::launch_kernel(k_loss, nblk, kB, 0, st, m, y, dy, g, rs, part);
 }
 
 void nn_output_grad(int32_t m, const float* y, const double* dy, const double* g, double rs, double T, float* dyv,
                     hipStream_t st) {
-    if (m > 0) k_out_grad<<<grid_for(m), kB, 0, st>>>(m, y, dy, g, rs, T, dyv);
+    if (m > 0) help::launch_kernel(k_out_grad, grid_for(m), kB, 0, st, m, y, dy, g, rs, T, dyv);
 }
 
 template <typename AT>
 void nn_out_bwd(int32_t m, const AT* z, const float* dyv, const float* wo, float* dpre, hipStream_t st) {
-    if (m > 0) k_out_bwd<AT><<<grid_for((int64_t)m * kNnH), kB, 0, st>>>(m, z, dyv, wo, dpre);
+    if (m > 0) help::launch_kernel(k_out_bwd<AT>, grid_for((int64_t)m * kNnH), kB, 0, st, m, z, dyv, wo, dpre);
 }
 
 template <typename AT>
 void nn_out_w(int32_t m, const AT* z, const float* dyv, float* partial, hipStream_t st) {
-    if (m > 0) k_out_w<AT><<<(m + kNnChunk - 1) / kNnChunk, kB, 0, st>>>(m, z, dyv, partial);
+    if (m > 0) help::launch_kernel(k_out_w<AT>, (m + kNnChunk - 1) / kNnChunk, kB, 0, st, m, z, dyv, partial);
 }
 
 template <typename AT>
 void nn_relu_bwd(int64_t N, const AT* h, const float* dh, float* dpre, hipStream_t st) {
-    if (N > 0) k_relu_bwd<AT><<<grid_for(N * kNnH), kB, 0, st>>>(N, h, dh, dpre);
+    if (N > 0) help::launch_kernel(k_relu_bwd<AT>, grid_for(N * kNnH), kB, 0, st, N, h, dh, dpre);
 }
 
 void nn_dense_bwd_in(int64_t N, const float* dpre, const float* W, int k1, int k2, float* din1, float* din2,
                      bool accumulate1, hipStream_t st) {
     if (N > 0)
-        k_dense_bwd_in<<<grid_cap((N * (k1 + k2) + kB - 1) / kB, 4096), kB, 0, st>>>(N, dpre, W, k1, k2, din1, din2,
+        help::launch_kernel(k_dense_bwd_in, grid_cap((N * (k1 + k2) + kB - 1) / kB, 4096), kB, 0, st, N, dpre, W, k1, k2, din1, din2,
                                                                                     accumulate1 ? 1 : 0);
 }
 
@@ -396,27 +397,27 @@ template <typename AT, typename IT>
 void nn_dense_bwd_w(int64_t N, const IT* in1, int k1, const AT* in2, int k2, const float* dpre, float* partial,
                     hipStream_t st) {
     if (N > 0)
-        k_dense_bwd_w<AT, IT><<<(int)((N + kNnChunk - 1) / kNnChunk), kB, 0, st>>>(N, in1, k1, in2, k2, dpre,
+        help::launch_kernel(k_dense_bwd_w<AT, IT>, (int)((N + kNnChunk - 1) / kNnChunk), kB, 0, st, N, in1, k1, in2, k2, dpre,
                                                                                    partial);
 }
 
 void nn_reduce_blocks(int npar, int nblk, const float* partial, float* out, hipStream_t st) {
-    k_reduce_blocks<<<grid_for(npar), kB, 0, st>>>(npar, nblk, partial, out);
+    help::launch_kernel(k_reduce_blocks, grid_for(npar), kB, 0, st, npar, nblk, partial, out);
 }
 
 void nn_mean_bwd(int32_t Nsrc, const int64_t* ptrT, const int32_t* idxT, const double* valT, const int64_t* ptrF,
                  const float* dout, float* din, hipStream_t st) {
-    if (Nsrc > 0) k_mean_bwd<<<(Nsrc + 3) / 4, kB, 0, st>>>(Nsrc, ptrT, idxT, valT, ptrF, dout, din);
+    if (Nsrc > 0) help::launch_kernel(k_mean_bwd, (Nsrc + 3) / 4, kB, 0, st, Nsrc, ptrT, idxT, valT, ptrF, dout, din);
 }
 
 void nn_adam(int npar, float* p, const float* g, float* mo, float* ve, float lr, float b1, float b2, float eps,
              int t, int bf16, float* pw, hipStream_t st) {
     const float c1 = 1.0f - std::pow(b1, (float)t), c2 = 1.0f - std::pow(b2, (float)t);
-    k_adam<<<grid_for(npar), kB, 0, st>>>(npar, p, g, mo, ve, lr, b1, b2, eps, c1, c2, bf16, pw);
+    help::launch_kernel(k_adam, grid_for(npar), kB, 0, st, npar, p, g, mo, ve, lr, b1, b2, eps, c1, c2, bf16, pw);
 }
 
 void nn_check_finite(int32_t m, const double* x, int32_t* flag, hipStream_t st) {
-    if (m > 0) k_finite<<<grid_for(m), kB, 0, st>>>(m, x, flag);
+    if (m > 0) help::launch_kernel(k_finite, grid_for(m), kB, 0, st, m, x, flag);
 }
 
 #define NN_INST(AT)                                                                                              \

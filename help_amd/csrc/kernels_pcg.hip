@@ -15,6 +15,7 @@
 #include <climits>
 #include <cmath>
 
+#include "launch.h"
 #include "pcg.h"
 
 namespace help {
@@ -357,56 +358,56 @@ void launch_csr_split(const RowSplitDev& rs, const int64_t* rowptr, const int32_
                       const double* x, double* y, bool square, hipStream_t st) {
     if (rs.m <= 0) return;
     if (square) {
-        k_csr_light<true><<<grid_for(rs.m), kB, 0, st>>>(rs.m, rowptr, colind, val, x, y);
-        if (rs.nchunk) k_csr_chunk<true><<<rs.nchunk, kB, 0, st>>>(rs.chunks, colind, val, x, rs.part);
+        help::launch_kernel(k_csr_light<true>, grid_for(rs.m), kB, 0, st, rs.m, rowptr, colind, val, x, y);
+        if (rs.nchunk) help::launch_kernel(k_csr_chunk<true>, rs.nchunk, kB, 0, st, rs.chunks, colind, val, x, rs.part);
     } else {
-        k_csr_light<false><<<grid_for(rs.m), kB, 0, st>>>(rs.m, rowptr, colind, val, x, y);
-        if (rs.nchunk) k_csr_chunk<false><<<rs.nchunk, kB, 0, st>>>(rs.chunks, colind, val, x, rs.part);
+        help::launch_kernel(k_csr_light<false>, grid_for(rs.m), kB, 0, st, rs.m, rowptr, colind, val, x, y);
+        if (rs.nchunk) help::launch_kernel(k_csr_chunk<false>, rs.nchunk, kB, 0, st, rs.chunks, colind, val, x, rs.part);
     }
-    if (rs.nheavy) k_csr_heavy<<<grid_for(rs.nheavy), kB, 0, st>>>(rs.nheavy, rs.hrow, rs.hptr, rs.part, y);
+    if (rs.nheavy) help::launch_kernel(k_csr_heavy, grid_for(rs.nheavy), kB, 0, st, rs.nheavy, rs.hrow, rs.hptr, rs.part, y);
 }
 
 void launch_csc_t_scaled(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
                          const double* dinv, const double* x, double* t, hipStream_t st) {
-    if (n > 0) k_csc_ts<<<grid_for(n), kB, 0, st>>>(n, colptr, rowind, val, dinv, x, t);
+    if (n > 0) help::launch_kernel(k_csc_ts, grid_for(n), kB, 0, st, n, colptr, rowind, val, dinv, x, t);
 }
 
 void launch_pc_argmax(int32_t m, int32_t k, const double* S, double* pval, int32_t* pidx, int32_t* perm,
                       int32_t* pos, int32_t* col_out, hipStream_t st) {
     const int nb = (int)std::min<int64_t>(kArgmaxBlocks, std::max<int64_t>(1, ((int64_t)m - k + kB - 1) / kB));
-    k_pc_argmax_part<<<nb, kB, 0, st>>>(m, k, perm, S, pval, pidx);
-    k_pc_pick<<<1, kB, 0, st>>>(nb, k, pval, pidx, perm, pos, col_out);
+    help::launch_kernel(k_pc_argmax_part, nb, kB, 0, st, m, k, perm, S, pval, pidx);
+    help::launch_kernel(k_pc_pick, 1, kB, 0, st, nb, k, pval, pidx, perm, pos, col_out);
 }
 
 void launch_pc_scatter(const int32_t* col, int32_t max_row_nnz, const int64_t* rowptr, const int32_t* colind,
                        const double* val, const double* dinv, double* sc, bool clear, hipStream_t st) {
     const int nb = (int)std::min<int64_t>(2048, std::max<int64_t>(1, ((int64_t)max_row_nnz + kB - 1) / kB));
-    k_pc_scatter<<<nb, kB, 0, st>>>(col, rowptr, colind, val, dinv, sc, clear ? 1 : 0);
+    help::launch_kernel(k_pc_scatter, nb, kB, 0, st, col, rowptr, colind, val, dinv, sc, clear ? 1 : 0);
 }
 
 void launch_pc_pivot(int32_t m, int32_t k, const int32_t* col, const double* g, const double* L, double delta,
                      double* D, double* w, hipStream_t st) {
-    k_pc_pivot<<<1, kB, 0, st>>>(m, k, col, g, L, delta, D, w);
+    help::launch_kernel(k_pc_pivot, 1, kB, 0, st, m, k, col, g, L, delta, D, w);
 }
 
 void launch_pc_column(int32_t m, int32_t k, const int32_t* col, const int32_t* pos, const double* g,
                       const double* w, const double* D, double* L, double* S, hipStream_t st) {
-    k_pc_column<<<grid_for(m), kB, 0, st>>>(m, k, col, pos, g, w, D, L, S);
+    help::launch_kernel(k_pc_column, grid_for(m), kB, 0, st, m, k, col, pos, g, w, D, L, S);
 }
 
 void launch_pc_l11(int32_t m, int32_t K, const int32_t* perm, const double* L, double* L11, double* X,
                    hipStream_t st) {
     if (K <= 0) return;
-    k_pc_l11_gather<<<grid_for((int64_t)K * K), kB, 0, st>>>(m, K, perm, L, L11);
-    k_pc_l11_inv<<<grid_for(K), kB, 0, st>>>(K, L11, X);
+    help::launch_kernel(k_pc_l11_gather, grid_for((int64_t)K * K), kB, 0, st, m, K, perm, L, L11);
+    help::launch_kernel(k_pc_l11_inv, grid_for(K), kB, 0, st, K, L11, X);
 }
 
 void launch_vdiv(int64_t n, const double* a, const double* b, double* out, hipStream_t st) {
-    if (n > 0) k_vdiv<<<grid_for(n), kB, 0, st>>>(n, a, b, out);
+    if (n > 0) help::launch_kernel(k_vdiv, grid_for(n), kB, 0, st, n, a, b, out);
 }
 
 void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st) {
-    if (n > 0) k_to_f32<<<grid_for(n), kB, 0, st>>>(n, x, y);
+    if (n > 0) help::launch_kernel(k_to_f32, grid_for(n), kB, 0, st, n, x, y);
 }
 
 void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* pos, const float* L32,
@@ -414,13 +415,13 @@ void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* p
                      hipStream_t st) {
     // K = 0: z = r / S
     const int nb = (int)(((int64_t)m + kLtRows - 1) / kLtRows);
-    if (K > 0) k_pca_y1<<<grid_for(K), kB, 0, st>>>(K, perm, X, r, y1);
-    k_pca_t2<<<grid_for(m), kB, 0, st>>>(m, K, pos, L32, y1, S, r, z);
+    if (K > 0) help::launch_kernel(k_pca_y1, grid_for(K), kB, 0, st, K, perm, X, r, y1);
+    help::launch_kernel(k_pca_t2, grid_for(m), kB, 0, st, m, K, pos, L32, y1, S, r, z);
     if (K > 0) {
-        k_pca_lt<<<nb, kB, 0, st>>>(m, K, L32, z, ltpart);
+        help::launch_kernel(k_pca_lt, nb, kB, 0, st, m, K, L32, z, ltpart);
         double* w = y1 + kPcgMaxK;
-        k_pca_w<<<grid_for(K), kB, 0, st>>>(K, nb, perm, y1, S, ltpart, w);
-        k_pca_final<<<grid_for(K), kB, 0, st>>>(K, perm, X, w, z);
+        help::launch_kernel(k_pca_w, grid_for(K), kB, 0, st, K, nb, perm, y1, S, ltpart, w);
+        help::launch_kernel(k_pca_final, grid_for(K), kB, 0, st, K, perm, X, w, z);
     }
 }
 

@@ -498,34 +498,34 @@ __global__ __launch_bounds__(256) void k_bwd_xinv(const SolveTask* __restrict__ 
 template <typename T>
 void launch_fwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big, const T* L,
                 const T* Dinv, double* x, double* W, double* y, hipStream_t st) {
-    if (ntiny > 0) k_fwd_tiny<T><<<(ntiny + 3) / 4, 256, 0, st>>>(list, ntiny, sym, L, Dinv, x, W);
-    if (count > ntiny) k_fwd<T><<<count - ntiny, 256, 0, st>>>(list + ntiny, sym, big, L, Dinv, x, W, y);
+    if (ntiny > 0) help::launch_kernel(k_fwd_tiny<T>, (ntiny + 3) / 4, 256, 0, st, list, ntiny, sym, L, Dinv, x, W);
+    if (count > ntiny) help::launch_kernel(k_fwd<T>, count - ntiny, 256, 0, st, list + ntiny, sym, big, L, Dinv, x, W, y);
 }
 template <typename T>
 void launch_fwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
                      const double* y, double* x, hipStream_t st) {
-    if (count > 0) k_fwd_xinv<T><<<count, 256, 0, st>>>(tasks, sym, Dinv, X, y, x);
+    if (count > 0) help::launch_kernel(k_fwd_xinv<T>, count, 256, 0, st, tasks, sym, Dinv, X, y, x);
 }
 template <typename T>
 void launch_bwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
                      const double* y, double* x, hipStream_t st) {
-    if (count > 0) k_bwd_xinv<T><<<count, 256, 0, st>>>(tasks, sym, Dinv, X, y, x);
+    if (count > 0) help::launch_kernel(k_bwd_xinv<T>, count, 256, 0, st, tasks, sym, Dinv, X, y, x);
 }
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* x,
                      double* W, hipStream_t st) {
-    if (count > 0) k_fwd_gemv<T><<<count, 256, 0, st>>>(tasks, sym, L, x, W);
+    if (count > 0) help::launch_kernel(k_fwd_gemv<T>, count, 256, 0, st, tasks, sym, L, x, W);
 }
 template <typename T>
 void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* x,
                      double* t, hipStream_t st) {
-    if (count > 0) k_bwd_gemv<T><<<count, 256, 0, st>>>(tasks, sym, L, x, t);
+    if (count > 0) help::launch_kernel(k_bwd_gemv<T>, count, 256, 0, st, tasks, sym, L, x, t);
 }
 template <typename T>
 void launch_bwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big, const T* L,
                 const T* Dinv, double* x, const double* t, double* y, hipStream_t st) {
-    if (count > ntiny) k_bwd<T><<<count - ntiny, 256, 0, st>>>(list + ntiny, sym, big, L, Dinv, x, t, y);
-    if (ntiny > 0) k_bwd_tiny<T><<<(ntiny + 3) / 4, 256, 0, st>>>(list, ntiny, sym, L, Dinv, x);
+    if (count > ntiny) help::launch_kernel(k_bwd<T>, count - ntiny, 256, 0, st, list + ntiny, sym, big, L, Dinv, x, t, y);
+    if (ntiny > 0) help::launch_kernel(k_bwd_tiny<T>, (ntiny + 3) / 4, 256, 0, st, list, ntiny, sym, L, Dinv, x);
 }
 
 #define HELP_INST(T)                                                                                  \

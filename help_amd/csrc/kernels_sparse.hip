@@ -250,21 +250,21 @@ __global__ void k_sub(int64_t n, const double* __restrict__ x, const double* __r
 void launch_dot(const double* a, const double* b, int64_t n, double* part, double* out,
                 hipStream_t st) {
     int nb = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + 255) / 256));
-    k_dot_partial<<<nb, 256, 0, st>>>(a, b, n, part);
-    k_dot_final<<<1, 256, 0, st>>>(part, nb, out);
+    help::launch_kernel(k_dot_partial, nb, 256, 0, st, a, b, n, part);
+    help::launch_kernel(k_dot_final, 1, 256, 0, st, part, nb, out);
 }
 void launch_axpy(int64_t n, double alpha, const double* x, double* y, hipStream_t st) {
-    if (n > 0) k_axpy<<<grid_for(n), kBlock, 0, st>>>(n, alpha, x, y);
+    if (n > 0) help::launch_kernel(k_axpy, grid_for(n), kBlock, 0, st, n, alpha, x, y);
 }
 void launch_xpby(int64_t n, const double* x, double beta, double* y, hipStream_t st) {
-    if (n > 0) k_xpby<<<grid_for(n), kBlock, 0, st>>>(n, x, beta, y);
+    if (n > 0) help::launch_kernel(k_xpby, grid_for(n), kBlock, 0, st, n, x, beta, y);
 }
 void launch_sub(int64_t n, const double* x, const double* y, double* z, hipStream_t st) {
-    if (n > 0) k_sub<<<grid_for(n), kBlock, 0, st>>>(n, x, y, z);
+    if (n > 0) help::launch_kernel(k_sub, grid_for(n), kBlock, 0, st, n, x, y, z);
 }
 
 void launch_dinv(const double* d, double* dinv, int32_t n, hipStream_t st) {
-    if (n > 0) k_dinv<<<grid_for(n), kBlock, 0, st>>>(d, dinv, n);
+    if (n > 0) help::launch_kernel(k_dinv, grid_for(n), kBlock, 0, st, d, dinv, n);
 }
 
 void launch_adat_assemble(const int64_t* pairptr, const int32_t* pair_p, const int32_t* pair_q,
@@ -272,7 +272,7 @@ void launch_adat_assemble(const int64_t* pairptr, const int32_t* pair_p, const i
                           int64_t nslots, const int64_t* dest, double* out_d, float* out_f,
                           hipStream_t st) {
     if (nslots > 0)
-        k_adat<<<grid_for(nslots), kBlock, 0, st>>>(pairptr, pair_p, pair_q, colof, val, dinv,
+        help::launch_kernel(k_adat, grid_for(nslots), kBlock, 0, st, pairptr, pair_p, pair_q, colof, val, dinv,
                                                     nslots, dest, out_d, out_f);
 }
 
@@ -281,67 +281,67 @@ void launch_adat_scaled_f32(const int64_t* pairptr, const int32_t* pair_p, const
                             const int32_t* slot_row, const int32_t* slot_col, const double* scale,
                             const int64_t* dest, float* out_f, hipStream_t st) {
     if (nslots > 0)
-        k_adat_scaled_f32<<<grid_for(nslots), kBlock, 0, st>>>(pairptr, pair_p, pair_q, colof,
+        help::launch_kernel(k_adat_scaled_f32, grid_for(nslots), kBlock, 0, st, pairptr, pair_p, pair_q, colof,
                                                                val, dinv, nslots, slot_row, slot_col, scale,
                                                                dest, out_f);
 }
 
 void launch_add_beta_scaled_f32(const int64_t* diagpos, int32_t m, double beta, const double* scale,
                                 float* Lf, hipStream_t st) {
-    if (m > 0) k_add_beta_scaled_f32<<<grid_for(m), kBlock, 0, st>>>(diagpos, m, beta, scale, Lf);
+    if (m > 0) help::launch_kernel(k_add_beta_scaled_f32, grid_for(m), kBlock, 0, st, diagpos, m, beta, scale, Lf);
 }
 
 void launch_diag_scale(int32_t m, const double* diag, double beta, double* s, hipStream_t st) {
-    if (m > 0) k_diag_scale<<<grid_for(m), kBlock, 0, st>>>(m, diag, beta, s);
+    if (m > 0) help::launch_kernel(k_diag_scale, grid_for(m), kBlock, 0, st, m, diag, beta, s);
 }
 
 void launch_scale_perm(int32_t m, double* v, const double* s, const int32_t* iperm, hipStream_t st) {
-    if (m > 0) k_scale_perm<<<grid_for(m), kBlock, 0, st>>>(m, v, s, iperm);
+    if (m > 0) help::launch_kernel(k_scale_perm, grid_for(m), kBlock, 0, st, m, v, s, iperm);
 }
 
 void launch_add_beta(const int64_t* diagpos, int32_t m, double beta, double* Ld, float* Lf,
                      hipStream_t st) {
-    if (m > 0) k_add_beta<<<grid_for(m), kBlock, 0, st>>>(diagpos, m, beta, Ld, Lf);
+    if (m > 0) help::launch_kernel(k_add_beta, grid_for(m), kBlock, 0, st, diagpos, m, beta, Ld, Lf);
 }
 
 void launch_spmv_csr(int32_t m, const int64_t* rowptr, const int32_t* colind, const double* val,
                      const double* x, double* y, hipStream_t st) {
-    if (m > 0) k_spmv_csr<<<grid_for(m), kBlock, 0, st>>>(m, rowptr, colind, val, x, y);
+    if (m > 0) help::launch_kernel(k_spmv_csr, grid_for(m), kBlock, 0, st, m, rowptr, colind, val, x, y);
 }
 
 void launch_spmv_csc_t(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
                        const double* x, double* y, hipStream_t st) {
-    if (n > 0) k_spmv_csc_t<<<grid_for(n), kBlock, 0, st>>>(n, colptr, rowind, val, x, y);
+    if (n > 0) help::launch_kernel(k_spmv_csc_t, grid_for(n), kBlock, 0, st, n, colptr, rowind, val, x, y);
 }
 
 void launch_rhs(int32_t m, const int64_t* rowptr, const int32_t* colind, const double* csrval,
                 const double* b1, const double* d, const double* b2, const int32_t* iperm,
                 double* rp, hipStream_t st) {
-    if (m > 0) k_rhs<<<grid_for(m), kBlock, 0, st>>>(m, rowptr, colind, csrval, b1, d, b2, iperm, rp);
+    if (m > 0) help::launch_kernel(k_rhs, grid_for(m), kBlock, 0, st, m, rowptr, colind, csrval, b1, d, b2, iperm, rp);
 }
 
 void launch_backsub(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
                     const double* b1, const double* d, const double* x2, double* x1,
                     hipStream_t st) {
-    if (n > 0) k_backsub<<<grid_for(n), kBlock, 0, st>>>(n, colptr, rowind, val, b1, d, x2, x1);
+    if (n > 0) help::launch_kernel(k_backsub, grid_for(n), kBlock, 0, st, n, colptr, rowind, val, b1, d, x2, x1);
 }
 
 void launch_gather_perm(int32_t m, const double* xp, const int32_t* iperm, double* x,
                         hipStream_t st) {
-    if (m > 0) k_gather_perm<<<grid_for(m), kBlock, 0, st>>>(m, xp, iperm, x);
+    if (m > 0) help::launch_kernel(k_gather_perm, grid_for(m), kBlock, 0, st, m, xp, iperm, x);
 }
 
 void launch_scatter_perm(int32_t m, const double* x, const int32_t* iperm, double* xp,
                          hipStream_t st) {
-    if (m > 0) k_scatter_perm<<<grid_for(m), kBlock, 0, st>>>(m, x, iperm, xp);
+    if (m > 0) help::launch_kernel(k_scatter_perm, grid_for(m), kBlock, 0, st, m, x, iperm, xp);
 }
 
 void launch_normal_matvec(int32_t m, int32_t n, const int64_t* rowptr, const int32_t* colind,
                           const double* csrval, const int64_t* colptr, const int32_t* rowind,
                           const double* cscval, const double* dinv, double beta, const double* x,
                           double* tmp_n, double* y, hipStream_t st) {
-    if (n > 0) k_csc_t_scaled<<<grid_for(n), kBlock, 0, st>>>(n, colptr, rowind, cscval, dinv, x, tmp_n);
-    if (m > 0) k_csr_plus_shift<<<grid_for(m), kBlock, 0, st>>>(m, rowptr, colind, csrval, tmp_n, beta, x, y);
+    if (n > 0) help::launch_kernel(k_csc_t_scaled, grid_for(n), kBlock, 0, st, n, colptr, rowind, cscval, dinv, x, tmp_n);
+    if (m > 0) help::launch_kernel(k_csr_plus_shift, grid_for(m), kBlock, 0, st, m, rowptr, colind, csrval, tmp_n, beta, x, y);
 }
 
 }  // namespace help

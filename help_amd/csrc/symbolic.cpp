@@ -32,35 +32,48 @@ CsrMatrix csc_to_csr(const CscMatrix& A) {
 // HELP Helper.cpp:273-329 (ADAt_Allocate, linker form) gives the slot order;
 // Helper.cpp:350-377 (ADAt_Calc) gives the order of the contributions.
 AdatPattern adat_pattern(const CscMatrix& A, const CsrMatrix& R, bool with_pairs) {
+    // Rows are independent: each thread keeps its own marker arrays, and
+    // every row's slots and pairs come out in the same order whatever the
+    // thread count.
     AdatPattern P;
     const int32_t m = A.m;
     P.rowptr.assign(m + 1, 0);
-    std::vector<int32_t> flag(m, -1), slot(m, 0);
-    // pass 1: slots per row (and pair counts)
-    std::vector<int64_t> npair;
-    for (int32_t i = 0; i < m; ++i) {
-        int64_t cnt = 0;
-        for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
-            int32_t j = R.colind[t];
-            for (int64_t q = R.cscpos[t]; q < A.colptr[j + 1]; ++q) {
-                int32_t k = A.rowind[q];
-                if (flag[k] != i) { flag[k] = i; ++cnt; }
+    // pass 1: slots per row
+#pragma omp parallel
+    {
+        std::vector<int32_t> flag(m, -1);
+#pragma omp for schedule(dynamic, 512)
+        for (int32_t i = 0; i < m; ++i) {
+            int64_t cnt = 0;
+            for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
+                int32_t j = R.colind[t];
+                for (int64_t q = R.cscpos[t]; q < A.colptr[j + 1]; ++q) {
+                    int32_t k = A.rowind[q];
+                    if (flag[k] != i) { flag[k] = i; ++cnt; }
+                }
             }
+            P.rowptr[i + 1] = cnt;
         }
-        P.rowptr[i + 1] = P.rowptr[i] + cnt;
     }
-    int64_t nup = P.rowptr[m];
+    for (int32_t i = 0; i < m; ++i) P.rowptr[i + 1] += P.rowptr[i];
+    const int64_t nup = P.rowptr[m];
     P.colind.resize(nup);
+    std::vector<int64_t> npair;
     if (with_pairs) npair.assign(nup, 0);
-    std::fill(flag.begin(), flag.end(), -1);
-    for (int32_t i = 0; i < m; ++i) {
-        int64_t s = P.rowptr[i];
-        for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
-            int32_t j = R.colind[t];
-            for (int64_t q = R.cscpos[t]; q < A.colptr[j + 1]; ++q) {
-                int32_t k = A.rowind[q];
-                if (flag[k] != i) { flag[k] = i; slot[k] = (int32_t)(s - P.rowptr[i]); P.colind[s++] = k; }
-                if (with_pairs) npair[P.rowptr[i] + slot[k]]++;
+    // pass 2: slots in discovery order (and pair counts)
+#pragma omp parallel
+    {
+        std::vector<int32_t> flag(m, -1), slot(m, 0);
+#pragma omp for schedule(dynamic, 512)
+        for (int32_t i = 0; i < m; ++i) {
+            int64_t s = P.rowptr[i];
+            for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
+                int32_t j = R.colind[t];
+                for (int64_t q = R.cscpos[t]; q < A.colptr[j + 1]; ++q) {
+                    int32_t k = A.rowind[q];
+                    if (flag[k] != i) { flag[k] = i; slot[k] = (int32_t)(s - P.rowptr[i]); P.colind[s++] = k; }
+                    if (with_pairs) npair[P.rowptr[i] + slot[k]]++;
+                }
             }
         }
     }
@@ -71,18 +84,24 @@ AdatPattern adat_pattern(const CscMatrix& A, const CsrMatrix& R, bool with_pairs
     if (np > INT32_MAX) throw std::runtime_error("too many ADA^T contribution pairs");
     P.pair_p.resize(np);
     P.pair_q.resize(np);
+    // pass 3: each slot's (p, q) products in the order of the row's entries
+    // (a slot's fill pointer belongs to its row alone)
     std::vector<int64_t> fillp(P.pairptr.begin(), P.pairptr.end() - 1);
-    std::fill(flag.begin(), flag.end(), -1);
-    for (int32_t i = 0; i < m; ++i) {
-        int64_t base = P.rowptr[i];
-        for (int64_t s = base; s < P.rowptr[i + 1]; ++s) slot[P.colind[s]] = (int32_t)(s - base);
-        for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
-            int32_t j = R.colind[t];
-            int64_t p = R.cscpos[t];
-            for (int64_t q = p; q < A.colptr[j + 1]; ++q) {
-                int64_t f = fillp[base + slot[A.rowind[q]]]++;
-                P.pair_p[f] = (int32_t)p;
-                P.pair_q[f] = (int32_t)q;
+#pragma omp parallel
+    {
+        std::vector<int32_t> slot(m, 0);
+#pragma omp for schedule(dynamic, 512)
+        for (int32_t i = 0; i < m; ++i) {
+            const int64_t base = P.rowptr[i];
+            for (int64_t s = base; s < P.rowptr[i + 1]; ++s) slot[P.colind[s]] = (int32_t)(s - base);
+            for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
+                int32_t j = R.colind[t];
+                int64_t p = R.cscpos[t];
+                for (int64_t q = p; q < A.colptr[j + 1]; ++q) {
+                    int64_t f = fillp[base + slot[A.rowind[q]]]++;
+                    P.pair_p[f] = (int32_t)p;
+                    P.pair_q[f] = (int32_t)q;
+                }
             }
         }
     }
@@ -91,24 +110,36 @@ AdatPattern adat_pattern(const CscMatrix& A, const CsrMatrix& R, bool with_pairs
 
 void adat_adjacency(int32_t m, const AdatPattern& P, std::vector<int64_t>& adjptr,
                     std::vector<int32_t>& adj) {
+    // symmetric adjacency without the diagonal; rows are sorted at the end,
+    // so the order in which threads fill them does not matter
     adjptr.assign(m + 1, 0);
+#pragma omp parallel for schedule(dynamic, 512)
     for (int32_t i = 0; i < m; ++i)
         for (int64_t s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
             int32_t k = P.colind[s];
             if (k == i) continue;
+#pragma omp atomic
             adjptr[i + 1]++;
+#pragma omp atomic
             adjptr[k + 1]++;
         }
     for (int32_t i = 0; i < m; ++i) adjptr[i + 1] += adjptr[i];
     adj.resize(adjptr[m]);
     std::vector<int64_t> fill(adjptr.begin(), adjptr.end() - 1);
+#pragma omp parallel for schedule(dynamic, 512)
     for (int32_t i = 0; i < m; ++i)
         for (int64_t s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
             int32_t k = P.colind[s];
             if (k == i) continue;
-            adj[fill[i]++] = k;
-            adj[fill[k]++] = i;
+            int64_t a, b;
+#pragma omp atomic capture
+            a = fill[i]++;
+#pragma omp atomic capture
+            b = fill[k]++;
+            adj[a] = k;
+            adj[b] = i;
         }
+#pragma omp parallel for schedule(dynamic, 512)
     for (int32_t i = 0; i < m; ++i) std::sort(adj.begin() + adjptr[i], adj.begin() + adjptr[i + 1]);
 }
 
@@ -159,7 +190,13 @@ std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
     for (int32_t i = n - 1; i >= 0; --i)
         if (status[i] == VAR) bucket_insert(i, deg[i]);
 
-    std::vector<int32_t> mark(n, 0);
+    // vm[j]: -1 unless j is a live variable (status VAR, nv > 0); else its
+    // latest stamp (membership of the current L_p, or of the pattern being
+    // compared in supervariable detection): one load answers both questions.
+    // em[e]: the same stamps for elements.
+    std::vector<int32_t> vm(n, 0), em(n, 0);
+    for (int32_t i = 0; i < n; ++i)
+        if (status[i] != VAR) vm[i] = -1;
     int32_t stamp = 0;
     int64_t wflg = 1;
     int64_t nel = 0, mindeg = 0;
@@ -167,7 +204,10 @@ std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
         for (int32_t j = v; j >= 0; j = sv_next[j]) order.push_back(j);
     };
     auto next_stamp = [&]() {
-        if (stamp == INT32_MAX) { std::fill(mark.begin(), mark.end(), 0); stamp = 0; }
+        if (stamp == INT32_MAX) {
+            for (int32_t i = 0; i < n; ++i) { if (vm[i] > 0) vm[i] = 0; em[i] = 0; }
+            stamp = 0;
+        }
         return ++stamp;
     };
     std::vector<int32_t> Lp_live;
@@ -179,16 +219,16 @@ std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
         bucket_remove(p);
         // --- new element p: L_p = (A_p U L_e for e in E_p) \ {p} ---
         int32_t st = next_stamp();
-        mark[p] = st;
+        vm[p] = -1;   // p becomes an element
         std::vector<int32_t>& Lp = Le[p];
         Lp.clear();
         int64_t degme = 0;
         for (int32_t j : Av[p])
-            if (status[j] == VAR && nv[j] > 0 && mark[j] != st) { mark[j] = st; Lp.push_back(j); degme += nv[j]; }
+            if (vm[j] >= 0 && vm[j] != st) { vm[j] = st; Lp.push_back(j); degme += nv[j]; }
         for (int32_t e : E[p]) {
             if (status[e] != ELEM) continue;
             for (int32_t j : Le[e])
-                if (status[j] == VAR && nv[j] > 0 && mark[j] != st) { mark[j] = st; Lp.push_back(j); degme += nv[j]; }
+                if (vm[j] >= 0 && vm[j] != st) { vm[j] = st; Lp.push_back(j); degme += nv[j]; }
             status[e] = DEAD;
             std::vector<int32_t>().swap(Le[e]);
         }
@@ -227,7 +267,8 @@ std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
             E[i].resize(k);
             k = 0;
             for (int32_t j : Av[i]) {
-                if (status[j] != VAR || nv[j] == 0 || mark[j] == st) continue;
+                const int32_t vj = vm[j];
+                if (vj < 0 || vj == st) continue;
                 ext += nv[j];
                 h += (uint64_t)j;
                 Av[i][k++] = j;
@@ -235,6 +276,7 @@ std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
             Av[i].resize(k);
             if (E[i].empty() && Av[i].empty()) {   // mass elimination with p
                 status[i] = DEAD;
+                vm[i] = -1;
                 nel += nv[i];
                 degme -= nv[i];
                 emit(i);
@@ -266,17 +308,18 @@ std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
                     if (E[i].size() != E[j].size() || Av[i].size() != Av[j].size()) continue;
                     if (!marked) {
                         st2 = next_stamp();
-                        for (int32_t e : E[i]) mark[e] = st2;
-                        for (int32_t v : Av[i]) mark[v] = st2;
+                        for (int32_t e : E[i]) em[e] = st2;
+                        for (int32_t v : Av[i]) vm[v] = st2;
                         marked = true;
                     }
                     bool same = true;
-                    for (int32_t e : E[j]) if (mark[e] != st2) { same = false; break; }
-                    if (same) for (int32_t v : Av[j]) if (mark[v] != st2) { same = false; break; }
+                    for (int32_t e : E[j]) if (em[e] != st2) { same = false; break; }
+                    if (same) for (int32_t v : Av[j]) if (vm[v] != st2) { same = false; break; }
                     if (!same) continue;
                     nv[i] += nv[j];
                     nv[j] = 0;
                     status[j] = DEAD;
+                    vm[j] = -1;
                     sv_next[sv_tail[i]] = j;
                     sv_tail[i] = sv_tail[j];
                     std::vector<int32_t>().swap(E[j]);
@@ -418,14 +461,18 @@ Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vec
     std::vector<int32_t> perm = perm_in.empty() ? amd_order(m, adjptr, adj) : perm_in;
     std::vector<int32_t> iperm(m);
     for (int32_t k = 0; k < m; ++k) iperm[perm[k]] = k;
-    // etree, then postorder it (does not change fill) and recompute
+    // etree, then postorder it: a postorder is a topological order of the
+    // tree, so the permuted matrix has the same fill and the same tree with
+    // its nodes relabelled
     std::vector<int32_t> par = etree(m, adjptr, adj, perm, iperm);
     std::vector<int32_t> post = postorder(par);
-    std::vector<int32_t> perm2(m);
-    for (int32_t k = 0; k < m; ++k) perm2[k] = perm[post[k]];
+    std::vector<int32_t> perm2(m), ipost(m);
+    for (int32_t k = 0; k < m; ++k) { perm2[k] = perm[post[k]]; ipost[post[k]] = k; }
     perm.swap(perm2);
     for (int32_t k = 0; k < m; ++k) iperm[perm[k]] = k;
-    par = etree(m, adjptr, adj, perm, iperm);
+    std::vector<int32_t> par2(m);
+    for (int32_t k = 0; k < m; ++k) par2[k] = par[post[k]] >= 0 ? ipost[par[post[k]]] : -1;
+    par.swap(par2);
     std::vector<int32_t> cc = colcounts(m, adjptr, adj, perm, iperm, par);
     S.perm = perm; S.iperm = iperm; S.parent = par; S.colcount = cc;
     for (int32_t j = 0; j < m; ++j) {

@@ -71,8 +71,10 @@ typedef struct help_opts {
     int32_t refine_fp64;     /* FP64 mode: refinement steps on the FP64 factor
                                 (0: one factor solve, as UseCholmod.cpp does)  */
     int32_t inverse_solve;   /* supernodes wider than 64 columns solve their
-                                dense triangle with an explicit inverse of L11:
-                                0 in the FP32 mode only, 1 always, -1 never    */
+                                dense triangle with an explicit inverse of L11
+                                (formed during the factorization on a second
+                                stream): 0 or 1 on, -1 off (block substitution
+                                with the 64x64 diagonal-block inverses)        */
 } help_opts;
 
 /* Defaults: FP64 direct, beta=1e-5, refine 20 / 1e-12 (FP32 mode; FP64 mode

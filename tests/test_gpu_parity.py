@@ -197,17 +197,19 @@ def test_solve_mcf_large_fronts():
     be.destruct()
 
 
-@pytest.mark.parametrize("mode", [L.MODE_DIRECT_FP64, L.MODE_DIRECT_FP32])
-def test_solve_with_explicit_inverse(mode):
+@pytest.mark.parametrize("mode,inverse", [(L.MODE_DIRECT_FP64, 1), (L.MODE_DIRECT_FP32, 1),
+                                          (L.MODE_DIRECT_FP64, -1), (L.MODE_DIRECT_FP32, -1)])
+def test_solve_with_explicit_inverse(mode, inverse):
     """Supernodes wider than one 64-column block solved through the explicit
-    inverse of L11 (always on in the FP32 mode, forced here in FP64 too):
-    solutions within 1e-10 of the oracle on well-conditioned d."""
+    inverse of L11 (inverse=1, the default) or by block substitution with the
+    diagonal-block inverses (inverse=-1): solutions within 1e-10 of the
+    oracle on well-conditioned d."""
     inst = instances.mcf(side=60, commodities=4, pairs=10, seed=7)
-    be = LinearEquation(mode=mode, inverse_solve=1)
+    be = LinearEquation(mode=mode, inverse_solve=inverse)
     be.construct(inst)
     sf, _, _, _ = be.supernodes()
     wide = int((np.diff(sf) > 64).sum())
-    _flush(f"explicit inverse mode={mode}: {wide} supernodes wider than 64 columns")
+    _flush(f"explicit inverse mode={mode} inverse={inverse}: {wide} supernodes wider than 64 columns")
     assert wide > 0
     rng = np.random.default_rng(12)
     d = log_uniform(rng, inst.n, -2, 2)
