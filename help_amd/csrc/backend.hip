@@ -285,12 +285,14 @@ void build_schedules(help_gpu* h) {
         double n = ncol(s), c = S.snrow[s];
         return wsz * (n * (n + 1) / 2 + (with_l21 ? (c - n) * n : 0.0));
     };
-    // each level's list: the tiny supernodes (one wave each) first
+    // each level's list of supernodes that are not wide: the tiny ones (one
+    // wave each) first; wide supernodes go to per-block tasks instead
     auto level_list = [&](int32_t l, std::vector<int32_t>& out) {
         int32_t ntiny = 0;
         for (int pass = 0; pass < 2; ++pass)
             for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
                 int32_t s = S.levelsn[t];
+                if (wide(s)) continue;
                 bool tiny = S.snrow[s] <= kTinyRows;
                 if (tiny != (pass == 0)) continue;
                 out.push_back(s);
@@ -306,16 +308,17 @@ void build_schedules(help_gpu* h) {
     };
     for (int32_t l = 0; l < S.nlevels; ++l) {
         int32_t off = (int32_t)h->list_fwd.size(), to = (int32_t)h->fgemv.size(), xo = (int32_t)h->fxs.size();
-        double w = 0, wg = 0, wx = 0;
+        double w = 0, wg = 0;
         int32_t ntiny = level_list(l, h->list_fwd);
-        for (int32_t t = off; t < (int32_t)h->list_fwd.size(); ++t) {
-            int32_t s = h->list_fwd[t];
+        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
+            int32_t s = S.levelsn[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
             bool big = is_big_solve(c, nc) || wide(s);
-            w += wide(s) ? 0.0 : tri_bytes(s, !big);
             if (wide(s)) {
-                wx += xinv_bytes(s);
+                w += xinv_bytes(s);
                 for (int32_t i = 0; i < nblk_of(s); ++i) h->fxs.push_back({s, i});
+            } else {
+                w += tri_bytes(s, !big);
             }
             if (big) {
                 h->sbig[s] = wide(s) ? 2 : 1;
@@ -323,21 +326,24 @@ void build_schedules(help_gpu* h) {
                 for (int32_t b = 0; nc + b * 64 < c; ++b) h->fgemv.push_back({s, b});
             }
         }
-        h->sops_fwd.push_back({OP_FWD, off, (int32_t)h->list_fwd.size() - off, w, 0, ntiny});
-        if ((int32_t)h->fxs.size() > xo) h->sops_fwd.push_back({OP_FWD_XINV, xo, (int32_t)h->fxs.size() - xo, wx});
+        Op o{OP_FWD, off, (int32_t)h->list_fwd.size() - off, w, 0, ntiny};
+        o.xoff = xo;
+        o.xcount = (int32_t)h->fxs.size() - xo;
+        h->sops_fwd.push_back(o);
         if ((int32_t)h->fgemv.size() > to) h->sops_fwd.push_back({OP_FWD_GEMV, to, (int32_t)h->fgemv.size() - to, wg});
     }
     for (int32_t l = S.nlevels - 1; l >= 0; --l) {
         int32_t off = (int32_t)h->list_bwd.size(), to = (int32_t)h->bgemv.size(), xo = (int32_t)h->bxs.size();
-        double w = 0, wg = 0, wx = 0;
+        double w = 0, wg = 0;
         int32_t ntiny = level_list(l, h->list_bwd);
-        for (int32_t t = off; t < (int32_t)h->list_bwd.size(); ++t) {
-            int32_t s = h->list_bwd[t];
+        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
+            int32_t s = S.levelsn[t];
             int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-            w += wide(s) ? 0.0 : tri_bytes(s, !h->sbig[s]);
             if (wide(s)) {
-                wx += xinv_bytes(s);
+                w += xinv_bytes(s);
                 for (int32_t j = 0; j < nblk_of(s); ++j) h->bxs.push_back({s, j});
+            } else {
+                w += tri_bytes(s, !h->sbig[s]);
             }
             if (h->sbig[s]) {
                 wg += wsz * (double)(c - nc) * nc;
@@ -345,8 +351,10 @@ void build_schedules(help_gpu* h) {
             }
         }
         if ((int32_t)h->bgemv.size() > to) h->sops_bwd.push_back({OP_BWD_GEMV, to, (int32_t)h->bgemv.size() - to, wg});
-        h->sops_bwd.push_back({OP_BWD, off, (int32_t)h->list_bwd.size() - off, w, 0, ntiny});
-        if ((int32_t)h->bxs.size() > xo) h->sops_bwd.push_back({OP_BWD_XINV, xo, (int32_t)h->bxs.size() - xo, wx});
+        Op o{OP_BWD, off, (int32_t)h->list_bwd.size() - off, w, 0, ntiny};
+        o.xoff = xo;
+        o.xcount = (int32_t)h->bxs.size() - xo;
+        h->sops_bwd.push_back(o);
     }
     // SpMV-class kernels: 12 bytes per nonzero (value + index) plus vectors
     const double nnz = (double)h->nnz, m = h->m, n = h->n;
@@ -433,7 +441,7 @@ void setup_device_factor(help_gpu* h) {
     h->d_bxs.upload(h->bxs, st);
     h->d_xtask.upload(h->xtask, st);
     h->d_xoff.upload(h->xoff, st);
-    if (h->use_xinv) h->d_y.alloc(h->m);
+    h->d_y.alloc(h->m);
     h->d_W.alloc(std::max<int64_t>(1, S.reloff[S.nsuper]));
     int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
     int64_t nd = h->dinvoff[S.nsuper];
@@ -520,22 +528,28 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
     HIPCHK(hipGetLastError());
 }
 
-// x (permuted numbering, FP64) <- (L L^T)^-1 x
+// x (permuted numbering, FP64) <- (L L^T)^-1 x; the forward result goes
+// to d_y, the backward one back to x
 template <typename T>
 void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x) {
     hipStream_t st = h->st;
     const T* X = xinv_store<T>(h);
+    double* y = h->d_y.p;
     for (const Op& o : h->sops_fwd) {
         ProfScope ps(h, o.kind, o.work);
-        if (o.kind == OP_FWD) launch_fwd<T>(h->d_list_fwd.p + o.off, o.count, o.ntiny, h->sym, h->d_big.p, L, Dinv, x, h->d_W.p, h->d_y.p, st);
-        else if (o.kind == OP_FWD_XINV) launch_fwd_xinv<T>(h->d_fxs.p + o.off, o.count, h->sym, Dinv, X, h->d_y.p, x, st);
-        else launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, x, h->d_W.p, st);
+        if (o.kind == OP_FWD)
+            launch_fwd_level<T>(h->d_list_fwd.p + o.off, o.count, o.ntiny, h->d_fxs.p + o.xoff, o.xcount, h->sym,
+                                h->d_big.p, L, Dinv, X, x, y, h->d_W.p, st);
+        else
+            launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, y, h->d_W.p, st);
     }
     for (const Op& o : h->sops_bwd) {
         ProfScope ps(h, o.kind, o.work);
-        if (o.kind == OP_BWD) launch_bwd<T>(h->d_list_bwd.p + o.off, o.count, o.ntiny, h->sym, h->d_big.p, L, Dinv, x, h->d_t.p, h->d_y.p, st);
-        else if (o.kind == OP_BWD_XINV) launch_bwd_xinv<T>(h->d_bxs.p + o.off, o.count, h->sym, Dinv, X, h->d_y.p, x, st);
-        else launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, st);
+        if (o.kind == OP_BWD)
+            launch_bwd_level<T>(h->d_list_bwd.p + o.off, o.count, o.ntiny, h->d_bxs.p + o.xoff, o.xcount, h->sym,
+                                h->d_big.p, L, Dinv, X, y, h->d_t.p, x, st);
+        else
+            launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, st);
     }
     HIPCHK(hipGetLastError());
 }

@@ -148,36 +148,34 @@ struct SolveTask {
     int32_t blk;          // 64-row block of L21 (forward) / 64-column block (backward)
 };
 constexpr int kTinyRows = 64;      // supernodes with at most this many rows take one wave
-// Forward: for every supernode of the level gather the children's update
-// vectors and solve with L11; small supernodes also apply L21 (big[s] == 0).
-// big[s] == 2: the supernode is wider than one block and solved with the
-// explicit inverse: the gathered right-hand side goes to y instead and
-// launch_fwd_xinv / launch_bwd_xinv finish it.
-// The first `ntiny` entries of `list` have snrow <= kTinyRows and are
-// solved one per wave, four per workgroup; the rest one per workgroup.
+// The forward solve L y = b reads b and writes y, the backward solve
+// L^T x = y reads y and writes x (x may share b's storage).  Per level:
+// launch_fwd_level, then launch_fwd_gemv for the L21 products of big
+// supernodes; launch_bwd_gemv, then launch_bwd_level.
+// launch_fwd_level / launch_bwd_level: `list` holds the level's supernodes
+// that are not wide, its first `ntiny` with snrow <= kTinyRows (one wave
+// each, four per workgroup), the rest one workgroup each; `xt` holds one task
+// per 64-row (forward) or 64-column (backward) block of the level's wide
+// supernodes, solved with the explicit inverse of L11:
+//   forward  y_s(block i) = sum_{j<=i} X_ij g_s(block j), g_s = b_s minus the
+//            children's update vectors;
+//   backward x_s(block j) = sum_{i>=j} X_ij^T (y_s - t_s)(block i).
+// Supernodes with big[s] != 0 leave their L21 products to the gemv kernels.
 template <typename T>
-void launch_fwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big,
-                const T* L, const T* Dinv, double* x, double* W, double* y, hipStream_t st);
-// Forward with the inverse: x_s(64-row block i) = sum_{j<=i} X_ij y_s(block j).
+void launch_fwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
+                      DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* b,
+                      double* y, double* W, hipStream_t st);
 template <typename T>
-void launch_fwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
-                     const double* y, double* x, hipStream_t st);
-// Backward with the inverse: x_s(64-column block j) = sum_{i>=j} X_ij^T y_s(block i).
-template <typename T>
-void launch_bwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
-                     const double* y, double* x, hipStream_t st);
-// Forward, big supernodes: W_s += L21 x_s by 64-row blocks.
+void launch_bwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
+                      DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* y,
+                      const double* t, double* x, hipStream_t st);
+// Forward, big supernodes: W_s += L21 y_s by 64-row blocks.
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
-                     const double* x, double* W, hipStream_t st);
+                     const double* y, double* W, hipStream_t st);
 // Backward, big supernodes: t[f+j] = sum_{i>=n} L21(i,j) x[row_i] by 16-column blocks.
 template <typename T>
 void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
                      const double* x, double* t, hipStream_t st);
-// Backward: x_s = L11^-T (x_s - L21^T x_rows) (big supernodes read t);
-// `list` and `ntiny` as in launch_fwd.
-template <typename T>
-void launch_bwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big,
-                const T* L, const T* Dinv, double* x, const double* t, double* y, hipStream_t st);
 
 }  // namespace help

@@ -65,6 +65,7 @@ struct Op {
     double work;       // algorithmic flops (factor classes) or bytes (the others)
     int32_t lds = 0;   // dynamic LDS elements per workgroup (small fronts)
     int32_t ntiny = 0; // solves: leading supernodes of the list solved one per wave
+    int32_t xoff = 0, xcount = 0;   // solves: the level's wide-supernode block tasks
 };
 
 inline const char* const kKindName[HELP_K_COUNT] = {

@@ -59,15 +59,20 @@ __device__ inline double dot_strided(const T* __restrict__ a, int64_t stride, co
 }
 
 // ------------------------------------------------------------- tiny (c <= 64)
+// The forward solve reads the right-hand side b and writes y; the backward
+// solve reads y and writes x (x may be b's storage).  Keeping input and
+// output apart lets the blocks of one wide supernode run in parallel
+// without reading entries another block is writing.
 template <typename T>
 __device__ void fwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, const T* __restrict__ Dinv,
-                         double* __restrict__ x, double* __restrict__ W, double* xw) {
+                         const double* __restrict__ b, double* __restrict__ y, double* __restrict__ W,
+                         double* xw) {
     const int lane = threadIdx.x & 63;
     const int32_t f = sym.sfirst[s];
     const int32_t n = sym.sfirst[s + 1] - f;
     const int32_t c = sym.snrow[s];
     const T* Lp = L + sym.loff[s];
-    xw[lane] = (lane < n) ? x[f + lane] : 0.0;
+    xw[lane] = (lane < n) ? b[f + lane] : 0.0;
     wave_sync();
     for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
         const int32_t ch = sym.child[t];
@@ -82,12 +87,12 @@ __device__ void fwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, 
     }
     // one diagonal block: x1 = Dinv x1
     const T* Db = (const T*)dinv_block(sym, Dinv, s, 0, sizeof(T));
-    const double y = (lane < n) ? dot_strided<T>(Db + lane, n, xw, n) : 0.0;
+    const double v = (lane < n) ? dot_strided<T>(Db + lane, n, xw, n) : 0.0;
     wave_sync();
-    if (lane < n) xw[lane] = y;
+    if (lane < n) xw[lane] = v;
     wave_sync();
     if (lane < n) {
-        x[f + lane] = y;
+        y[f + lane] = v;
     } else if (lane < c) {
         const double a = dot_strided<T>(Lp + lane, c, xw, n);
         W[sym.reloff[s] + lane - n] = xw[lane] + a;
@@ -96,7 +101,7 @@ __device__ void fwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, 
 
 template <typename T>
 __device__ void bwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, const T* __restrict__ Dinv,
-                         double* __restrict__ x, double* xw) {
+                         const double* __restrict__ y, double* __restrict__ x, double* xw) {
     const int lane = threadIdx.x & 63;
     const int32_t f = sym.sfirst[s];
     const int32_t n = sym.sfirst[s + 1] - f;
@@ -108,7 +113,7 @@ __device__ void bwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, 
     double v = 0.0;
     if (lane < n) {
         const T* col = Lp + (int64_t)lane * c;
-        v = x[f + lane] - dot_strided<T>(col + n, 1, xw + n, c - n);
+        v = y[f + lane] - dot_strided<T>(col + n, 1, xw + n, c - n);
     }
     wave_sync();
     if (lane < n) xw[lane] = v;
@@ -119,28 +124,8 @@ __device__ void bwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, 
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
-                                                  const T* __restrict__ L, const T* __restrict__ Dinv,
-                                                  double* __restrict__ x, double* __restrict__ W) {
-    __shared__ double xw[4][64];
-    const int wave = threadIdx.x >> 6;
-    const int32_t idx = blockIdx.x * 4 + wave;
-    if (idx < count) fwd_tiny<T>(list[idx], sym, L, Dinv, x, W, xw[wave]);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
-                                                  const T* __restrict__ L, const T* __restrict__ Dinv,
-                                                  double* __restrict__ x) {
-    __shared__ double xw[4][64];
-    const int wave = threadIdx.x >> 6;
-    const int32_t idx = blockIdx.x * 4 + wave;
-    if (idx < count) bwd_tiny<T>(list[idx], sym, L, Dinv, x, xw[wave]);
-}
-
 // ------------------------------------------------- one workgroup per supernode
-// x1 = Dinv_b x1 for the 64-column block at jb: thread (r, part) sums the
+// x1 = Dinv x1 for the 64-column block at jb: thread (r, part) sums the
 // columns k = part (mod 4), the four parts combine in a fixed order.
 template <typename T, bool Trans>
 __device__ inline void apply_dinv(const T* __restrict__ Db, int32_t nb, double* xs, double (*red)[64]) {
@@ -163,15 +148,13 @@ __device__ inline void apply_dinv(const T* __restrict__ Db, int32_t nb, double* 
     __syncthreads();
 }
 
+// Forward for a supernode that is not wide: gather the children's update
+// vectors, walk the 64-column blocks with the diagonal-block inverses and
+// (big[s] == 0) apply L21 here; big supernodes leave L21 to k_fwd_gemv.
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd(const int32_t* __restrict__ list, DevSym sym,
-                                             const int8_t* __restrict__ big,
-                                             const T* __restrict__ L, const T* __restrict__ Dinv,
-                                             double* __restrict__ x, double* __restrict__ W,
-                                             double* __restrict__ Y) {
-    __shared__ double xs[kMaxLdsCols];
-    __shared__ double red[4][64];
-    const int32_t s = list[blockIdx.x];
+__device__ void fwd_sn(int32_t s, const DevSym& sym, const int8_t* __restrict__ big, const T* __restrict__ L,
+                       const T* __restrict__ Dinv, const double* __restrict__ b, double* __restrict__ y,
+                       double* __restrict__ W, double* xs, double (*red)[64]) {
     const int32_t f = sym.sfirst[s];
     const int32_t n = sym.sfirst[s + 1] - f;
     const int32_t c = sym.snrow[s];
@@ -180,7 +163,7 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t* __restrict__ list, D
     double* Ws = W + sym.reloff[s];
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int32_t t = tid; t < cu; t += nt) Ws[t] = 0.0;
-    for (int32_t t = tid; t < n; t += nt) xs[t] = x[f + t];
+    for (int32_t t = tid; t < n; t += nt) xs[t] = b[f + t];
     __syncthreads();
     for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
         const int32_t ch = sym.child[t];
@@ -194,10 +177,6 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t* __restrict__ list, D
             else Ws[rr - n] += v;
         }
         __syncthreads();
-    }
-    if (big[s] == 2) {   // solved with the explicit inverse by k_fwd_xinv
-        for (int32_t t = tid; t < n; t += nt) Y[f + t] = xs[t];
-        return;
     }
     const int32_t rend = big[s] ? n : c;
     for (int32_t jb = 0; jb < n; jb += 64) {
@@ -225,13 +204,126 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t* __restrict__ list, D
         }
         __syncthreads();
     }
-    for (int32_t t = tid; t < n; t += nt) x[f + t] = xs[t];
+    for (int32_t t = tid; t < n; t += nt) y[f + t] = xs[t];
+}
+
+// Forward, wide supernode, 64-row block i with the explicit inverse:
+// y[f+64i+r] = sum_{j<=i} sum_q X_ij[r][q] g[64j+q] (X_ii = Dinv_i), where g
+// is the supernode's gathered right-hand side.  Every block gathers the g
+// entries it needs itself (children in order, the same operations as the
+// gather of fwd_sn); block 0 also starts the update vector W_s.  Lanes over
+// r, wave w takes q = 16w..16w+15 of every block, the four partial sums
+// combine in a fixed order.
+template <typename T>
+__device__ void fwd_xinv(const SolveTask tk, const DevSym& sym, const T* __restrict__ Dinv,
+                         const T* __restrict__ X, const double* __restrict__ b, double* __restrict__ y,
+                         double* __restrict__ W, double* ys, double (*red)[64]) {
+    const int32_t s = tk.s, i = tk.blk;
+    const int32_t f = sym.sfirst[s];
+    const int32_t n = sym.sfirst[s + 1] - f;
+    const int32_t cu = sym.snrow[s] - n;
+    double* Ws = W + sym.reloff[s];
+    const int tid = threadIdx.x, r = tid & 63, w = tid >> 6;
+    const int32_t hi = min(n, 64 * (i + 1));
+    if (i == 0)
+        for (int32_t t = tid; t < cu; t += 256) Ws[t] = 0.0;
+    for (int32_t t = tid; t < hi; t += 256) ys[t] = b[f + t];
+    __syncthreads();
+    for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
+        const int32_t ch = sym.child[t];
+        const int32_t chc = sym.snrow[ch] - (sym.sfirst[ch + 1] - sym.sfirst[ch]);
+        const int32_t* rel = sym.relmap + sym.reloff[ch];
+        const double* Wc = W + sym.reloff[ch];
+        for (int32_t u = tid; u < chc; u += 256) {
+            const int32_t rr = rel[u];
+            const double v = Wc[u];
+            if (rr < n) {
+                if (rr < hi) ys[rr] -= v;
+            } else if (i == 0) {
+                Ws[rr - n] += v;
+            }
+        }
+        __syncthreads();
+    }
+    // the 16 loads of a block are staged in registers before the products
+    // (one load in flight at a time otherwise), and masked terms multiply
+    // by zero instead of branching around their loads
+    double a0 = 0.0, a1 = 0.0;
+    for (int32_t j = 0; j < i; ++j) {
+        const T* Xb = X + xblock_offset(sym.xoff, s, i, j) + r;
+        const double* yj = ys + 64 * j + 16 * w;
+        T xv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) xv[u] = Xb[64 * (16 * w + u)];
+#pragma unroll
+        for (int u = 0; u < 16; u += 2) {
+            a0 += (double)xv[u] * yj[u];
+            a1 += (double)xv[u + 1] * yj[u + 1];
+        }
+    }
+    {
+        const int32_t nb = min(64, n - 64 * i);
+        const T* Db = Dinv + sym.dinvoff[s] + (int64_t)i * 4096;
+        const int rr = min(r, nb - 1);
+        const double* yi = ys + 64 * i;
+        T dv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) dv[u] = Db[min(16 * w + u, nb - 1) * nb + rr];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = 16 * w + u;
+            const double yv = (r < nb && q < nb) ? yi[min(q, nb - 1)] : 0.0;
+            a0 += (double)dv[u] * yv;
+        }
+    }
+    red[w][r] = a0 + a1;
+    __syncthreads();
+    if (w == 0 && 64 * i + r < n) y[f + 64 * i + r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+}
+
+// One launch per level: blocks [0, ntb) solve four tiny supernodes each
+// (list[0, ntiny)), the next nrest blocks one supernode each
+// (list[ntiny, ntiny + nrest)), the last nx blocks one 64-row block of a
+// wide supernode each.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_level(const int32_t* __restrict__ list, int32_t ntiny, int32_t nrest,
+                                                   const SolveTask* __restrict__ xt, DevSym sym,
+                                                   const int8_t* __restrict__ big, const T* __restrict__ L,
+                                                   const T* __restrict__ Dinv, const T* __restrict__ X,
+                                                   const double* __restrict__ b, double* __restrict__ y,
+                                                   double* __restrict__ W) {
+    __shared__ double xs[kMaxLdsCols];
+    __shared__ double red[4][64];
+    const int32_t ntb = (ntiny + 3) / 4;
+    const int32_t bid = blockIdx.x;
+    if (bid < ntb) {
+        const int wave = threadIdx.x >> 6;
+        const int32_t idx = bid * 4 + wave;
+        if (idx < ntiny) fwd_tiny<T>(list[idx], sym, L, Dinv, b, y, W, red[wave]);
+    } else if (bid < ntb + nrest) {
+        fwd_sn<T>(list[ntiny + bid - ntb], sym, big, L, Dinv, b, y, W, xs, red);
+    } else {
+        fwd_xinv<T>(xt[bid - ntb - nrest], sym, Dinv, X, b, y, W, xs, red);
+    }
+}
+
+// levels made of tiny supernodes only (the leaves: many of them, and a
+// kernel with a small register and LDS footprint)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
+                                                  const T* __restrict__ L, const T* __restrict__ Dinv,
+                                                  const double* __restrict__ b, double* __restrict__ y,
+                                                  double* __restrict__ W) {
+    __shared__ double xw[4][64];
+    const int wave = threadIdx.x >> 6;
+    const int32_t idx = blockIdx.x * 4 + wave;
+    if (idx < count) fwd_tiny<T>(list[idx], sym, L, Dinv, b, y, W, xw[wave]);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_gemv(const SolveTask* __restrict__ tasks, DevSym sym,
                                                   const T* __restrict__ L,
-                                                  const double* __restrict__ x,
+                                                  const double* __restrict__ y,
                                                   double* __restrict__ W) {
     __shared__ double xs[kMaxLdsCols];
     __shared__ double red[4][64];
@@ -243,7 +335,7 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const SolveTask* __restrict__ 
     const T* Lp = L + sym.loff[s];
     double* Ws = W + sym.reloff[s];
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int32_t t = tid; t < n; t += nt) xs[t] = x[f + t];
+    for (int32_t t = tid; t < n; t += nt) xs[t] = y[f + t];
     __syncthreads();
     // 64 rows, 4 threads per row splitting the columns, fixed combine order
     const int32_t r = tid & 63, part = tid >> 6;
@@ -311,16 +403,12 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const SolveTask* __restrict__ 
     }
 }
 
+// Backward for a supernode that is not wide: x_s = L11^-T (y_s - L21^T x(rows));
+// big supernodes take the L21 product t from k_bwd_gemv.
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, DevSym sym,
-                                             const int8_t* __restrict__ big,
-                                             const T* __restrict__ L, const T* __restrict__ Dinv,
-                                             double* __restrict__ x, const double* __restrict__ tin,
-                                             double* __restrict__ Y) {
-    __shared__ double xs[kMaxLdsCols];
-    __shared__ double xr[kMaxLdsRows];
-    __shared__ double red[4][64];
-    const int32_t s = list[blockIdx.x];
+__device__ void bwd_sn(int32_t s, const DevSym& sym, const int8_t* __restrict__ big, const T* __restrict__ L,
+                       const T* __restrict__ Dinv, const double* __restrict__ y, const double* __restrict__ tin,
+                       double* __restrict__ x, double* xs, double* xr, double (*red)[64]) {
     const int32_t f = sym.sfirst[s];
     const int32_t n = sym.sfirst[s + 1] - f;
     const int32_t c = sym.snrow[s];
@@ -329,12 +417,8 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, D
     const int32_t* rows = sym.rowidx + sym.rowoff[s];
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6;
-    if (big[s] == 2) {   // solved with the explicit inverse by k_bwd_xinv
-        for (int32_t t = tid; t < n; t += nt) Y[f + t] = x[f + t] - tin[f + t];
-        return;
-    }
     if (big[s]) {
-        for (int32_t t = tid; t < n; t += nt) xs[t] = x[f + t] - tin[f + t];
+        for (int32_t t = tid; t < n; t += nt) xs[t] = y[f + t] - tin[f + t];
     } else {
         const bool lds = cu <= kMaxLdsRows;
         if (lds)
@@ -343,7 +427,7 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, D
         // few rows below the diagonal block: a thread per column
         if (cu <= 64) {
             for (int32_t j = tid; j < n; j += nt)
-                xs[j] = x[f + j] - dot_strided<T>(Lp + (int64_t)j * c + n, 1, xr, cu);
+                xs[j] = y[f + j] - dot_strided<T>(Lp + (int64_t)j * c + n, 1, xr, cu);
         } else
         for (int32_t j = wave; j < n; j += 4) {
             const T* col = Lp + (int64_t)j * c + n;
@@ -367,7 +451,7 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, D
                 for (; i < cu; i += 64) a0 += (double)col[i] * x[rows[n + i]];
             }
             const double acc = wave_sum((a0 + a1) + (a2 + a3));
-            if (lane == 0) xs[j] = x[f + j] - acc;
+            if (lane == 0) xs[j] = y[f + j] - acc;
         }
     }
     __syncthreads();
@@ -403,66 +487,20 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t* __restrict__ list, D
     for (int32_t t = tid; t < n; t += nt) x[f + t] = xs[t];
 }
 
-// ------------------------------------------ wide supernodes, explicit inverse
-// Forward, 64-row block i: x[f+64i+r] = sum_{j<=i} sum_q X_ij[r][q] y[f+64j+q]
-// with X_ii = Dinv_i.  Lanes over r, wave w takes q = 16w..16w+15 of every
-// block, the four partial sums combine in a fixed order.
+// Backward, wide supernode, 64-column block j with the explicit inverse:
+// x[f+64j+q] = sum_{i>=j} sum_r X_ij[r][q] (y - t)[f+64i+r].  Wave w takes the
+// 16 columns q = 16w.., lanes over r; the sum over blocks and lanes has a
+// fixed order.
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_xinv(const SolveTask* __restrict__ tasks, DevSym sym,
-                                                  const T* __restrict__ Dinv, const T* __restrict__ X,
-                                                  const double* __restrict__ Y, double* __restrict__ x) {
-    __shared__ double ys[kMaxLdsCols];
-    __shared__ double red[4][64];
-    const SolveTask tk = tasks[blockIdx.x];
-    const int32_t s = tk.s, i = tk.blk;
-    const int32_t f = sym.sfirst[s];
-    const int32_t n = sym.sfirst[s + 1] - f;
-    const int tid = threadIdx.x, r = tid & 63, w = tid >> 6;
-    const int32_t hi = min(n, 64 * (i + 1));
-    for (int32_t t = tid; t < hi; t += 256) ys[t] = Y[f + t];
-    __syncthreads();
-    double a0 = 0.0, a1 = 0.0;
-    for (int32_t j = 0; j < i; ++j) {
-        const T* Xb = X + xblock_offset(sym.xoff, s, i, j) + r;
-        const double* yj = ys + 64 * j + 16 * w;
-#pragma unroll
-        for (int u = 0; u < 16; u += 2) {
-            a0 += (double)Xb[64 * (16 * w + u)] * yj[u];
-            a1 += (double)Xb[64 * (16 * w + u + 1)] * yj[u + 1];
-        }
-    }
-    {
-        const int32_t nb = min(64, n - 64 * i);
-        const T* Db = Dinv + sym.dinvoff[s] + (int64_t)i * 4096;
-        const int rr = min(r, nb - 1);
-        const double* yi = ys + 64 * i;
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int q = 16 * w + u;
-            const double dv = (double)Db[min(q, nb - 1) * nb + rr];
-            a0 += (r < nb && q < nb) ? dv * yi[min(q, nb - 1)] : 0.0;
-        }
-    }
-    red[w][r] = a0 + a1;
-    __syncthreads();
-    if (w == 0 && 64 * i + r < n) x[f + 64 * i + r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
-}
-
-// Backward, 64-column block j: x[f+64j+q] = sum_{i>=j} sum_r X_ij[r][q] y[f+64i+r].
-// Wave w takes the 16 columns q = 16w.., lanes over r; the sum over blocks
-// and lanes has a fixed order.
-template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_xinv(const SolveTask* __restrict__ tasks, DevSym sym,
-                                                  const T* __restrict__ Dinv, const T* __restrict__ X,
-                                                  const double* __restrict__ Y, double* __restrict__ x) {
-    __shared__ double ys[kMaxLdsCols];
-    const SolveTask tk = tasks[blockIdx.x];
+__device__ void bwd_xinv(const SolveTask tk, const DevSym& sym, const T* __restrict__ Dinv,
+                         const T* __restrict__ X, const double* __restrict__ y, const double* __restrict__ tin,
+                         double* __restrict__ x, double* ys) {
     const int32_t s = tk.s, j = tk.blk;
     const int32_t f = sym.sfirst[s];
     const int32_t n = sym.sfirst[s + 1] - f;
     const int32_t nblk = (n + 63) / 64;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int32_t t = 64 * j + tid; t < n; t += 256) ys[t] = Y[f + t];
+    for (int32_t t = 64 * j + tid; t < n; t += 256) ys[t] = y[f + t] - tin[f + t];
     __syncthreads();
     double acc[16];
 #pragma unroll
@@ -482,8 +520,11 @@ __global__ __launch_bounds__(256) void k_bwd_xinv(const SolveTask* __restrict__ 
         const T* Xb = X + xblock_offset(sym.xoff, s, i, j) + lane + 64 * 16 * w;
         const int32_t nbi = min(64, n - 64 * i);
         const double yv = lane < nbi ? ys[64 * i + min(lane, nbi - 1)] : 0.0;
+        T xv[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) acc[u] += (double)Xb[64 * u] * yv;
+        for (int u = 0; u < 16; ++u) xv[u] = Xb[64 * u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc[u] += (double)xv[u] * yv;
     }
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
@@ -493,54 +534,88 @@ __global__ __launch_bounds__(256) void k_bwd_xinv(const SolveTask* __restrict__ 
     }
 }
 
+// One launch per level, block kinds as in k_fwd_level.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_level(const int32_t* __restrict__ list, int32_t ntiny, int32_t nrest,
+                                                   const SolveTask* __restrict__ xt, DevSym sym,
+                                                   const int8_t* __restrict__ big, const T* __restrict__ L,
+                                                   const T* __restrict__ Dinv, const T* __restrict__ X,
+                                                   const double* __restrict__ y, const double* __restrict__ tin,
+                                                   double* __restrict__ x) {
+    __shared__ double xs[kMaxLdsCols];
+    __shared__ double xr[kMaxLdsRows];
+    __shared__ double red[4][64];
+    const int32_t ntb = (ntiny + 3) / 4;
+    const int32_t bid = blockIdx.x;
+    if (bid < ntb) {
+        const int wave = threadIdx.x >> 6;
+        const int32_t idx = bid * 4 + wave;
+        if (idx < ntiny) bwd_tiny<T>(list[idx], sym, L, Dinv, y, x, red[wave]);
+    } else if (bid < ntb + nrest) {
+        bwd_sn<T>(list[ntiny + bid - ntb], sym, big, L, Dinv, y, tin, x, xs, xr, red);
+    } else {
+        bwd_xinv<T>(xt[bid - ntb - nrest], sym, Dinv, X, y, tin, x, xs);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
+                                                  const T* __restrict__ L, const T* __restrict__ Dinv,
+                                                  const double* __restrict__ y, double* __restrict__ x) {
+    __shared__ double xw[4][64];
+    const int wave = threadIdx.x >> 6;
+    const int32_t idx = blockIdx.x * 4 + wave;
+    if (idx < count) bwd_tiny<T>(list[idx], sym, L, Dinv, y, x, xw[wave]);
+}
+
 }  // namespace
 
 template <typename T>
-void launch_fwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big, const T* L,
-                const T* Dinv, double* x, double* W, double* y, hipStream_t st) {
-    if (ntiny > 0) help::launch_kernel(k_fwd_tiny<T>, (ntiny + 3) / 4, 256, 0, st, list, ntiny, sym, L, Dinv, x, W);
-    if (count > ntiny) help::launch_kernel(k_fwd<T>, count - ntiny, 256, 0, st, list + ntiny, sym, big, L, Dinv, x, W, y);
+void launch_fwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
+                      DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* b,
+                      double* y, double* W, hipStream_t st) {
+    const int32_t nrest = count - ntiny;
+    if (nrest == 0 && nx == 0) {
+        if (ntiny > 0) help::launch_kernel(k_fwd_tiny<T>, (ntiny + 3) / 4, 256, 0, st, list, ntiny, sym, L, Dinv, b, y, W);
+        return;
+    }
+    const int32_t grid = (ntiny + 3) / 4 + nrest + nx;
+    help::launch_kernel(k_fwd_level<T>, grid, 256, 0, st, list, ntiny, nrest, xt, sym, big, L, Dinv, X, b, y, W);
 }
 template <typename T>
-void launch_fwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
-                     const double* y, double* x, hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_fwd_xinv<T>, count, 256, 0, st, tasks, sym, Dinv, X, y, x);
+void launch_bwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
+                      DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* y,
+                      const double* t, double* x, hipStream_t st) {
+    const int32_t nrest = count - ntiny;
+    if (nrest == 0 && nx == 0) {
+        if (ntiny > 0) help::launch_kernel(k_bwd_tiny<T>, (ntiny + 3) / 4, 256, 0, st, list, ntiny, sym, L, Dinv, y, x);
+        return;
+    }
+    const int32_t grid = (ntiny + 3) / 4 + nrest + nx;
+    help::launch_kernel(k_bwd_level<T>, grid, 256, 0, st, list, ntiny, nrest, xt, sym, big, L, Dinv, X, y, t, x);
 }
 template <typename T>
-void launch_bwd_xinv(const SolveTask* tasks, int32_t count, DevSym sym, const T* Dinv, const T* X,
-                     const double* y, double* x, hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_bwd_xinv<T>, count, 256, 0, st, tasks, sym, Dinv, X, y, x);
-}
-template <typename T>
-void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* x,
+void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* y,
                      double* W, hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_fwd_gemv<T>, count, 256, 0, st, tasks, sym, L, x, W);
+    if (count > 0) help::launch_kernel(k_fwd_gemv<T>, count, 256, 0, st, tasks, sym, L, y, W);
 }
 template <typename T>
 void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* x,
                      double* t, hipStream_t st) {
     if (count > 0) help::launch_kernel(k_bwd_gemv<T>, count, 256, 0, st, tasks, sym, L, x, t);
 }
-template <typename T>
-void launch_bwd(const int32_t* list, int32_t count, int32_t ntiny, DevSym sym, const int8_t* big, const T* L,
-                const T* Dinv, double* x, const double* t, double* y, hipStream_t st) {
-    if (count > ntiny) help::launch_kernel(k_bwd<T>, count - ntiny, 256, 0, st, list + ntiny, sym, big, L, Dinv, x, t, y);
-    if (ntiny > 0) help::launch_kernel(k_bwd_tiny<T>, (ntiny + 3) / 4, 256, 0, st, list, ntiny, sym, L, Dinv, x);
-}
 
-#define HELP_INST(T)                                                                                  \
-    template void launch_fwd<T>(const int32_t*, int32_t, int32_t, DevSym, const int8_t*, const T*,   \
-                                const T*, double*, double*, double*, hipStream_t);                    \
-    template void launch_fwd_xinv<T>(const SolveTask*, int32_t, DevSym, const T*, const T*,           \
-                                     const double*, double*, hipStream_t);                            \
-    template void launch_bwd_xinv<T>(const SolveTask*, int32_t, DevSym, const T*, const T*,           \
-                                     const double*, double*, hipStream_t);                            \
-    template void launch_fwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,      \
-                                     double*, hipStream_t);                                           \
-    template void launch_bwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,      \
-                                     double*, hipStream_t);                                           \
-    template void launch_bwd<T>(const int32_t*, int32_t, int32_t, DevSym, const int8_t*, const T*,    \
-                                const T*, double*, const double*, double*, hipStream_t);
+#define HELP_INST(T)                                                                                   \
+    template void launch_fwd_level<T>(const int32_t*, int32_t, int32_t, const SolveTask*, int32_t,     \
+                                      DevSym, const int8_t*, const T*, const T*, const T*,              \
+                                      const double*, double*, double*, hipStream_t);                    \
+    template void launch_bwd_level<T>(const int32_t*, int32_t, int32_t, const SolveTask*, int32_t,     \
+                                      DevSym, const int8_t*, const T*, const T*, const T*,              \
+                                      const double*, const double*, double*, hipStream_t);              \
+    template void launch_fwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,       \
+                                     double*, hipStream_t);                                            \
+    template void launch_bwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,       \
+                                     double*, hipStream_t);
 HELP_INST(double)
 HELP_INST(float)
 #undef HELP_INST
