@@ -169,19 +169,16 @@ void build_schedules(help_gpu* h) {
         }
         return b;
     };
-    // inverse blocks of a level's wide supernodes (small and large fronts),
-    // right-looking: the launch for step t finalises X_kj with k = j + t and
-    // adds its terms to the partial sums of the blocks below
-    auto push_xinv = [&](int32_t l) {
+    // inverse blocks of the given wide supernodes, right-looking: the launch
+    // for step t finalises X_kj with k = j + t and adds its terms to the
+    // partial sums of the blocks below
+    auto push_xinv = [&](const std::vector<int32_t>& sns) {
         int32_t maxnb = 0;
-        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t)
-            if (wide(S.levelsn[t])) maxnb = std::max(maxnb, nblk_of(S.levelsn[t]));
+        for (int32_t s : sns) maxnb = std::max(maxnb, nblk_of(s));
         for (int32_t step = 0; step < maxnb; ++step) {
             int32_t xo = (int32_t)h->xtask.size();
             double wx = 0;
-            for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
-                int32_t s = S.levelsn[t];
-                if (!wide(s)) continue;
+            for (int32_t s : sns) {
                 const int32_t nb = nblk_of(s);
                 for (int32_t j = 0; j + step < nb; ++j) {
                     const int32_t k = j + step;
@@ -195,41 +192,73 @@ void build_schedules(help_gpu* h) {
             if ((int32_t)h->xtask.size() > xo) h->fops.push_back({OP_XINV, xo, (int32_t)h->xtask.size() - xo, wx});
         }
     };
-    for (int32_t l = 0; l < S.nlevels; ++l) {
-        std::vector<int32_t> large;
-        int32_t off = (int32_t)h->list_small.size();
-        double w = 0;
-        int32_t lds = 0;
-        for (int32_t t = S.levelptr[l]; t < S.levelptr[l + 1]; ++t) {
-            int32_t s = S.levelsn[t];
-            int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-            if (h->opts.front_path == 0 && is_small_front(c, nc)) {
+    // Phases, as soon as possible.  A small front is one SMALL launch; a
+    // large front with P panels takes phases start .. start+P-1: extend-add
+    // and panel 0 in the first, panel b in phase start+b, and the update
+    // matrix U -= L21 L21^T in the last.  A front starts once its children's
+    // update matrices are complete: the phase after a large child's last
+    // one; a small child's own phase when the parent is large (its SMALL
+    // launch comes first within the phase), the next one when both are small.
+    // Supernodes are postordered, so children come before their parent.
+    // Within a phase the launches run SMALL, EA, POTRF, TRSM, GEMM (in-panel
+    // trailing updates), GEMM (update matrices), then the side-stream XINV
+    // steps of the wide fronts finished in the phase.
+    const int32_t ns = S.nsuper;
+    std::vector<int32_t> start(ns), done(ns), npan(ns);
+    std::vector<char> smallf(ns);
+    int32_t nphase = 0;
+    for (int32_t s = 0; s < ns; ++s) {
+        const int32_t nc = ncol(s), c = S.snrow[s];
+        smallf[s] = h->opts.front_path == 0 && is_small_front(c, nc);
+        npan[s] = smallf[s] ? 1 : (nc + kTile - 1) / kTile;
+        int32_t r = 0;
+        for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
+            const int32_t ch = S.child[t];
+            r = std::max(r, done[ch] + ((smallf[ch] && !smallf[s]) ? 0 : 1));
+        }
+        start[s] = r;
+        done[s] = r + npan[s] - 1;
+        nphase = std::max(nphase, done[s] + 1);
+    }
+    // fronts by phase: small ones, large ones starting, large ones active
+    // (some panel in the phase), large ones finishing
+    std::vector<std::vector<int32_t>> at_small(nphase), at_start(nphase), at_active(nphase), at_done(nphase);
+    for (int32_t s = 0; s < ns; ++s) {
+        if (smallf[s]) { at_small[start[s]].push_back(s); continue; }
+        at_start[start[s]].push_back(s);
+        for (int32_t p = start[s]; p <= done[s]; ++p) at_active[p].push_back(s);
+        at_done[done[s]].push_back(s);
+    }
+    for (int32_t p = 0; p < nphase; ++p) {
+        if (!at_small[p].empty()) {
+            int32_t off = (int32_t)h->list_small.size();
+            double w = 0;
+            int32_t lds = 0;
+            for (int32_t s : at_small[p]) {
+                const int32_t nc = ncol(s), c = S.snrow[s];
                 h->list_small.push_back(s);
                 w += sn_flops(s);
-                int32_t nbm = std::min(kTile, nc);
+                const int32_t nbm = std::min(kTile, nc);
                 lds = std::max(lds, c * nc + nbm * (nbm + 1));
-            } else {
-                large.push_back(s);
             }
-        }
-        if ((int32_t)h->list_small.size() > off)
             h->fops.push_back({OP_SMALL, off, (int32_t)h->list_small.size() - off, w, lds});
-        if (large.empty()) { push_xinv(l); continue; }
-        // extend-add by 32-column chunks of each front
-        off = (int32_t)h->ea.size();
-        w = 0;
-        for (int32_t s : large) {
-            int32_t c = S.snrow[s];
+        }
+        if (at_active[p].empty()) continue;
+        // extend-add by kEaCols-column chunks of each starting front
+        int32_t off = (int32_t)h->ea.size();
+        double w = 0;
+        for (int32_t s : at_start[p]) {
+            const int32_t c = S.snrow[s];
             if (S.childptr[s + 1] == S.childptr[s]) continue;
             w += ea_bytes(s);
-            for (int32_t c0 = 0; c0 < c; c0 += 32) {
-                const int32_t c1 = std::min(c, c0 + 32);
+            for (int32_t c0 = 0; c0 < c; c0 += kEaCols) {
+                const int32_t c1 = std::min(c, c0 + kEaCols);
                 h->ea.push_back({s, c0, c1, (int32_t)h->earange.size()});
                 // per child: the update columns that land in [c0, c1) (the
                 // relative map of a child is ascending)
                 for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
                     const int32_t ch = S.child[t];
-                    const int32_t chc = S.snrow[ch] - (S.sfirst[ch + 1] - S.sfirst[ch]);
+                    const int32_t chc = S.snrow[ch] - ncol(ch);
                     const int32_t* rel = S.relmap.data() + S.reloff[ch];
                     h->earange.push_back((int32_t)(std::lower_bound(rel, rel + chc, c0) - rel));
                     h->earange.push_back((int32_t)(std::lower_bound(rel, rel + chc, c1) - rel));
@@ -237,45 +266,44 @@ void build_schedules(help_gpu* h) {
             }
         }
         if ((int32_t)h->ea.size() > off) h->fops.push_back({OP_EA, off, (int32_t)h->ea.size() - off, w});
-        int32_t maxn = 0;
-        for (int32_t s : large) maxn = std::max(maxn, S.sfirst[s + 1] - S.sfirst[s]);
-        for (int32_t b = 0; b < maxn; b += kTile) {
-            int32_t po = (int32_t)h->potrf.size(), to = (int32_t)h->trsm.size(),
-                    go = (int32_t)h->gemm.size();
-            double wp = 0, wt = 0, wg = 0;
-            for (int32_t s : large) {
-                int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
-                if (b >= nc) continue;
-                int32_t nb = std::min(kTile, nc - b);
-                h->potrf.push_back({s, b, 0});
-                wp += (double)nb * nb * nb / 3.0;
-                for (int32_t r0 = b + nb, rb = 1; r0 < c; r0 += kTile, ++rb) {
-                    h->trsm.push_back({s, b, rb});
-                    wt += (double)nb * nb * std::min(kTile, c - r0);
-                }
-                for (int32_t c0 = b + nb; c0 < nc; c0 += kTile)
-                    for (int32_t r0 = c0; r0 < c; r0 += kTile) {
-                        h->gemm.push_back({s, r0, c0, b, b + nb, nc});
-                        wg += 2.0 * nb * tile_lower_entries(r0, c0, c, nc);
-                    }
+        // the panel of every active front
+        int32_t po = (int32_t)h->potrf.size(), to = (int32_t)h->trsm.size(), go = (int32_t)h->gemm.size();
+        double wp = 0, wt = 0, wg = 0;
+        for (int32_t s : at_active[p]) {
+            const int32_t nc = ncol(s), c = S.snrow[s];
+            const int32_t b = (p - start[s]) * kTile;
+            const int32_t nb = std::min(kTile, nc - b);
+            h->potrf.push_back({s, b, 0});
+            wp += (double)nb * nb * nb / 3.0;
+            for (int32_t r0 = b + nb, rb = 1; r0 < c; r0 += kTile, ++rb) {
+                h->trsm.push_back({s, b, rb});
+                wt += (double)nb * nb * std::min(kTile, c - r0);
             }
-            h->fops.push_back({OP_POTRF, po, (int32_t)h->potrf.size() - po, wp});
-            if ((int32_t)h->trsm.size() > to) h->fops.push_back({OP_TRSM, to, (int32_t)h->trsm.size() - to, wt});
-            if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
+            for (int32_t c0 = b + nb; c0 < nc; c0 += kTile)
+                for (int32_t r0 = c0; r0 < c; r0 += kTile) {
+                    h->gemm.push_back({s, r0, c0, b, b + nb, nc});
+                    wg += 2.0 * nb * tile_lower_entries(r0, c0, c, nc);
+                }
         }
-        // U -= L21 L21^T with the whole panel as inner dimension
-        int32_t go = (int32_t)h->gemm.size();
-        double wg = 0;
-        for (int32_t s : large) {
-            int32_t nc = S.sfirst[s + 1] - S.sfirst[s], c = S.snrow[s];
+        h->fops.push_back({OP_POTRF, po, (int32_t)h->potrf.size() - po, wp});
+        if ((int32_t)h->trsm.size() > to) h->fops.push_back({OP_TRSM, to, (int32_t)h->trsm.size() - to, wt});
+        if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
+        // U -= L21 L21^T with the whole panel as inner dimension, for the
+        // fronts whose last panel is this phase's
+        go = (int32_t)h->gemm.size();
+        wg = 0;
+        std::vector<int32_t> wide_done;
+        for (int32_t s : at_done[p]) {
+            const int32_t nc = ncol(s), c = S.snrow[s];
             for (int32_t c0 = nc; c0 < c; c0 += kTile)
                 for (int32_t r0 = c0; r0 < c; r0 += kTile) {
                     h->gemm.push_back({s, r0, c0, 0, nc, c});
                     wg += 2.0 * nc * tile_lower_entries(r0, c0, c, c);
                 }
+            if (wide(s)) wide_done.push_back(s);
         }
         if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
-        push_xinv(l);
+        push_xinv(wide_done);
     }
     // triangular solves
     // algorithmic bytes of the solves: each stored factor entry is read once

@@ -54,7 +54,10 @@ struct PanelTask {
 };
 
 // Extend-add task: add every child's update matrix into the columns
-// [c0, c1) of the front of supernode s.
+// [c0, c1) of the front of supernode s (kEaCols columns per task: the
+// launches of the large fronts need many workgroups to cover the load
+// latency of the scattered read-modify-writes).
+constexpr int kEaCols = 8;
 struct EaTask {
     int32_t s;
     int32_t c0, c1;

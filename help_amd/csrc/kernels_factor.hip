@@ -150,9 +150,10 @@ __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict_
 }
 
 // ------------------------------------------------------------ extend-add
-// Task = 32 columns [c0, c1) of a front.  For each child in order, the
-// child columns that land in the range are spread over the 4 waves (a
-// column per wave), the lanes take the rows, 4 rows per lane in flight.
+// Task = kEaCols columns [c0, c1) of a front.  For each child in order, the
+// child columns that land in the range are spread over the 4 waves (two
+// columns per wave), the lanes take the rows, 4 rows per lane in flight.
+// Children with no column in the range are skipped without a barrier.
 template <typename T>
 __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks,
                                                     const int32_t* __restrict__ ranges, DevSym sym,
@@ -174,6 +175,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
         // the child columns that land in [c0, c1), precomputed on the host
         const int32_t lo = ranges[tk.r0 + 2 * (t - sym.childptr[s])];
         const int32_t hi2 = ranges[tk.r0 + 2 * (t - sym.childptr[s]) + 1];
+        if (lo >= hi2) continue;   // uniform over the workgroup
         // each wave takes two child columns at a time, 4 rows of each per lane
         // in flight; all loads are issued before the read-modify-writes
         for (int32_t j0 = lo + wave; j0 < hi2; j0 += 8) {
