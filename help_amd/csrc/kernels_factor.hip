@@ -541,16 +541,32 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
         if (kb + kKc < tk.k1) load(kb + kKc);
         mfma_tile<T>(As[buf], Bs[buf], kKc, lane, wr, wc, acc);
     }
+    // read-modify-write of C: every load is issued before the first store
+    // (entries outside the lower part or the front read the front's first
+    // element and are not written back)
+    T cv[2][2][4];
+    T* cp[2][2][4];
+#pragma unroll
     for (int a = 0; a < 2; ++a)
+#pragma unroll
         for (int b = 0; b < 2; ++b)
+#pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int32_t i = tk.row0 + wr + a * 16 + Mfma<T>::row(lane, r);
-                int32_t j = tk.col0 + wc + b * 16 + (lane & 15);
-                if (i >= c || j >= tk.col_end || i < j) continue;
-                T v = acc[a][b][r];
-                if (j < n) Lw[(int64_t)j * c + i] -= v;
-                else Up[(int64_t)(j - n) * cu + (i - n)] -= v;
+                const int32_t i = tk.row0 + wr + a * 16 + Mfma<T>::row(lane, r);
+                const int32_t j = tk.col0 + wc + b * 16 + (lane & 15);
+                const bool ok = i < c && j < tk.col_end && i >= j;
+                T* p = (j < n) ? Lw + (int64_t)j * c + i : Up + (int64_t)(j - n) * cu + (i - n);
+                p = ok ? p : nullptr;
+                cp[a][b][r] = p;
+                cv[a][b][r] = *(ok ? p : Lw);
             }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (cp[a][b][r]) *cp[a][b][r] = cv[a][b][r] - acc[a][b][r];
 }
 
 // X := X L_bb^-T for the 64 rows of block rb-1 below the diagonal block,
@@ -713,13 +729,29 @@ __global__ __launch_bounds__(256) void k_xinv(const XTask* __restrict__ tasks, D
     __syncthreads();
     mfma_tile_ld<T>(As, Bs, kLdb, lane, wr, wc, acc);
     T* Sij = Sb + xblock_offset(sym.xoff, s, i, j);
+    // all loads of the previous partial sums before the stores (at t = 0
+    // the block is written fresh; its stale contents are read and dropped)
+    T prev[2][2][4];
+#pragma unroll
     for (int a = 0; a < 2; ++a)
+#pragma unroll
         for (int q = 0; q < 2; ++q)
+#pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int rw = wr + a * 16 + Mfma<T>::row(lane, r);
                 const int cl = wc + q * 16 + (lane & 15);
-                const T prev = (k == j) ? T(0) : Sij[cl * kTile + rw];
-                Sij[cl * kTile + rw] = prev + acc[a][q][r];
+                prev[a][q][r] = Sij[cl * kTile + rw];
+            }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rw = wr + a * 16 + Mfma<T>::row(lane, r);
+                const int cl = wc + q * 16 + (lane & 15);
+                const T pv = (k == j) ? T(0) : prev[a][q][r];
+                Sij[cl * kTile + rw] = pv + acc[a][q][r];
             }
 }
 

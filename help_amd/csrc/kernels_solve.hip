@@ -58,6 +58,55 @@ __device__ inline double dot_strided(const T* __restrict__ a, int64_t stride, co
     return a0 + a1;
 }
 
+// dst[i] = src[idx[i]] for i < cnt, four per thread in flight (the index
+// loads first, then the value loads)
+__device__ inline void gather_idx(double* dst, const double* __restrict__ src, const int32_t* __restrict__ idx,
+                                  int32_t cnt, int tid, int nt) {
+    for (int32_t i0 = tid; i0 < cnt; i0 += 4 * nt) {
+        int32_t ii[4];
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ii[q] = idx[min(i0 + q * nt, cnt - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = src[ii[q]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (i0 + q * nt < cnt) dst[i0 + q * nt] = v[q];
+    }
+}
+
+// One child's update vector into the supernode's gathered right-hand side
+// (entries rr < n: g[rr] -= v for rr < lim) and its own update vector
+// (rr >= n: Ws[rr - n] += v, when `upd`), four entries per thread in flight.
+// The relative map of a child is injective, so the entries are independent.
+__device__ inline void gather_child(double* g, int32_t lim, double* __restrict__ Ws, bool upd, int32_t n,
+                                    int32_t cu, const int32_t* __restrict__ rel, const double* __restrict__ Wc,
+                                    int32_t chc, int tid, int nt) {
+    for (int32_t u0 = tid; u0 < chc; u0 += 4 * nt) {
+        int32_t rr[4];
+        double v[4], wv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t u = min(u0 + q * nt, chc - 1);
+            rr[q] = rel[u];
+            v[q] = Wc[u];
+        }
+        if (upd && cu > 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) wv[q] = Ws[min(max(rr[q] - n, 0), cu - 1)];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (u0 + q * nt >= chc) continue;
+            if (rr[q] < n) {
+                if (rr[q] < lim) g[rr[q]] -= v[q];
+            } else if (upd) {
+                Ws[rr[q] - n] = wv[q] + v[q];
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------- tiny (c <= 64)
 // The forward solve reads the right-hand side b and writes y; the backward
 // solve reads y and writes x (x may be b's storage).  Keeping input and
@@ -170,12 +219,7 @@ __device__ void fwd_sn(int32_t s, const DevSym& sym, const int8_t* __restrict__ 
         const int32_t chc = sym.snrow[ch] - (sym.sfirst[ch + 1] - sym.sfirst[ch]);
         const int32_t* rel = sym.relmap + sym.reloff[ch];
         const double* Wc = W + sym.reloff[ch];
-        for (int32_t u = tid; u < chc; u += nt) {
-            int32_t rr = rel[u];
-            double v = Wc[u];
-            if (rr < n) xs[rr] -= v;
-            else Ws[rr - n] += v;
-        }
+        gather_child(xs, n, Ws, true, n, cu, rel, Wc, chc, tid, nt);
         __syncthreads();
     }
     const int32_t rend = big[s] ? n : c;
@@ -234,15 +278,7 @@ __device__ void fwd_xinv(const SolveTask tk, const DevSym& sym, const T* __restr
         const int32_t chc = sym.snrow[ch] - (sym.sfirst[ch + 1] - sym.sfirst[ch]);
         const int32_t* rel = sym.relmap + sym.reloff[ch];
         const double* Wc = W + sym.reloff[ch];
-        for (int32_t u = tid; u < chc; u += 256) {
-            const int32_t rr = rel[u];
-            const double v = Wc[u];
-            if (rr < n) {
-                if (rr < hi) ys[rr] -= v;
-            } else if (i == 0) {
-                Ws[rr - n] += v;
-            }
-        }
+        gather_child(ys, hi, Ws, i == 0, n, cu, rel, Wc, chc, tid, 256);
         __syncthreads();
     }
     // the 16 loads of a block are staged in registers before the products
@@ -377,7 +413,7 @@ __global__ __launch_bounds__(256) void k_bwd_gemv(const SolveTask* __restrict__ 
     const int lane = tid & 63, wave = tid >> 6;
     const bool lds = cu <= kMaxGemvRows;
     if (lds)
-        for (int32_t i = tid; i < cu; i += 256) xr[i] = x[rows[n + i]];
+        gather_idx(xr, x, rows + n, cu, tid, 256);
     __syncthreads();
     // each wave: 4 columns at once, lanes over the rows
     const int32_t j0 = tk.blk * 16 + wave * 4;
@@ -422,7 +458,7 @@ __device__ void bwd_sn(int32_t s, const DevSym& sym, const int8_t* __restrict__ 
     } else {
         const bool lds = cu <= kMaxLdsRows;
         if (lds)
-            for (int32_t i = tid; i < cu; i += nt) xr[i] = x[rows[n + i]];
+            gather_idx(xr, x, rows + n, cu, tid, nt);
         __syncthreads();
         // few rows below the diagonal block: a thread per column
         if (cu <= 64) {
