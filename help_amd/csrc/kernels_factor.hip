@@ -18,6 +18,8 @@
 // two-wave diagonal-block Cholesky + inverse, the panel rows times the
 // inverse on MFMA, an MFMA GEMM on the trailing panel columns, and finally
 // one MFMA SYRK for U with the full inner dimension n.
+#include <type_traits>
+
 #include "device.h"
 
 namespace help {
@@ -486,6 +488,170 @@ __device__ inline void mfma_tile(const T* As, const T* Bs, int kc, int lane, int
     }
 }
 
+// ------------------------- diagonal block, blocked form with MFMA updates
+// Same result as k_potrf_diag (L11 = L_unit diag(sqrt(d)) in place, Dinv =
+// diag(1/sqrt(d)) L_unit^-1), LDL^T by 4-column panels.  The 64x64 block and
+// X = L_unit^-1 live in MFMA accumulator layout: wave w holds the 16x16
+// tiles (w, 0..3) of each.  Block step kb, panel owner wave ow = kb/4:
+//  * wave ow reads column block kb from S with the rows on the lanes and
+//    factors it: pivot and in-panel values come from other lanes by
+//    readlane, the multipliers m = a/d go to Mn (negated) and the unscaled
+//    columns to Uf.  It then finishes the panel's 4 rows of X: they leave
+//    its accumulators through XR, take the unit-lower 4x4 substitution with
+//    the columns on the lanes, and go to XF (final, column-major) and back
+//    to XR;
+//  * barrier; every tile row below the panel applies A -= m U^T and
+//    X -= m XR as one V_MFMA_F64_16X16X4_F64 per live tile (K = the 4 panel
+//    columns) and stores its part of column block kb+1 to S; barrier.
+// Two barriers per 4 pivots instead of per-pivot publishing, and the
+// trailing updates run on the matrix cores.
+__device__ inline double read_lane(double v, int l) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_potrf_mfma(const PanelTask* __restrict__ tasks, DevSym sym, double* __restrict__ L,
+                  double* __restrict__ Dinv, int32_t* __restrict__ err) {
+    typedef double T;
+    typedef Mfma<double>::acc_t acc_t;
+    constexpr int kS = kTile + 1;
+    __shared__ T S[kTile * kS];
+    __shared__ T Uf[kTile * kTile];        // column j before scaling (step j), Uf[j*64 + r]
+    __shared__ T XF[kTile * kTile];        // final X, column-major
+    __shared__ T Mn[kTile * 4];            // -m of the current panel, Mn[r*4 + jj]
+    __shared__ T XR[4 * kTile];            // the current panel's rows of X
+    __shared__ T rps[kTile], pivs[kTile], sdv[kTile];
+    const PanelTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s, b = tk.b;
+    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
+    const int32_t c = sym.snrow[s];
+    const int nb = min(kTile, n - b);
+    T* P = L + sym.loff[s] + (int64_t)b * c + b;
+    T* Db = Dinv + dinv_block_offset(sym, s, b);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int lc = lane & 15, lk = lane >> 4;
+    {
+        // lower triangle, identity outside nb, zero above; clamped loads
+        const int r = lane, rr = min(r, nb - 1);
+        T v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k = wave + 4 * q;
+            v[q] = P[(int64_t)min(k, nb - 1) * c + rr];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k = wave + 4 * q;
+            S[k * kS + r] = (r < nb && k < nb) ? ((r >= k) ? v[q] : T(0)) : ((r == k) ? T(1) : T(0));
+        }
+    }
+    __syncthreads();
+    acc_t acc[4], xacc[4];
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * wave + Mfma<double>::row(lane, r), col = 16 * tj + lc;
+            acc[tj][r] = S[col * kS + row];
+            xacc[tj][r] = (row == col) ? T(1) : T(0);
+        }
+    int nbad = 0;
+    for (int kb = 0; kb < 16; ++kb) {
+        if (wave == (kb >> 2)) {
+            T p[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) p[jj] = S[(4 * kb + jj) * kS + lane];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int j = 4 * kb + jj;
+                T d = read_lane(p[jj], j);
+                T akj[4];
+#pragma unroll
+                for (int kk = jj + 1; kk < 4; ++kk) akj[kk] = read_lane(p[jj], 4 * kb + kk);
+                const bool bad = !(d > T(0));
+                nbad += (bad && j < nb) ? 1 : 0;
+                d = bad ? Huge<T>::v : d;
+                const T rp = T(1) / d;
+                const T m = p[jj] * rp;
+                Uf[j * kTile + lane] = p[jj];
+                Mn[lane * 4 + jj] = -m;
+#pragma unroll
+                for (int kk = jj + 1; kk < 4; ++kk) p[kk] -= m * akj[kk];
+                if (lane == 0) { rps[j] = rp; pivs[j] = d; }
+            }
+            // the panel's rows of X (rows 4(kb%4).. of this wave's tiles)
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = Mfma<double>::row(lane, r);
+                    if ((row >> 2) == (kb & 3)) XR[(row & 3) * kTile + 16 * tj + lc] = xacc[tj][r];
+                }
+            wave_sync();
+            T xs[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xs[u] = XR[u * kTile + lane];
+#pragma unroll
+            for (int u = 1; u < 4; ++u)
+#pragma unroll
+                for (int up = 0; up < u; ++up) xs[u] += Mn[(4 * kb + u) * 4 + up] * xs[up];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                XR[u * kTile + lane] = xs[u];
+                XF[lane * kTile + 4 * kb + u] = xs[u];
+            }
+        }
+        __syncthreads();
+        if (16 * wave + 15 >= 4 * kb + 4) {
+            const T aop = Mn[(16 * wave + lc) * 4 + lk];
+            const T xop = (16 * wave + lc > 4 * kb + 3) ? aop : T(0);
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                if (16 * tj + 15 >= 4 * kb + 4) {          // live columns (uniform)
+                    const T bop = Uf[(4 * kb + lk) * kTile + 16 * tj + lc];
+                    acc[tj] = Mfma<double>::fma(aop, bop, acc[tj]);
+                }
+                if (tj <= wave && 16 * tj <= 4 * kb + 3) {  // nonzero part of X (uniform)
+                    const T bx = XR[lk * kTile + 16 * tj + lc];
+                    xacc[tj] = Mfma<double>::fma(xop, bx, xacc[tj]);
+                }
+            }
+            if (kb + 1 < 16) {
+                const int tj1 = (kb + 1) >> 2, jc = 4 * ((kb + 1) & 3);
+                if (lc >= jc && lc < jc + 4) {
+#pragma unroll
+                    for (int tj = 0; tj < 4; ++tj)
+                        if (tj == tj1)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                S[(16 * tj + lc) * kS + 16 * wave + Mfma<double>::row(lane, r)] = acc[tj][r];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0 && nbad) atomicAdd(err, nbad);
+    if (t < kTile) sdv[t] = sqrt(pivs[t]);
+    __syncthreads();
+    // L11[r][k] = raw_r(step k) / d_k * sqrt(d_k); Dinv[r][k] = X[r][k] / sqrt(d_r)
+    const int r = lane;
+    const T isd = T(1) / sdv[r];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k = wave + 4 * q;
+        const T sk = sdv[k];
+        const T f = (r > k) ? rps[k] * sk : T(0);
+        T v = Uf[k * kTile + r] * f;
+        v = (r == k) ? sk : v;
+        if (r < nb && k < nb) P[(int64_t)k * c + r] = v;
+        const T xv = XF[k * kTile + r] * ((r >= k) ? isd : T(0));
+        if (r < nb && k < nb) Db[k * nb + r] = xv;
+    }
+}
+
 constexpr int kKc = 16;   // inner-dimension chunk staged in LDS
 
 // C[row0:+64, col0:+64] -= P[rows, k0:k1] * P[cols, k0:k1]^T, lower part only,
@@ -792,7 +958,12 @@ void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, h
 template <typename T>
 void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv, int32_t* err,
                        hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_potrf_diag<T>, count, 128, 0, st, tasks, sym, L, Dinv, err);
+    if (count <= 0) return;
+    // FP64: the blocked MFMA form; FP32 keeps the two-wave register-blocked form
+    if constexpr (std::is_same<T, double>::value)
+        help::launch_kernel(k_potrf_mfma, count, 256, 0, st, tasks, sym, L, Dinv, err);
+    else
+        help::launch_kernel(k_potrf_diag<T>, count, 128, 0, st, tasks, sym, L, Dinv, err);
 }
 template <typename T>
 void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, const T* Dinv,
