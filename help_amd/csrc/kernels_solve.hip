@@ -285,7 +285,29 @@ __device__ void fwd_xinv(const SolveTask tk, const DevSym& sym, const T* __restr
     // (one load in flight at a time otherwise), and masked terms multiply
     // by zero instead of branching around their loads
     double a0 = 0.0, a1 = 0.0;
-    for (int32_t j = 0; j < i; ++j) {
+    int32_t j = 0;
+    for (; j + 1 < i; j += 2) {   // two blocks' loads in flight, summed in block order
+        const T* Xb0 = X + xblock_offset(sym.xoff, s, i, j) + r;
+        const T* Xb1 = X + xblock_offset(sym.xoff, s, i, j + 1) + r;
+        T xv0[16], xv1[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) xv0[u] = Xb0[64 * (16 * w + u)];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) xv1[u] = Xb1[64 * (16 * w + u)];
+        const double* yj0 = ys + 64 * j + 16 * w;
+        const double* yj1 = yj0 + 64;
+#pragma unroll
+        for (int u = 0; u < 16; u += 2) {
+            a0 += (double)xv0[u] * yj0[u];
+            a1 += (double)xv0[u + 1] * yj0[u + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u += 2) {
+            a0 += (double)xv1[u] * yj1[u];
+            a1 += (double)xv1[u + 1] * yj1[u + 1];
+        }
+    }
+    if (j < i) {
         const T* Xb = X + xblock_offset(sym.xoff, s, i, j) + r;
         const double* yj = ys + 64 * j + 16 * w;
         T xv[16];
@@ -464,20 +486,42 @@ __device__ void bwd_sn(int32_t s, const DevSym& sym, const int8_t* __restrict__ 
         if (cu <= 64) {
             for (int32_t j = tid; j < n; j += nt)
                 xs[j] = y[f + j] - dot_strided<T>(Lp + (int64_t)j * c + n, 1, xr, cu);
-        } else
-        for (int32_t j = wave; j < n; j += 4) {
-            const T* col = Lp + (int64_t)j * c + n;
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int32_t i = lane;
-            if (lds) {
+        } else if (lds) {
+            // four columns per wave at a time, their loads in flight together;
+            // each column keeps its own four accumulators and combine order
+            for (int32_t j0 = 4 * wave; j0 < n; j0 += 16) {
+                const int32_t nq = min(4, n - j0);
+                const T* col[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) col[q] = Lp + (int64_t)(j0 + min(q, nq - 1)) * c + n;
+                double a0[4], a1[4], a2[4], a3[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a0[q] = a1[q] = a2[q] = a3[q] = 0.0;
+                int32_t i = lane;
                 for (; i + 192 < cu; i += 256) {
-                    a0 += (double)col[i] * xr[i];
-                    a1 += (double)col[i + 64] * xr[i + 64];
-                    a2 += (double)col[i + 128] * xr[i + 128];
-                    a3 += (double)col[i + 192] * xr[i + 192];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        a0[q] += (double)col[q][i] * xr[i];
+                        a1[q] += (double)col[q][i + 64] * xr[i + 64];
+                        a2[q] += (double)col[q][i + 128] * xr[i + 128];
+                        a3[q] += (double)col[q][i + 192] * xr[i + 192];
+                    }
                 }
-                for (; i < cu; i += 64) a0 += (double)col[i] * xr[i];
-            } else {
+                for (; i < cu; i += 64) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) a0[q] += (double)col[q][i] * xr[i];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double acc = wave_sum((a0[q] + a1[q]) + (a2[q] + a3[q]));
+                    if (lane == 0 && q < nq) xs[j0 + q] = y[f + j0 + q] - acc;
+                }
+            }
+        } else {
+            for (int32_t j = wave; j < n; j += 4) {
+                const T* col = Lp + (int64_t)j * c + n;
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+                int32_t i = lane;
                 for (; i + 192 < cu; i += 256) {
                     a0 += (double)col[i] * x[rows[n + i]];
                     a1 += (double)col[i + 64] * x[rows[n + i + 64]];
@@ -485,9 +529,9 @@ __device__ void bwd_sn(int32_t s, const DevSym& sym, const int8_t* __restrict__ 
                     a3 += (double)col[i + 192] * x[rows[n + i + 192]];
                 }
                 for (; i < cu; i += 64) a0 += (double)col[i] * x[rows[n + i]];
+                const double acc = wave_sum((a0 + a1) + (a2 + a3));
+                if (lane == 0) xs[j] = y[f + j] - acc;
             }
-            const double acc = wave_sum((a0 + a1) + (a2 + a3));
-            if (lane == 0) xs[j] = y[f + j] - acc;
         }
     }
     __syncthreads();
@@ -552,7 +596,24 @@ __device__ void bwd_xinv(const SolveTask tk, const DevSym& sym, const T* __restr
             acc[u] += (double)Db[q * nb + rr] * yv;
         }
     }
-    for (int32_t i = j + 1; i < nblk; ++i) {
+    int32_t i = j + 1;
+    for (; i + 1 < nblk; i += 2) {   // two blocks' loads in flight, summed in block order
+        const T* Xb0 = X + xblock_offset(sym.xoff, s, i, j) + lane + 64 * 16 * w;
+        const T* Xb1 = X + xblock_offset(sym.xoff, s, i + 1, j) + lane + 64 * 16 * w;
+        const int32_t nb1 = min(64, n - 64 * (i + 1));
+        const double yv0 = ys[64 * i + lane];       // block i is full (i + 1 < nblk)
+        const double yv1 = lane < nb1 ? ys[64 * (i + 1) + min(lane, nb1 - 1)] : 0.0;
+        T xv0[16], xv1[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) xv0[u] = Xb0[64 * u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) xv1[u] = Xb1[64 * u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc[u] += (double)xv0[u] * yv0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc[u] += (double)xv1[u] * yv1;
+    }
+    if (i < nblk) {
         const T* Xb = X + xblock_offset(sym.xoff, s, i, j) + lane + 64 * 16 * w;
         const int32_t nbi = min(64, n - 64 * i);
         const double yv = lane < nbi ? ys[64 * i + min(lane, nbi - 1)] : 0.0;
