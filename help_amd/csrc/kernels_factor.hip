@@ -103,14 +103,37 @@ __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict_
         const int32_t chc = sym.snrow[ch] - chn;
         const int32_t* rel = sym.relmap + sym.reloff[ch];
         const T* Uc = U + sym.uoff[ch];
+        // four entries per thread in flight: every load (the child's entry,
+        // the relative map, the parent's update-matrix entry) before the
+        // adds; the relative map is injective, so the targets are distinct
         const int64_t tot = (int64_t)chc * chc;
-        for (int64_t idx = tid; idx < tot; idx += nt) {
-            int32_t jj = (int32_t)(idx / chc), ii = (int32_t)(idx - (int64_t)jj * chc);
-            if (ii < jj) continue;
-            int32_t r = rel[ii], cc = rel[jj];
-            T v = Uc[idx];
-            if (cc < n) F[cc * c + r] += v;
-            else Up[(int64_t)(cc - n) * cu + (r - n)] += v;
+        for (int64_t idx0 = tid; idx0 < tot; idx0 += 4 * nt) {
+            bool ok[4];
+            int32_t rr[4], cc[4];
+            T v[4], old[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t idx = idx0 + (int64_t)q * nt;
+                const int64_t id = idx < tot ? idx : 0;
+                const int32_t jj = (int32_t)(id / chc), ii = (int32_t)(id - (int64_t)jj * chc);
+                ok[q] = idx < tot && ii >= jj;
+                rr[q] = rel[ii];
+                cc[q] = rel[jj];
+                v[q] = Uc[id];
+            }
+            if (cu > 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool up = ok[q] && cc[q] >= n;
+                    old[q] = Up[up ? (int64_t)(cc[q] - n) * cu + (rr[q] - n) : 0];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (!ok[q]) continue;
+                if (cc[q] < n) F[cc[q] * c + rr[q]] += v[q];
+                else Up[(int64_t)(cc[q] - n) * cu + (rr[q] - n)] = old[q] + v[q];
+            }
         }
         __syncthreads();
     }
@@ -140,14 +163,33 @@ __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict_
         const int32_t nb = min(kTile, n - b);
         invert_lower_block<T>(F + (int64_t)b * c + b, c, nb, Dinv + dinv_block_offset(sym, s, b), X);
     }
-    // U -= L21 L21^T
+    // U -= L21 L21^T, four entries per thread with their loads first
     const int32_t tot = cu * cu;
-    for (int32_t idx = tid; idx < tot; idx += nt) {
-        int32_t jj = idx / cu, ii = idx - jj * cu;
-        if (ii < jj) continue;
-        T acc = 0;
-        for (int32_t j = 0; j < n; ++j) acc += F[j * c + n + ii] * F[j * c + n + jj];
-        Up[idx] -= acc;
+    for (int32_t idx0 = tid; idx0 < tot; idx0 += 4 * nt) {
+        bool ok[4];
+        int32_t pos[4];
+        T old[4], acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t idx = idx0 + q * nt;
+            const int32_t id = idx < tot ? idx : 0;
+            const int32_t jj = id / cu, ii = id - jj * cu;
+            ok[q] = idx < tot && ii >= jj;
+            pos[q] = id;
+            old[q] = Up[id];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            T a = 0;
+            if (ok[q]) {
+                const int32_t jj = pos[q] / cu, ii = pos[q] - jj * cu;
+                for (int32_t j = 0; j < n; ++j) a += F[j * c + n + ii] * F[j * c + n + jj];
+            }
+            acc[q] = a;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ok[q]) Up[pos[q]] = old[q] - acc[q];
     }
 }
 
