@@ -415,13 +415,14 @@ void setup_device_common(help_gpu* h) {
     h->d_part.alloc(kDotPartials); h->d_scal.alloc(8);
     h->d_err.alloc(1);
     std::vector<help::RowChunk> chunks;
-    std::vector<int32_t> hrow, hptr;
-    help::build_row_split(m, h->R.rowptr, chunks, hrow, hptr);
+    std::vector<int32_t> hrow, hptr, mrow;
+    help::build_row_split(m, h->R.rowptr, chunks, hrow, hptr, mrow);
     for (int32_t i = 0; i < m; ++i)
         h->max_row_nnz = std::max<int32_t>(h->max_row_nnz, (int32_t)(h->R.rowptr[i + 1] - h->R.rowptr[i]));
     h->d_chunks.upload(chunks, st);
     h->d_hrow.upload(hrow, st);
     h->d_hptr.upload(hptr, st);
+    h->d_mrow.upload(mrow, st);
     h->d_rpart.alloc(std::max<size_t>(1, chunks.size()));
     h->rows.m = m;
     h->rows.nchunk = (int32_t)chunks.size();
@@ -430,6 +431,8 @@ void setup_device_common(help_gpu* h) {
     h->rows.hrow = h->d_hrow.p;
     h->rows.hptr = h->d_hptr.p;
     h->rows.part = h->d_rpart.p;
+    h->rows.nmed = (int32_t)mrow.size();
+    h->rows.mrow = h->d_mrow.p;
 }
 
 // symbolic structures, schedules and factor storage of the direct modes

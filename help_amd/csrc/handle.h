@@ -170,7 +170,7 @@ struct help_gpu {
     double work_adat = 0, work_rhs = 0, work_backsub = 0;
     // row partition of A for products with heavy rows (all modes)
     DBuf<help::RowChunk> d_chunks;
-    DBuf<int32_t> d_hrow, d_hptr;
+    DBuf<int32_t> d_hrow, d_hptr, d_mrow;
     DBuf<double> d_rpart;
     help::RowSplitDev rows;
     int32_t max_row_nnz = 0;

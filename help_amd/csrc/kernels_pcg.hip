@@ -2,8 +2,9 @@
 //
 // * Row-split CSR products: A has rows of wildly different lengths (Zipf row
 //   popularity gives rows with millions of nonzeros), so rows up to
-//   kHeavyRow nonzeros take a thread and longer rows are cut into chunks of
-//   kRowChunk, one workgroup each, combined in chunk order: deterministic.
+//   kLightRow nonzeros take a thread, rows up to kHeavyRow a wave, and
+//   longer rows are cut into chunks of kRowChunk, one workgroup each,
+//   combined in chunk order: deterministic.
 // * Greedy partial Cholesky (PCG.cpp:38-103): per pivot an argmax over the
 //   Schur diagonal, the pivot row of A scattered into an n-vector, one
 //   product with A for the pivot's column of ADA^T, and a left-looking
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(256) void k_csr_light(int32_t m, const int64_t* __r
     const int32_t i = blockIdx.x * kB + threadIdx.x;
     if (i >= m) return;
     const int64_t b = rowptr[i], e = rowptr[i + 1];
-    if (e - b > kHeavyRow) return;
+    if (e - b > kLightRow) return;
     // 8 terms' loads in flight (indices past the row clamped, terms dropped)
     double acc = 0.0;
     for (int64_t p = b; p < e; p += 8) {
@@ -67,6 +68,35 @@ __global__ __launch_bounds__(256) void k_csr_light(int32_t m, const int64_t* __r
         for (int u = 0; u < 8; ++u) acc = (p + u < e) ? acc + t[u] : acc;
     }
     y[i] = acc;
+}
+
+// a wave per medium row: lane-strided partial sums with four terms in
+// flight, combined by a fixed butterfly
+template <bool SQ>
+__global__ __launch_bounds__(256) void k_csr_medium(int32_t nmed, const int32_t* __restrict__ mrow,
+                                                    const int64_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ colind,
+                                                    const double* __restrict__ val,
+                                                    const double* __restrict__ x, double* __restrict__ y) {
+    const int32_t h = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (h >= nmed) return;
+    const int32_t i = mrow[h];
+    const int64_t b = rowptr[i], e = rowptr[i + 1];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t p0 = b + lane; p0 < e; p0 += 256) {
+        double t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t q = p0 + 64 * u < e ? p0 + 64 * u : e - 1;
+            const double v = val[q];
+            t[u] = (SQ ? v * v : v) * x[colind[q]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = (p0 + 64 * u < e) ? a[u] + t[u] : a[u];
+    }
+    const double s = wave_sum((a[0] + a[1]) + (a[2] + a[3]));
+    if (lane == 0) y[i] = s;
 }
 
 template <bool SQ>
@@ -263,13 +293,22 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ x, float* __restr
 
 // ------------------------------------------------------ preconditioner apply
 // y1 = L11^-1 r(pivots)
-__global__ void k_pca_y1(int32_t K, const int32_t* __restrict__ perm, const double* __restrict__ X,
-                         const double* __restrict__ r, double* __restrict__ y1) {
-    const int32_t i = blockIdx.x * kB + threadIdx.x;
+__global__ __launch_bounds__(256) void k_pca_y1(int32_t K, const int32_t* __restrict__ perm,
+                                                const double* __restrict__ X, const double* __restrict__ r,
+                                                double* __restrict__ y1) {
+    // a wave per row i of X (lanes over the columns, coalesced)
+    const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (i >= K) return;
-    double s = 0.0;
-    for (int32_t c = 0; c <= i; ++c) s += X[(int64_t)i * K + c] * r[perm[c]];
-    y1[i] = s;
+    double a0 = 0.0, a1 = 0.0;
+    int32_t c = lane;
+    for (; c + 64 <= i; c += 128) {
+        a0 += X[(int64_t)i * K + c] * r[perm[c]];
+        a1 += X[(int64_t)i * K + c + 64] * r[perm[c + 64]];
+    }
+    if (c <= i) a0 += X[(int64_t)i * K + c] * r[perm[c]];
+    const double s = wave_sum(a0 + a1);
+    if (lane == 0) y1[i] = s;
 }
 
 // z_j = (r_j - sum_c L(j,c) y1_c) / S_j off the pivots, 0 on them
@@ -318,36 +357,52 @@ __global__ __launch_bounds__(256) void k_pca_lt(int32_t m, int32_t K, const floa
     }
 }
 
-// w_c = y1_c / S(pivot c) - sum_b ltpart[c][b]
-__global__ void k_pca_w(int32_t K, int32_t nb, const int32_t* __restrict__ perm, const double* __restrict__ y1,
-                        const double* __restrict__ S, const double* __restrict__ ltpart, double* __restrict__ w) {
-    const int32_t c = blockIdx.x * kB + threadIdx.x;
+// w_c = y1_c / S(pivot c) - sum_b ltpart[c][b]; a wave per column c
+__global__ __launch_bounds__(256) void k_pca_w(int32_t K, int32_t nb, const int32_t* __restrict__ perm,
+                                               const double* __restrict__ y1, const double* __restrict__ S,
+                                               const double* __restrict__ ltpart, double* __restrict__ w) {
+    const int32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (c >= K) return;
-    double s = 0.0;
-    for (int32_t b = 0; b < nb; ++b) s += ltpart[(int64_t)c * nb + b];
-    w[c] = y1[c] / S[perm[c]] - s;
+    double a0 = 0.0, a1 = 0.0;
+    int32_t b = lane;
+    for (; b + 64 < nb; b += 128) {
+        a0 += ltpart[(int64_t)c * nb + b];
+        a1 += ltpart[(int64_t)c * nb + b + 64];
+    }
+    if (b < nb) a0 += ltpart[(int64_t)c * nb + b];
+    const double s = wave_sum(a0 + a1);
+    if (lane == 0) w[c] = y1[c] / S[perm[c]] - s;
 }
 
-// z(pivot i) = sum_{c >= i} X[c][i] w_c   (L11^-T w)
+// z(pivot i) = sum_{c >= i} X[c][i] w_c   (L11^-T w): a thread per i, the
+// reads of a row c coalesced across the threads, 8 terms in flight
 __global__ void k_pca_final(int32_t K, const int32_t* __restrict__ perm, const double* __restrict__ X,
                             const double* __restrict__ w, double* __restrict__ z) {
     const int32_t i = blockIdx.x * kB + threadIdx.x;
     if (i >= K) return;
-    double s = 0.0;
-    for (int32_t c = i; c < K; ++c) s += X[(int64_t)c * K + i] * w[c];
-    z[perm[i]] = s;
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int32_t c = i;
+    for (; c + 7 < K; c += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += X[(int64_t)(c + u) * K + i] * w[c + u];
+    }
+    for (; c < K; ++c) a[0] += X[(int64_t)c * K + i] * w[c];
+    z[perm[i]] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
 }  // namespace
 
 void build_row_split(int32_t m, const std::vector<int64_t>& rowptr, std::vector<RowChunk>& chunks,
-                     std::vector<int32_t>& hrow, std::vector<int32_t>& hptr) {
+                     std::vector<int32_t>& hrow, std::vector<int32_t>& hptr, std::vector<int32_t>& mrow) {
     chunks.clear();
     hrow.clear();
+    mrow.clear();
     hptr.assign(1, 0);
     for (int32_t i = 0; i < m; ++i) {
         const int64_t b = rowptr[i], e = rowptr[i + 1];
-        if (e - b <= kHeavyRow) continue;
+        if (e - b <= kLightRow) continue;
+        if (e - b <= kHeavyRow) { mrow.push_back(i); continue; }
         hrow.push_back(i);
         for (int64_t p = b; p < e; p += kRowChunk) chunks.push_back({i, p, std::min(e, p + kRowChunk)});
         hptr.push_back((int32_t)chunks.size());
@@ -357,6 +412,11 @@ void build_row_split(int32_t m, const std::vector<int64_t>& rowptr, std::vector<
 void launch_csr_split(const RowSplitDev& rs, const int64_t* rowptr, const int32_t* colind, const double* val,
                       const double* x, double* y, bool square, hipStream_t st) {
     if (rs.m <= 0) return;
+    if (rs.nmed) {
+        const int nbm = (rs.nmed + 3) / 4;
+        if (square) help::launch_kernel(k_csr_medium<true>, nbm, kB, 0, st, rs.nmed, rs.mrow, rowptr, colind, val, x, y);
+        else help::launch_kernel(k_csr_medium<false>, nbm, kB, 0, st, rs.nmed, rs.mrow, rowptr, colind, val, x, y);
+    }
     if (square) {
         help::launch_kernel(k_csr_light<true>, grid_for(rs.m), kB, 0, st, rs.m, rowptr, colind, val, x, y);
         if (rs.nchunk) help::launch_kernel(k_csr_chunk<true>, rs.nchunk, kB, 0, st, rs.chunks, colind, val, x, rs.part);
@@ -415,12 +475,12 @@ void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* p
                      hipStream_t st) {
     // K = 0: z = r / S
     const int nb = (int)(((int64_t)m + kLtRows - 1) / kLtRows);
-    if (K > 0) help::launch_kernel(k_pca_y1, grid_for(K), kB, 0, st, K, perm, X, r, y1);
+    if (K > 0) help::launch_kernel(k_pca_y1, (K + 3) / 4, kB, 0, st, K, perm, X, r, y1);
     help::launch_kernel(k_pca_t2, grid_for(m), kB, 0, st, m, K, pos, L32, y1, S, r, z);
     if (K > 0) {
         help::launch_kernel(k_pca_lt, nb, kB, 0, st, m, K, L32, z, ltpart);
         double* w = y1 + kPcgMaxK;
-        help::launch_kernel(k_pca_w, grid_for(K), kB, 0, st, K, nb, perm, y1, S, ltpart, w);
+        help::launch_kernel(k_pca_w, (K + 3) / 4, kB, 0, st, K, nb, perm, y1, S, ltpart, w);
         help::launch_kernel(k_pca_final, grid_for(K), kB, 0, st, K, perm, X, w, z);
     }
 }

@@ -11,6 +11,7 @@
 namespace help {
 
 constexpr int kPcgMaxK = 1024;      // partial-Cholesky columns held in LDS by the kernels
+constexpr int kLightRow = 64;       // CSR rows up to this many nonzeros take one thread
 constexpr int kHeavyRow = 1024;     // CSR rows longer than this are split into chunks
 constexpr int kRowChunk = 4096;     // nonzeros per chunk of a heavy row
 constexpr int kLtRows = 2048;       // rows per workgroup of the transposed factor product
@@ -21,11 +22,14 @@ struct RowChunk {
 };
 
 // Row partition of a CSR matrix for deterministic products with heavy rows:
-// rows up to kHeavyRow nonzeros take one thread, the others are summed in
-// chunks of kRowChunk by one workgroup each and combined in chunk order.
+// rows up to kLightRow nonzeros take one thread (sequential sum), rows up to
+// kHeavyRow one wave (lane-strided partial sums, fixed reduction tree), the
+// others are summed in chunks of kRowChunk by one workgroup each and
+// combined in chunk order.
 struct RowSplitDev {
     int32_t m = 0;
-    int32_t nchunk = 0, nheavy = 0;
+    int32_t nchunk = 0, nheavy = 0, nmed = 0;
+    const int32_t* mrow = nullptr;   // medium rows
     const RowChunk* chunks = nullptr;
     const int32_t* hrow = nullptr;   // heavy rows
     const int32_t* hptr = nullptr;   // chunk range of heavy row h: [hptr[h], hptr[h+1])
@@ -33,7 +37,7 @@ struct RowSplitDev {
 };
 
 void build_row_split(int32_t m, const std::vector<int64_t>& rowptr, std::vector<RowChunk>& chunks,
-                     std::vector<int32_t>& hrow, std::vector<int32_t>& hptr);
+                     std::vector<int32_t>& hrow, std::vector<int32_t>& hptr, std::vector<int32_t>& mrow);
 
 // y = A x (square = false) or y = (A .* A) x (square = true)
 void launch_csr_split(const RowSplitDev& rs, const int64_t* rowptr, const int32_t* colind,
