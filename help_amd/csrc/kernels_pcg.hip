@@ -115,13 +115,17 @@ __global__ __launch_bounds__(256) void k_csr_chunk(const RowChunk* __restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ void k_csr_heavy(int32_t nheavy, const int32_t* __restrict__ hrow, const int32_t* __restrict__ hptr,
-                            const double* __restrict__ part, double* __restrict__ y) {
-    const int32_t h = blockIdx.x * kB + threadIdx.x;
+// a wave per heavy row: its chunk partials summed lane-strided, fixed tree
+__global__ __launch_bounds__(256) void k_csr_heavy(int32_t nheavy, const int32_t* __restrict__ hrow,
+                                                   const int32_t* __restrict__ hptr,
+                                                   const double* __restrict__ part, double* __restrict__ y) {
+    const int32_t h = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (h >= nheavy) return;
     double acc = 0.0;
-    for (int32_t c = hptr[h]; c < hptr[h + 1]; ++c) acc += part[c];
-    y[hrow[h]] = acc;
+    for (int32_t c = hptr[h] + lane; c < hptr[h + 1]; c += 64) acc += part[c];
+    const double s = wave_sum(acc);
+    if (lane == 0) y[hrow[h]] = s;
 }
 
 __global__ void k_csc_ts(int32_t n, const int64_t* __restrict__ colptr, const int32_t* __restrict__ rowind,
@@ -268,16 +272,36 @@ __global__ void k_pc_l11_gather(int32_t m, int32_t K, const int32_t* __restrict_
     L11[idx] = (c <= i) ? L[(int64_t)c * m + perm[i]] : 0.0;
 }
 
-__global__ void k_pc_l11_inv(int32_t K, const double* __restrict__ L11, double* __restrict__ X) {
-    const int32_t c = blockIdx.x * kB + threadIdx.x;
+// Column c of X = L11^-1 (unit lower) by forward substitution, a wave per
+// column: the column lives in LDS, row i's dot over t in [c, i) has the
+// lanes over t (coalesced reads of row i of L11) and a fixed reduction.
+__global__ __launch_bounds__(256) void k_pc_l11_inv(int32_t K, const double* __restrict__ L11,
+                                                    double* __restrict__ X) {
+    __shared__ double xs[4][kPcgMaxK];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t c = blockIdx.x * 4 + wave;
     if (c >= K) return;
-    for (int32_t i = 0; i < c; ++i) X[(int64_t)i * K + c] = 0.0;
-    X[(int64_t)c * K + c] = 1.0;
+    double* x = xs[wave];
+    for (int32_t i = lane; i < K; i += 64) x[i] = (i == c) ? 1.0 : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int32_t i = c + 1; i < K; ++i) {
-        double s = 0.0;
-        for (int32_t t = c; t < i; ++t) s += L11[(int64_t)i * K + t] * X[(int64_t)t * K + c];
-        X[(int64_t)i * K + c] = -s;
+        const double* Li = L11 + (int64_t)i * K;
+        double a0 = 0.0, a1 = 0.0;
+        int32_t t = c + lane;
+        for (; t + 64 < i; t += 128) {
+            a0 += Li[t] * x[t];
+            a1 += Li[t + 64] * x[t + 64];
+        }
+        if (t < i) a0 += Li[t] * x[t];
+        const double s = wave_sum(a0 + a1);
+        if (lane == 0) x[i] = -s;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    for (int32_t i = lane; i < K; i += 64) X[(int64_t)i * K + c] = x[i];
 }
 
 __global__ void k_vdiv(int64_t n, const double* __restrict__ a, const double* __restrict__ b,
@@ -424,7 +448,7 @@ void launch_csr_split(const RowSplitDev& rs, const int64_t* rowptr, const int32_
         help::launch_kernel(k_csr_light<false>, grid_for(rs.m), kB, 0, st, rs.m, rowptr, colind, val, x, y);
         if (rs.nchunk) help::launch_kernel(k_csr_chunk<false>, rs.nchunk, kB, 0, st, rs.chunks, colind, val, x, rs.part);
     }
-    if (rs.nheavy) help::launch_kernel(k_csr_heavy, grid_for(rs.nheavy), kB, 0, st, rs.nheavy, rs.hrow, rs.hptr, rs.part, y);
+    if (rs.nheavy) help::launch_kernel(k_csr_heavy, (rs.nheavy + 3) / 4, kB, 0, st, rs.nheavy, rs.hrow, rs.hptr, rs.part, y);
 }
 
 void launch_csc_t_scaled(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
@@ -459,7 +483,7 @@ void launch_pc_l11(int32_t m, int32_t K, const int32_t* perm, const double* L, d
                    hipStream_t st) {
     if (K <= 0) return;
     help::launch_kernel(k_pc_l11_gather, grid_for((int64_t)K * K), kB, 0, st, m, K, perm, L, L11);
-    help::launch_kernel(k_pc_l11_inv, grid_for(K), kB, 0, st, K, L11, X);
+    help::launch_kernel(k_pc_l11_inv, (K + 3) / 4, kB, 0, st, K, L11, X);
 }
 
 void launch_vdiv(int64_t n, const double* a, const double* b, double* out, hipStream_t st) {
