@@ -3,6 +3,9 @@
     python -m help_amd.prof_workload solve --config 1 --renews 1 --solves 3
         construct the backend on a benchmark instance, then renew and solve
         with a fixed log-uniform d (the factor and solve kernels);
+    python -m help_amd.prof_workload solve --config 3 --mode pcg --pcg-max-iter 60
+        the same in the matrix-free PCG mode, each solve capped at a fixed
+        number of CG iterations;
     python -m help_amd.prof_workload copy --mib 256
         one device-to-device copy of known size: the reference that scales
         the FETCH_SIZE / WRITE_SIZE counters to bytes.
@@ -19,8 +22,9 @@ def _solve(args):
     from . import instances
     from .backend import LinearEquation
     inst = instances.by_config(args.config)
-    mode = L.MODE_DIRECT_FP32 if args.mode == "fp32" else L.MODE_DIRECT_FP64
-    be = LinearEquation(mode=mode)
+    mode = {"fp64": L.MODE_DIRECT_FP64, "fp32": L.MODE_DIRECT_FP32, "pcg": L.MODE_PCG}[args.mode]
+    extra = {"pcg_max_iter": args.pcg_max_iter} if args.mode == "pcg" else {}
+    be = LinearEquation(mode=mode, **extra)
     be.construct(inst)
     rng = np.random.default_rng(args.seed)
     d = np.exp(rng.uniform(-args.decades, args.decades, inst.n) * np.log(10.0))
@@ -34,7 +38,8 @@ def _solve(args):
     for k in range(args.solves):
         t = time.time()
         be.solve(d, b1, b2)
-        print(f"solve {k}: {1e3 * (time.time() - t):.2f} ms", flush=True)
+        extra = f", {be.solve_stats.pcg_iters} CG iterations" if args.mode == "pcg" else ""
+        print(f"solve {k}: {1e3 * (time.time() - t):.2f} ms{extra}", flush=True)
     be.destruct()
 
 
@@ -54,7 +59,9 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     s = sub.add_parser("solve")
     s.add_argument("--config", type=int, default=1)
-    s.add_argument("--mode", default="fp64", choices=["fp64", "fp32"])
+    s.add_argument("--mode", default="fp64", choices=["fp64", "fp32", "pcg"])
+    s.add_argument("--pcg-max-iter", type=int, default=60,
+                   help="PCG mode: CG iterations per solve (a fixed amount of work)")
     s.add_argument("--renews", type=int, default=1)
     s.add_argument("--solves", type=int, default=3)
     s.add_argument("--decades", type=float, default=2.0)
