@@ -204,6 +204,18 @@ void build_schedules(help_gpu* h) {
     // trailing updates), GEMM (update matrices), then the side-stream XINV
     // steps of the wide fronts finished in the phase.
     const int32_t ns = S.nsuper;
+    // A front whose update matrix is small takes its U -= L21 L21^T in the
+    // GEMM launch of every panel (the accumulator parked in Uacc between
+    // panels) instead of a launch of its own after the last panel.
+    auto u_tiles = [&](int32_t s) {
+        const int64_t t = (S.snrow[s] - ncol(s) + kTile - 1) / kTile;
+        return t * (t + 1) / 2;
+    };
+    auto folded = [&](int32_t s, int32_t panels) {
+        const int64_t ut = u_tiles(s);
+        return ut > 0 && ut <= 256 && ut * panels <= 1024;
+    };
+    h->use_uacc = false;
     std::vector<int32_t> start(ns), done(ns), npan(ns);
     std::vector<char> smallf(ns);
     int32_t nphase = 0;
@@ -284,6 +296,21 @@ void build_schedules(help_gpu* h) {
                     h->gemm.push_back({s, r0, c0, b, b + nb, nc});
                     wg += 2.0 * nb * tile_lower_entries(r0, c0, c, nc);
                 }
+            if (folded(s, npan[s])) {
+                const int32_t pb = p - start[s], P = npan[s];
+                int32_t mode = kGemmNext, k0 = b, k1 = b + nb;
+                if (P == 1) { mode = kGemmDirect; k0 = 0; k1 = nc; }
+                else if (pb == 0) mode = kGemmFirst;
+                else if (pb == P - 1) mode = kGemmLast;
+                if (P > 1) h->use_uacc = true;
+                for (int32_t c0 = nc; c0 < c; c0 += kTile)
+                    for (int32_t r0 = c0; r0 < c; r0 += kTile) {
+                        GemmTask g{s, r0, c0, k0, k1, c};
+                        g.mode = mode;
+                        h->gemm.push_back(g);
+                        wg += 2.0 * (k1 - k0) * tile_lower_entries(r0, c0, c, c);
+                    }
+            }
         }
         h->fops.push_back({OP_POTRF, po, (int32_t)h->potrf.size() - po, wp});
         if ((int32_t)h->trsm.size() > to) h->fops.push_back({OP_TRSM, to, (int32_t)h->trsm.size() - to, wt});
@@ -295,12 +322,13 @@ void build_schedules(help_gpu* h) {
         std::vector<int32_t> wide_done;
         for (int32_t s : at_done[p]) {
             const int32_t nc = ncol(s), c = S.snrow[s];
+            if (wide(s)) wide_done.push_back(s);
+            if (folded(s, npan[s])) continue;
             for (int32_t c0 = nc; c0 < c; c0 += kTile)
                 for (int32_t r0 = c0; r0 < c; r0 += kTile) {
                     h->gemm.push_back({s, r0, c0, 0, nc, c});
                     wg += 2.0 * nc * tile_lower_entries(r0, c0, c, c);
                 }
-            if (wide(s)) wide_done.push_back(s);
         }
         if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
         push_xinv(wide_done);
@@ -482,6 +510,7 @@ void setup_device_factor(help_gpu* h) {
         h->d_scale.alloc(h->m);
         h->d_L32.alloc(std::max<int64_t>(1, nl));
         h->d_U32.alloc(std::max<int64_t>(1, nu));
+        if (h->use_uacc) h->d_Uacc32.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv32.alloc(std::max<int64_t>(1, nd));
         if (h->use_xinv) {
             h->d_X32.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
@@ -490,6 +519,7 @@ void setup_device_factor(help_gpu* h) {
     } else {
         h->d_L64.alloc(std::max<int64_t>(1, nl));
         h->d_U64.alloc(std::max<int64_t>(1, nu));
+        if (h->use_uacc) h->d_Uacc64.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv64.alloc(std::max<int64_t>(1, nd));
         if (h->use_xinv) {
             h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
@@ -520,6 +550,12 @@ T* xinv_store(help_gpu* h) {
     return (T*)h->d_X64.p;
 }
 template <typename T>
+T* uacc_store(help_gpu* h) {
+    if (std::is_same<T, float>::value) return (T*)h->d_Uacc32.p;
+    return (T*)h->d_Uacc64.p;
+}
+
+template <typename T>
 T* xinv_partial(help_gpu* h) {
     if (std::is_same<T, float>::value) return (T*)h->d_S32.p;
     return (T*)h->d_S64.p;
@@ -547,7 +583,7 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
             case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->d_earange.p, h->sym, L, U, st); break;
             case OP_POTRF: launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, Dinv, h->d_err.p, st); break;
             case OP_TRSM: launch_trsm_rows<T>(h->d_trsm.p + o.off, o.count, h->sym, L, Dinv, st); break;
-            case OP_GEMM: launch_gemm<T>(h->d_gemm.p + o.off, o.count, h->sym, L, U, st); break;
+            case OP_GEMM: launch_gemm<T>(h->d_gemm.p + o.off, o.count, h->sym, L, U, uacc_store<T>(h), st); break;
             case OP_XINV: launch_xinv<T>(h->d_xtask.p + o.off, o.count, h->sym, L, Dinv, xinv_store<T>(h), xinv_partial<T>(h), h->st_side); break;
             default: break;
         }
@@ -682,6 +718,7 @@ void ensure_fp64_factor(help_gpu* h) {
     const Supernodal& S = h->S;
     h->d_L64.alloc(std::max<int64_t>(1, S.loff[S.nsuper]));
     h->d_U64.alloc(std::max<int64_t>(1, S.uoff[S.nsuper]));
+    if (h->use_uacc) h->d_Uacc64.alloc(std::max<int64_t>(1, S.uoff[S.nsuper]));
     h->d_Dinv64.alloc(std::max<int64_t>(1, h->dinvoff[S.nsuper]));
     if (h->use_xinv) {
         h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));

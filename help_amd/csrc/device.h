@@ -43,7 +43,14 @@ struct GemmTask {
     int32_t row0, col0;   // tile origin (front coordinates)
     int32_t k0, k1;       // inner range (panel columns)
     int32_t col_end;      // columns >= col_end are not updated
+    // update-matrix tile built over several panels (the accumulator is parked
+    // in Uacc, same layout as U, between panels): kGemmDirect C -= A B^T;
+    // kGemmFirst Uacc = A B^T; kGemmNext Uacc += A B^T; kGemmLast
+    // C -= Uacc + A B^T.  The MFMA accumulation runs in the same k order as
+    // one product over the whole panel, so the result is the same.
+    int32_t mode = 0;
 };
+enum : int32_t { kGemmDirect = 0, kGemmFirst = 1, kGemmNext = 2, kGemmLast = 3 };
 
 // Panel task: factor the diagonal block [b, b+nb) of front s (every task of
 // the step does it redundantly in LDS) and solve the rows of block rb below it.
@@ -131,7 +138,7 @@ template <typename T>
 void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, const T* Dinv,
                       hipStream_t st);
 template <typename T>
-void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st);
+void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, T* Uacc, hipStream_t st);
 template <typename T>
 void launch_mfma_probe(const T* a, const T* b, T* out, hipStream_t st);
 // off-diagonal inverse blocks X_ij = -Dinv_i S_ij, S_ij = sum_{k=j}^{i-1} L_ik X_kj

@@ -115,6 +115,7 @@ struct help_gpu {
     std::vector<SolveTask> fgemv, bgemv;
     // explicit inverse of L11 for supernodes wider than one 64-column block
     bool use_xinv = false;
+    bool use_uacc = false;   // some update matrix is built over several panels (Uacc)
     std::vector<int64_t> xoff;             // nsuper+1 offsets of the inverse blocks
     std::vector<XTask> xtask;
     std::vector<SolveTask> fxs, bxs;       // row blocks (forward) / column blocks (backward)
@@ -135,12 +136,12 @@ struct help_gpu {
         d_rowidx, d_relmap, d_childptr, d_child, d_list_small, d_list_fwd, d_list_bwd, d_err;
     DBuf<double> d_val, d_csrval, d_d, d_dinv, d_b1, d_b2, d_x1, d_x2, d_rp, d_W, d_t, d_tn, d_r,
         d_c, d_z, d_p, d_q, d_part, d_scal, d_slotvals;
-    DBuf<double> d_L64, d_U64, d_Dinv64;
+    DBuf<double> d_L64, d_U64, d_Dinv64, d_Uacc64;
     // FP32 mode: symmetric diagonal scaling of the factored matrix
     std::vector<int32_t> slot_row;
     DBuf<int32_t> d_slot_row, d_slot_col;
     DBuf<double> d_scale;
-    DBuf<float> d_L32, d_U32, d_Dinv32;
+    DBuf<float> d_L32, d_U32, d_Dinv32, d_Uacc32;
     DBuf<int8_t> d_big;
     DBuf<EaTask> d_ea;
     DBuf<int32_t> d_earange;

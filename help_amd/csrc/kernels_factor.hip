@@ -702,7 +702,7 @@ constexpr int kKc = 16;   // inner-dimension chunk staged in LDS
 // the current one (one barrier per chunk).
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, DevSym sym,
-                                              T* __restrict__ L, T* __restrict__ U) {
+                                              T* __restrict__ L, T* __restrict__ U, T* __restrict__ Uacc) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int kPer = kKc / 4;
     __shared__ T As[2][kKc * kTile];
@@ -721,6 +721,22 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
     acc_t acc[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    // update-matrix tile continued from an earlier panel: reload the parked
+    // accumulator (entries outside the tile's lower part read element 0)
+    T* Ua = Uacc ? Uacc + sym.uoff[s] : nullptr;
+    if (tk.mode == kGemmNext || tk.mode == kGemmLast) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int32_t i = tk.row0 + wr + a * 16 + Mfma<T>::row(lane, r);
+                    const int32_t j = tk.col0 + wc + b * 16 + (lane & 15);
+                    const bool ok = i < c && j < tk.col_end && i >= j;
+                    acc[a][b][r] = Ua[ok ? (int64_t)(j - n) * cu + (i - n) : 0];
+                }
+    }
     const int li = tid & 63, lk = tid >> 6;
     const int32_t ia = tk.row0 + li, ib = tk.col0 + li;
     const bool ain = ia < c, bin = ib < c;
@@ -748,6 +764,20 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
         __syncthreads();
         if (kb + kKc < tk.k1) load(kb + kKc);
         mfma_tile<T>(As[buf], Bs[buf], kKc, lane, wr, wc, acc);
+    }
+    if (tk.mode == kGemmFirst || tk.mode == kGemmNext) {
+        // park the accumulator for the next panel
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int32_t i = tk.row0 + wr + a * 16 + Mfma<T>::row(lane, r);
+                    const int32_t j = tk.col0 + wc + b * 16 + (lane & 15);
+                    if (i < c && j < tk.col_end && i >= j) Ua[(int64_t)(j - n) * cu + (i - n)] = acc[a][b][r];
+                }
+        return;
     }
     // read-modify-write of C: every load is issued before the first store
     // (entries outside the lower part or the front read the front's first
@@ -994,8 +1024,8 @@ void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges
     if (count > 0) help::launch_kernel(k_extend_add<T>, count, 256, 0, st, tasks, ranges, sym, L, U);
 }
 template <typename T>
-void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_gemm<T>, count, 256, 0, st, tasks, sym, L, U);
+void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, T* Uacc, hipStream_t st) {
+    if (count > 0) help::launch_kernel(k_gemm<T>, count, 256, 0, st, tasks, sym, L, U, Uacc);
 }
 template <typename T>
 void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv, int32_t* err,
@@ -1022,7 +1052,7 @@ void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, cons
     template void launch_small_fronts<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,      \
                                          int32_t, hipStream_t);                                     \
     template void launch_extend_add<T>(const EaTask*, int32_t, const int32_t*, DevSym, T*, T*, hipStream_t); \
-    template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, hipStream_t);           \
+    template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, T*, hipStream_t);           \
     template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, T*, int32_t*,        \
                                        hipStream_t);                                                \
     template void launch_trsm_rows<T>(const PanelTask*, int32_t, DevSym, T*, const T*, hipStream_t);     \
