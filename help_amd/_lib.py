@@ -116,6 +116,8 @@ SIGNATURES = {
     "help_gpu_destroy": (C.c_int, [_P]),
     "help_gpu_renew": (C.c_int, [_P, _D, C.POINTER(RenewStats)]),
     "help_gpu_solve": (C.c_int, [_P, _D, _D, _D, _D, _D, C.POINTER(SolveStats)]),
+    "help_gpu_solve2": (C.c_int, [_P, _D, _D, _D, _D, _D, _D, _D, _D, _D, C.POINTER(SolveStats),
+                                  C.POINTER(SolveStats)]),
     "help_gpu_last_error": (C.c_char_p, [_P]),
     "help_gpu_symbolic_info": (C.c_int, [_P, C.POINTER(SymbolicInfo)]),
     "help_gpu_symbolic_arrays": (C.c_int, [_P, _I32, _I32, _I32]),
@@ -139,6 +141,7 @@ SIGNATURES = {
                                                 C.POINTER(C.c_int64), _D]),
     "help_gpu_backend_solve_stats": (C.c_int, [C.POINTER(Backend), C.POINTER(C.c_int64),
                                                C.POINTER(C.c_int64), _D]),
+    "help_gpu_backend_solve2": (C.c_int, [C.POINTER(Backend), _D, _D, _D, _D, _D, _D, _D, _D, _D]),
     "help_mpnn_num_params": (C.c_int, []),
     "help_mpnn_new": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "help_mpnn_free": (None, [_P]),

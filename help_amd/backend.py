@@ -109,6 +109,18 @@ class LinearEquation:
                                            L.dptr(x2), C.byref(self.solve_stats)), "solve")
         return x1, x2
 
+    def solve2(self, d, b1a, b2a, b1b, b2b):
+        """Two solves with the same d in one call (help_gpu_solve2)."""
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        bs = [np.ascontiguousarray(v, dtype=np.float64) for v in (b1a, b2a, b1b, b2b)]
+        x1a, x1b = np.empty(self.n), np.empty(self.n)
+        x2a, x2b = np.empty(self.m), np.empty(self.m)
+        sb = L.SolveStats()
+        self._check(L.lib().help_gpu_solve2(self.h, L.dptr(d), L.dptr(bs[0]), L.dptr(bs[1]), L.dptr(x1a),
+                                            L.dptr(x2a), L.dptr(bs[2]), L.dptr(bs[3]), L.dptr(x1b),
+                                            L.dptr(x2b), C.byref(self.solve_stats), C.byref(sb)), "solve2")
+        return (x1a, x2a), (x1b, x2b)
+
     # ---- introspection for parity tests ----
     def symbolic_info(self):
         info = L.SymbolicInfo()

@@ -437,8 +437,11 @@ void setup_device_common(help_gpu* h) {
     h->d_colind.upload(h->csrcol, st);
     h->d_csrval.upload(h->csrval, st);
     const int32_t m = h->m, n = h->n;
-    h->d_d.alloc(n); h->d_dinv.alloc(n); h->d_b1.alloc(n); h->d_x1.alloc(n); h->d_tn.alloc(n);
-    h->d_b2.alloc(m); h->d_x2.alloc(m); h->d_rp.alloc(m); h->d_t.alloc(m); h->d_r.alloc(m); h->d_c.alloc(m);
+    // b1, x1, b2, x2, rp, t and c hold two right-hand sides (help_gpu_solve2)
+    h->d_d.alloc(n); h->d_dinv.alloc(n); h->d_b1.alloc(2 * (size_t)n); h->d_x1.alloc(2 * (size_t)n);
+    h->d_tn.alloc(n);
+    h->d_b2.alloc(2 * (size_t)m); h->d_x2.alloc(2 * (size_t)m); h->d_rp.alloc(2 * (size_t)m);
+    h->d_t.alloc(2 * (size_t)m); h->d_r.alloc(m); h->d_c.alloc(2 * (size_t)m);
     h->d_z.alloc(m); h->d_p.alloc(m); h->d_q.alloc(m);
     h->d_part.alloc(kDotPartials); h->d_scal.alloc(8);
     h->d_err.alloc(1);
@@ -500,8 +503,10 @@ void setup_device_factor(help_gpu* h) {
     h->d_bxs.upload(h->bxs, st);
     h->d_xtask.upload(h->xtask, st);
     h->d_xoff.upload(h->xoff, st);
-    h->d_y.alloc(h->m);
-    h->d_W.alloc(std::max<int64_t>(1, S.reloff[S.nsuper]));
+    // two right-hand sides' worth of solve vectors (help_gpu_solve2)
+    h->w_len = std::max<int64_t>(1, S.reloff[S.nsuper]);
+    h->d_y.alloc(2 * (size_t)h->m);
+    h->d_W.alloc(2 * (size_t)h->w_len);
     int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
     int64_t nd = h->dinvoff[S.nsuper];
     if (h->use_fp32) {
@@ -596,27 +601,32 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
 }
 
 // x (permuted numbering, FP64) <- (L L^T)^-1 x; the forward result goes
-// to d_y, the backward one back to x
+// to d_y, the backward one back to x.  nrhs right-hand sides at x + r m
+// (r < nrhs) are solved by the same launches, each with exactly the
+// operations of a single solve.
 template <typename T>
-void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x) {
+void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x, int nrhs = 1) {
     hipStream_t st = h->st;
     const T* X = xinv_store<T>(h);
     double* y = h->d_y.p;
+    RhsStride rs;
+    rs.v = h->m;
+    rs.w = h->w_len;
     for (const Op& o : h->sops_fwd) {
-        ProfScope ps(h, o.kind, o.work);
+        ProfScope ps(h, o.kind, o.work * nrhs);
         if (o.kind == OP_FWD)
             launch_fwd_level<T>(h->d_list_fwd.p + o.off, o.count, o.ntiny, h->d_fxs.p + o.xoff, o.xcount, h->sym,
-                                h->d_big.p, L, Dinv, X, x, y, h->d_W.p, st);
+                                h->d_big.p, L, Dinv, X, x, y, h->d_W.p, nrhs, rs, st);
         else
-            launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, y, h->d_W.p, st);
+            launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, y, h->d_W.p, nrhs, rs, st);
     }
     for (const Op& o : h->sops_bwd) {
-        ProfScope ps(h, o.kind, o.work);
+        ProfScope ps(h, o.kind, o.work * nrhs);
         if (o.kind == OP_BWD)
             launch_bwd_level<T>(h->d_list_bwd.p + o.off, o.count, o.ntiny, h->d_bxs.p + o.xoff, o.xcount, h->sym,
-                                h->d_big.p, L, Dinv, X, y, h->d_t.p, x, st);
+                                h->d_big.p, L, Dinv, X, y, h->d_t.p, x, nrhs, rs, st);
         else
-            launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, st);
+            launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, nrhs, rs, st);
     }
     HIPCHK(hipGetLastError());
 }
@@ -1052,6 +1062,88 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
     }
 }
 
+int help_gpu_solve2(help_gpu* h, const double* d, const double* b1a, const double* b2a, double* x1a,
+                    double* x2a, const double* b1b, const double* b2b, double* x1b, double* x2b,
+                    help_solve_stats* ssa, help_solve_stats* ssb) {
+    if (!h || !d || !b1a || !b2a || !x1a || !x2a || !b1b || !b2b || !x1b || !x2b) return HELP_ERR_ARG;
+    if (!h->device || !h->factored) { h->err = "solve2: call renew first"; return HELP_ERR_STATE; }
+    const int max_it = h->fp32_off ? h->opts.refine_max : h->opts.refine_fp64;
+    if (h->use_pcg || h->factor_fp32 || max_it > 0) {
+        // iterative paths decide per right-hand side: two single solves
+        int st = help_gpu_solve(h, d, b1a, b2a, x1a, x2a, ssa);
+        return st != HELP_OK ? st : help_gpu_solve(h, d, b1b, b2b, x1b, x2b, ssb);
+    }
+    try {
+        hipStream_t st = h->st;
+        const int32_t m = h->m, n = h->n;
+        const double* b1[2] = {b1a, b1b};
+        const double* b2[2] = {b2a, b2b};
+        double* x1[2] = {x1a, x1b};
+        double* x2[2] = {x2a, x2b};
+        HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * n, hipMemcpyDefault, st));
+        for (int r = 0; r < 2; ++r) {
+            HIPCHK(hipMemcpyAsync(h->d_b1.p + (size_t)r * n, b1[r], sizeof(double) * n, hipMemcpyDefault, st));
+            HIPCHK(hipMemcpyAsync(h->d_b2.p + (size_t)r * m, b2[r], sizeof(double) * m, hipMemcpyDefault, st));
+        }
+        HIPCHK(hipEventRecord(h->ev[0], st));
+        {
+            ProfScope ps(h, HELP_K_RHS, 2 * h->work_rhs);
+            for (int r = 0; r < 2; ++r)
+                launch_rhs(m, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p + (size_t)r * n, h->d_d.p,
+                           h->d_b2.p + (size_t)r * m, h->d_iperm.p, h->d_rp.p + (size_t)r * m, st);
+        }
+        HIPCHK(hipEventRecord(h->ev[1], st));
+        // the FP64 factor solve of both right-hand sides, then each one's
+        // residual norms (reported only, as in the single solve)
+        for (int r = 0; r < 2; ++r)
+            launch_gather_perm(m, h->d_rp.p + (size_t)r * m, h->d_iperm.p, h->d_c.p + (size_t)r * m, st);
+        run_trisolve<double>(h, h->d_L64.p, h->d_Dinv64.p, h->d_rp.p, 2);
+        launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
+        for (int r = 0; r < 2; ++r) {
+            double* x = h->d_x2.p + (size_t)r * m;
+            const double* rhs = h->d_c.p + (size_t)r * m;
+            launch_gather_perm(m, h->d_rp.p + (size_t)r * m, h->d_iperm.p, x, st);
+            launch_normal_matvec(m, n, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_colptr.p,
+                                 h->d_rowind.p, h->d_val.p, h->d_dinv.p, h->opts.beta, x, h->d_tn.p, h->d_q.p,
+                                 st);
+            launch_sub(m, rhs, h->d_q.p, h->d_r.p, st);
+            launch_dot(rhs, rhs, m, h->d_part.p, h->d_scal.p + 2 + 2 * r, st);
+            launch_dot(h->d_r.p, h->d_r.p, m, h->d_part.p, h->d_scal.p + 3 + 2 * r, st);
+        }
+        HIPCHK(hipMemcpyAsync(h->h_pin, h->d_scal.p + 2, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(h->ev[2], st));
+        {
+            ProfScope ps(h, HELP_K_BACKSUB, 2 * h->work_backsub);
+            for (int r = 0; r < 2; ++r)
+                launch_backsub(n, h->d_colptr.p, h->d_rowind.p, h->d_val.p, h->d_b1.p + (size_t)r * n, h->d_d.p,
+                               h->d_x2.p + (size_t)r * m, h->d_x1.p + (size_t)r * n, st);
+        }
+        HIPCHK(hipEventRecord(h->ev[3], st));
+        HIPCHK(hipGetLastError());
+        for (int r = 0; r < 2; ++r) {
+            HIPCHK(hipMemcpyAsync(x1[r], h->d_x1.p + (size_t)r * n, sizeof(double) * n, hipMemcpyDefault, st));
+            HIPCHK(hipMemcpyAsync(x2[r], h->d_x2.p + (size_t)r * m, sizeof(double) * m, hipMemcpyDefault, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        if (h->prof) collect_prof(h);
+        help_solve_stats* ss[2] = {ssa, ssb};
+        for (int r = 0; r < 2; ++r) {
+            if (!ss[r]) continue;
+            help_solve_stats loc{};
+            const double rr0 = h->h_pin[2 * r], rr = h->h_pin[2 * r + 1];
+            loc.rel_residual = rr0 == 0.0 ? 0.0 : std::sqrt(rr / rr0);
+            // the shared launches' times, split evenly
+            loc.ms_rhs = 0.5 * elapsed_ms(h->ev[0], h->ev[1]);
+            loc.ms_solve = 0.5 * elapsed_ms(h->ev[1], h->ev[2]);
+            loc.ms_back = 0.5 * elapsed_ms(h->ev[2], h->ev[3]);
+            *ss[r] = loc;
+        }
+        return HELP_OK;
+    } catch (const std::exception& e) {
+        return fail(h, e);
+    }
+}
+
 int help_gpu_symbolic_info(const help_gpu* h, help_symbolic_info* info) {
     if (!h || !info) return HELP_ERR_ARG;
     if (h->use_pcg) return HELP_ERR_STATE;   // no ordering or factor in PCG mode
@@ -1288,6 +1380,24 @@ static int gb_solve(void* ctx, const double* d, const double* b1, const double* 
         c->refine_steps += ss.refine_steps;
         c->pcg_iters += ss.pcg_iters;
         c->max_rel_residual = std::max(c->max_rel_residual, ss.rel_residual);
+    }
+    return st;
+}
+
+int help_gpu_backend_solve2(help_backend* be, const double* d, const double* b1a, const double* b2a,
+                            double* x1a, double* x2a, const double* b1b, const double* b2b, double* x1b,
+                            double* x2b) {
+    if (!be || !be->ctx) return HELP_ERR_ARG;
+    auto* c = (GpuBackendCtx*)be->ctx;
+    help_solve_stats ss[2]{};
+    int st = help_gpu_solve2(c->h, d, b1a, b2a, x1a, x2a, b1b, b2b, x1b, x2b, &ss[0], &ss[1]);
+    if (st == HELP_OK) {
+        for (const help_solve_stats& s : ss) {
+            c->n_solve += 1;
+            c->refine_steps += s.refine_steps;
+            c->pcg_iters += s.pcg_iters;
+            c->max_rel_residual = std::max(c->max_rel_residual, s.rel_residual);
+        }
     }
     return st;
 }

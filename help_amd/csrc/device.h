@@ -171,21 +171,28 @@ constexpr int kTinyRows = 64;      // supernodes with at most this many rows tak
 //            children's update vectors;
 //   backward x_s(block j) = sum_{i>=j} X_ij^T (y_s - t_s)(block i).
 // Supernodes with big[s] != 0 leave their L21 products to the gemv kernels.
+// Every launch solves nrhs right-hand sides at once (blockIdx.y): right-hand
+// side r uses the vectors at offset r * rs.v and the update vectors at
+// offset r * rs.w, with the same operations as a single solve.
+struct RhsStride {
+    int64_t v = 0;   // b, y, x, t
+    int64_t w = 0;   // W
+};
 template <typename T>
 void launch_fwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
                       DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* b,
-                      double* y, double* W, hipStream_t st);
+                      double* y, double* W, int nrhs, RhsStride rs, hipStream_t st);
 template <typename T>
 void launch_bwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
                       DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* y,
-                      const double* t, double* x, hipStream_t st);
+                      const double* t, double* x, int nrhs, RhsStride rs, hipStream_t st);
 // Forward, big supernodes: W_s += L21 y_s by 64-row blocks.
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
-                     const double* y, double* W, hipStream_t st);
+                     const double* y, double* W, int nrhs, RhsStride rs, hipStream_t st);
 // Backward, big supernodes: t[f+j] = sum_{i>=n} L21(i,j) x[row_i] by 16-column blocks.
 template <typename T>
 void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,
-                     const double* x, double* t, hipStream_t st);
+                     const double* x, double* t, int nrhs, RhsStride rs, hipStream_t st);
 
 }  // namespace help

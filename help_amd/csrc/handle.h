@@ -151,6 +151,7 @@ struct help_gpu {
     DBuf<XTask> d_xtask;
     DBuf<int64_t> d_xoff;
     DBuf<float> d_X32, d_S32;              // inverse blocks and their partial sums
+    int64_t w_len = 1;            // update-vector entries per right-hand side
     DBuf<double> d_X64, d_S64, d_y;
     DevSym sym{};
     // side stream of the factorization: the inverse blocks of a finished

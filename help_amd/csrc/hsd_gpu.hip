@@ -122,7 +122,7 @@ struct HsdGpu {
     const double *csrval, *cscval;
     DBuf<double> c_, b_, u_;
     // device: iterates and work vectors
-    DBuf<double> x, y, z, xu, yu, zu, xf, D, D_u, r_p, r_d, sle1, sle2, bot1, bot1u, bot2, bot2u, dir1,
+    DBuf<double> x, y, z, xu, yu, zu, xf, D, D_u, r_p, r_d, sle1, sle2, sle1b, sle2b, bot1, bot1u, bot2, bot2u, dir1,
         dir1u, dir2, dir2u, pp, ppu, qq, qqu, uu, uuu, vv, vvu, hxz, hxuzu, dx, dxu, dy, dyu, dz, dzu,
         dx_b, dxu_b, dy_b, dyu_b, dz_b, dzu_b, part, out, tmpm, tmpn;
     double tau = 1, kappa = 1, rho = 1, inf0 = 1;
@@ -174,9 +174,9 @@ struct HsdGpu {
             if (k > 0 && src) HIPCHK(hipMemcpyAsync(d.p, src, sizeof(double) * k, hipMemcpyHostToDevice, st));
         };
         up(c_, l->c, n); up(b_, l->b, m); up(u_, l->ub, n_ub);
-        for (DBuf<double>* v : {&x, &z, &D, &r_d, &sle1, &bot1, &dir1, &pp, &uu, &hxz, &dx, &dz, &dx_b, &dz_b})
+        for (DBuf<double>* v : {&x, &z, &D, &r_d, &sle1, &sle1b, &bot1, &dir1, &pp, &uu, &hxz, &dx, &dz, &dx_b, &dz_b})
             v->alloc(std::max(n, 1));
-        for (DBuf<double>* v : {&y, &r_p, &sle2, &bot2, &dir2, &qq, &vv, &dy, &dy_b}) v->alloc(std::max(m, 1));
+        for (DBuf<double>* v : {&y, &r_p, &sle2, &sle2b, &bot2, &dir2, &qq, &vv, &dy, &dy_b}) v->alloc(std::max(m, 1));
         for (DBuf<double>* v : {&xu, &yu, &zu, &D_u, &bot1u, &bot2u, &dir1u, &dir2u, &ppu, &qqu, &uuu, &vvu,
                                 &hxuzu, &dxu, &dyu, &dzu, &dxu_b, &dyu_b, &dzu_b})
             v->alloc(std::max(n_ub, 1));
@@ -198,6 +198,15 @@ struct HsdGpu {
         Timer t(&t_solve);
         ++n_solve;
         int s = be->solve(be->ctx, D.p, b1, b2, x1, x2);
+        if (s && !backend_error) backend_error = s;
+    }
+    // two solves with the same D in one backend call (help_gpu_solve2)
+    void solve2(const double* b1a, const double* b2a, double* x1a, double* x2a, const double* b1b,
+                const double* b2b, double* x1b, double* x2b) {
+        Timer t(&t_solve);
+        n_solve += 2;
+        int s = help_gpu_backend_solve2(const_cast<help_backend*>(be), D.p, b1a, b2a, x1a, x2a, b1b, b2b, x1b,
+                                        x2b);
         if (s && !backend_error) backend_error = s;
     }
 
@@ -276,16 +285,20 @@ struct HsdGpu {
         infe = tau / (kappa * inf0);
     }
 
-    // HSDSolver.cpp:156-208
-    void solve_augmented(const double* s, const double* s_u, const double* t, const double* t_u, double* p,
-                         double* p_u, double* q, double* q_u) {
-        double* s1 = sle1.p;
+    // HSDSolver.cpp:156-208 in two halves around the solve, so that two
+    // systems can share one backend call: the right-hand side (s1, s2) ...
+    void augmented_rhs(const double* s, const double* s_u, const double* t, const double* t_u, double* s1,
+                       double* s2) {
         const double* Dup = D_u.p;
         const int nlbfr = n_lb + n_fr;
         map(nlbfr, [=] __device__(int64_t i) { s1[i] = s[i]; });
         map(n_ub, [=] __device__(int64_t i) { s1[i] += Dup[i] * t_u[i] - s_u[i]; });
-        HIPCHK(hipMemcpyAsync(sle2.p, t, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-        solve(sle1.p, sle2.p, p, q);
+        HIPCHK(hipMemcpyAsync(s2, t, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+    }
+    // ... and, with the solution (p, q) of that system, the remaining parts
+    void augmented_finish(const double* s_u, const double* t_u, double* p, double* p_u, double* q,
+                          double* q_u) {
+        const double* Dup = D_u.p;
         map(m, [=] __device__(int64_t i) { q[i] = -q[i]; });
         map(n_ub, [=] __device__(int64_t i) {
             p_u[i] = t_u[i] - p[i];
@@ -308,8 +321,9 @@ struct HsdGpu {
         return r;
     }
 
-    // HSDSolver.cpp:226-261
-    double bottom() {
+    // HSDSolver.cpp:226-261: the right-hand side of the bottom system into
+    // (sle1b, sle2b); bottom_finish() takes its solution from (pp, qq)
+    void bottom_rhs() {
         double *b1 = bot1.p, *b1u = bot1u.p, *b2 = bot2.p, *b2u = bot2u.p;
         const double *cp = c_.p, *bp = b_.p, *upp = u_.p, *xfp = xf.p, *xp = x.p;
         const double rt = rho / tau;
@@ -318,23 +332,44 @@ struct HsdGpu {
         map(n_fr, [=] __device__(int64_t i) { b1[nlb + i] -= rt * xfp[i] * xp[nlb + i]; });
         map(m, [=] __device__(int64_t i) { b2[i] = -bp[i]; });
         map(n_ub, [=] __device__(int64_t i) { b1u[i] = 0; b2u[i] = -upp[i]; });
-        solve_augmented(b1, b1u, b2, b2u, pp.p, ppu.p, qq.p, qqu.p);
+        augmented_rhs(b1, b1u, b2, b2u, sle1b.p, sle2b.p);
+    }
+    double bottom_finish() {
+        augmented_finish(bot1u.p, bot2u.p, pp.p, ppu.p, qq.p, qqu.p);
         return mix_dots(pp.p, qq.p, qqu.p);
+    }
+
+    // The bottom system and the first (affine) direction share D, and the
+    // direction's right-hand side does not depend on the bottom solution,
+    // so both systems go through one solve2 call; the arithmetic is the
+    // same as solving them one after the other (HSDSolver.cpp:797-800).
+    int bottom_and_direction(double eta, double& DOT, double* hrxz, double* hrxuzu, double hrtk, double* odx,
+                             double* odxu, double& odtau, double* ody, double* odyu, double* odz, double* odzu,
+                             double& odkappa, double& rpn, double& rdn, double& rgn) {
+        const double UP = direction_rhs(eta, hrxz, hrxuzu, hrtk);
+        solve2(sle1b.p, sle2b.p, pp.p, qq.p, sle1.p, sle2.p, uu.p, vv.p);
+        DOT = bottom_finish();
+        return direction_finish(eta, DOT, UP, hrxz, hrxuzu, hrtk, odx, odxu, odtau, ody, odyu, odz, odzu,
+                                odkappa, rpn, rdn, rgn);
     }
 
     // HSDSolver.cpp:281-473
     int direction(double eta, double DOT, double* hrxz, double* hrxuzu, double hrtk, double* odx, double* odxu,
                   double& odtau, double* ody, double* odyu, double* odz, double* odzu, double& odkappa,
                   double& rpn, double& rdn, double& rgn) {
+        const double UP = direction_rhs(eta, hrxz, hrxuzu, hrtk);
+        solve(sle1.p, sle2.p, uu.p, vv.p);
+        return direction_finish(eta, DOT, UP, hrxz, hrxuzu, hrtk, odx, odxu, odtau, ody, odyu, odz, odzu,
+                                odkappa, rpn, rdn, rgn);
+    }
+    // the direction's right-hand side into (sle1, sle2); returns UP
+    double direction_rhs(double eta, double* hrxz, double* hrxuzu, double hrtk) {
         Timer tm(&t_direction);
-        const double solve0 = t_solve;
         double *d1 = dir1.p, *d1u = dir1u.p, *d2 = dir2.p, *d2u = dir2u.p;
         const double *rdp = r_d.p, *rpp = r_p.p, *xp = x.p, *zp = z.p, *xup = xu.p, *zup = zu.p,
-                     *yup = yu.p, *cp = c_.p, *bp = b_.p, *upp = u_.p, *xfp = xf.p;
-        const double tau_ = tau, kappa_ = kappa, rho_ = rho;
-        const int nlb = n_lb, nfr = n_fr;
-        const int64_t* rptr = rowptr; const int32_t* rcol = colind; const double* rval = csrval;
-        const int64_t* cptr = colptr; const int32_t* crow = rowind; const double* cval = cscval;
+                     *yup = yu.p, *cp = c_.p, *upp = u_.p;
+        const double tau_ = tau;
+        const int nlb = n_lb;
         map(n, [=] __device__(int64_t i) { d1[i] = eta * rdp[i]; });
         map(n_ub, [=] __device__(int64_t i) {
             d1u[i] = eta * (zup[i] + yup[i]);
@@ -364,8 +399,22 @@ struct HsdGpu {
         UP += reduce<1>(n_lb, [=] __device__(int64_t i, double* v) {
                   v[0] = xp[i] < zp[i] ? cp[i] * hrxz[i] : 0.0;
               })[0];
+        augmented_rhs(d1, d1u, d2, d2u, sle1.p, sle2.p);
+        return UP;
+    }
+    // the rest of the direction from the solution (uu, vv) of that system
+    int direction_finish(double eta, double DOT, double UP, double* hrxz, double* hrxuzu, double hrtk,
+                         double* odx, double* odxu, double& odtau, double* ody, double* odyu, double* odz,
+                         double* odzu, double& odkappa, double& rpn, double& rdn, double& rgn) {
+        Timer tm(&t_direction);
+        double *d1 = dir1.p, *d1u = dir1u.p, *d2 = dir2.p, *d2u = dir2u.p;
+        const double *rdp = r_d.p, *rpp = r_p.p, *xp = x.p, *zp = z.p, *xup = xu.p, *zup = zu.p,
+                     *yup = yu.p, *cp = c_.p, *bp = b_.p, *upp = u_.p, *xfp = xf.p;
+        const double tau_ = tau, rho_ = rho;
+        const int nlb = n_lb, nfr = n_fr;
+        const int64_t* cptr = colptr; const int32_t* crow = rowind; const double* cval = cscval;
         double DOWN = D_g - DOT;
-        solve_augmented(d1, d1u, d2, d2u, uu.p, uuu.p, vv.p, vvu.p);
+        augmented_finish(d1u, d2u, uu.p, uuu.p, vv.p, vvu.p);
         if (DOWN <= 1e-13) odtau = 0;
         else odtau = (UP - mix_dots(uu.p, vv.p, vvu.p)) / std::max(D_g, DOWN);
         const double dt = odtau;
@@ -419,8 +468,6 @@ struct HsdGpu {
                   })[0];
         rgn = eta * r_g + odkappa - mix_dots(odx, ody, odyu) - rho * tmp * dt / (tau * tau);
         rgn = rgn * rgn;
-        (void)kappa_;
-        t_direction -= t_solve - solve0;
         if (rpn + rdn <= 1e-10 || rpn + rdn + rgn <= sum_r_norm * 0.1) return 1;
         return 0;
     }
@@ -559,11 +606,11 @@ struct HsdGpu {
         newton_parameters();
         if (status != HELP_HSD_OK) return;
         renew();
-        double DOT = bottom();
-        double rpn, rdn, rgn, phi = 1.0, eta = 1.0;
+        bottom_rhs();
+        double DOT, rpn, rdn, rgn, phi = 1.0, eta = 1.0;
         set_hat(mu0, 0.0, 0.0, false);
-        int newton_ok = direction(eta, DOT, hxz.p, hxuzu.p, htk, dx.p, dxu.p, dtau, dy.p, dyu.p, dz.p, dzu.p,
-                                  dkappa, rpn, rdn, rgn);
+        int newton_ok = bottom_and_direction(eta, DOT, hxz.p, hxuzu.p, htk, dx.p, dxu.p, dtau, dy.p, dyu.p, dz.p,
+                                             dzu.p, dkappa, rpn, rdn, rgn);
         double p_step = 0, d_step = 0;
         if (newton_ok) get_stepsize(p_step, d_step);
         else phi = 0.0;
@@ -603,11 +650,11 @@ struct HsdGpu {
                 (rel_gap <= kGapTol && pinf <= kPinfTol && dinf <= kDinfTol))
                 break;
             renew();
-            double DOT = bottom();
-            double rpn, rdn, rgn, phi = 1.0, eta = 1.0;
+            bottom_rhs();
+            double DOT, rpn, rdn, rgn, phi = 1.0, eta = 1.0;
             set_hat(0.0, 0.0, 0.0, false);
-            int newton_ok = direction(eta, DOT, hxz.p, hxuzu.p, htk, dx.p, dxu.p, dtau, dy.p, dyu.p, dz.p,
-                                      dzu.p, dkappa, rpn, rdn, rgn);
+            int newton_ok = bottom_and_direction(eta, DOT, hxz.p, hxuzu.p, htk, dx.p, dxu.p, dtau, dy.p, dyu.p,
+                                                 dz.p, dzu.p, dkappa, rpn, rdn, rgn);
             double p_step = 0, d_step = 0, gamma;
             if (newton_ok) {
                 get_stepsize(p_step, d_step);

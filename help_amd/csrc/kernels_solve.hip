@@ -349,9 +349,10 @@ __global__ __launch_bounds__(256) void k_fwd_level(const int32_t* __restrict__ l
                                                    const int8_t* __restrict__ big, const T* __restrict__ L,
                                                    const T* __restrict__ Dinv, const T* __restrict__ X,
                                                    const double* __restrict__ b, double* __restrict__ y,
-                                                   double* __restrict__ W) {
+                                                   double* __restrict__ W, RhsStride rs) {
     __shared__ double xs[kMaxLdsCols];
     __shared__ double red[4][64];
+    b += blockIdx.y * rs.v; y += blockIdx.y * rs.v; W += blockIdx.y * rs.w;
     const int32_t ntb = (ntiny + 3) / 4;
     const int32_t bid = blockIdx.x;
     if (bid < ntb) {
@@ -371,8 +372,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
                                                   const T* __restrict__ L, const T* __restrict__ Dinv,
                                                   const double* __restrict__ b, double* __restrict__ y,
-                                                  double* __restrict__ W) {
+                                                  double* __restrict__ W, RhsStride rs) {
     __shared__ double xw[4][64];
+    b += blockIdx.y * rs.v; y += blockIdx.y * rs.v; W += blockIdx.y * rs.w;
     const int wave = threadIdx.x >> 6;
     const int32_t idx = blockIdx.x * 4 + wave;
     if (idx < count) fwd_tiny<T>(list[idx], sym, L, Dinv, b, y, W, xw[wave]);
@@ -382,9 +384,10 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_gemv(const SolveTask* __restrict__ tasks, DevSym sym,
                                                   const T* __restrict__ L,
                                                   const double* __restrict__ y,
-                                                  double* __restrict__ W) {
+                                                  double* __restrict__ W, RhsStride rs) {
     __shared__ double xs[kMaxLdsCols];
     __shared__ double red[4][64];
+    y += blockIdx.y * rs.v; W += blockIdx.y * rs.w;
     const SolveTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s;
     const int32_t f = sym.sfirst[s];
@@ -421,8 +424,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_gemv(const SolveTask* __restrict__ tasks, DevSym sym,
                                                   const T* __restrict__ L,
                                                   const double* __restrict__ x,
-                                                  double* __restrict__ tout) {
+                                                  double* __restrict__ tout, RhsStride rs) {
     __shared__ double xr[kMaxGemvRows];
+    x += blockIdx.y * rs.v; tout += blockIdx.y * rs.v;
     const SolveTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s;
     const int32_t f = sym.sfirst[s];
@@ -638,10 +642,11 @@ __global__ __launch_bounds__(256) void k_bwd_level(const int32_t* __restrict__ l
                                                    const int8_t* __restrict__ big, const T* __restrict__ L,
                                                    const T* __restrict__ Dinv, const T* __restrict__ X,
                                                    const double* __restrict__ y, const double* __restrict__ tin,
-                                                   double* __restrict__ x) {
+                                                   double* __restrict__ x, RhsStride rs) {
     __shared__ double xs[kMaxLdsCols];
     __shared__ double xr[kMaxLdsRows];
     __shared__ double red[4][64];
+    y += blockIdx.y * rs.v; tin += blockIdx.y * rs.v; x += blockIdx.y * rs.v;
     const int32_t ntb = (ntiny + 3) / 4;
     const int32_t bid = blockIdx.x;
     if (bid < ntb) {
@@ -658,8 +663,10 @@ __global__ __launch_bounds__(256) void k_bwd_level(const int32_t* __restrict__ l
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
                                                   const T* __restrict__ L, const T* __restrict__ Dinv,
-                                                  const double* __restrict__ y, double* __restrict__ x) {
+                                                  const double* __restrict__ y, double* __restrict__ x,
+                                                  RhsStride rs) {
     __shared__ double xw[4][64];
+    y += blockIdx.y * rs.v; x += blockIdx.y * rs.v;
     const int wave = threadIdx.x >> 6;
     const int32_t idx = blockIdx.x * 4 + wave;
     if (idx < count) bwd_tiny<T>(list[idx], sym, L, Dinv, y, x, xw[wave]);
@@ -670,49 +677,56 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int32_t* __restrict__ li
 template <typename T>
 void launch_fwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
                       DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* b,
-                      double* y, double* W, hipStream_t st) {
+                      double* y, double* W, int nrhs, RhsStride rs, hipStream_t st) {
     const int32_t nrest = count - ntiny;
     if (nrest == 0 && nx == 0) {
-        if (ntiny > 0) help::launch_kernel(k_fwd_tiny<T>, (ntiny + 3) / 4, 256, 0, st, list, ntiny, sym, L, Dinv, b, y, W);
+        if (ntiny > 0)
+            help::launch_kernel(k_fwd_tiny<T>, dim3((ntiny + 3) / 4, nrhs), 256, 0, st, list, ntiny, sym, L, Dinv, b,
+                                y, W, rs);
         return;
     }
     const int32_t grid = (ntiny + 3) / 4 + nrest + nx;
-    help::launch_kernel(k_fwd_level<T>, grid, 256, 0, st, list, ntiny, nrest, xt, sym, big, L, Dinv, X, b, y, W);
+    help::launch_kernel(k_fwd_level<T>, dim3(grid, nrhs), 256, 0, st, list, ntiny, nrest, xt, sym, big, L, Dinv, X,
+                        b, y, W, rs);
 }
 template <typename T>
 void launch_bwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
                       DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* y,
-                      const double* t, double* x, hipStream_t st) {
+                      const double* t, double* x, int nrhs, RhsStride rs, hipStream_t st) {
     const int32_t nrest = count - ntiny;
     if (nrest == 0 && nx == 0) {
-        if (ntiny > 0) help::launch_kernel(k_bwd_tiny<T>, (ntiny + 3) / 4, 256, 0, st, list, ntiny, sym, L, Dinv, y, x);
+        if (ntiny > 0)
+            help::launch_kernel(k_bwd_tiny<T>, dim3((ntiny + 3) / 4, nrhs), 256, 0, st, list, ntiny, sym, L, Dinv, y,
+                                x, rs);
         return;
     }
     const int32_t grid = (ntiny + 3) / 4 + nrest + nx;
-    help::launch_kernel(k_bwd_level<T>, grid, 256, 0, st, list, ntiny, nrest, xt, sym, big, L, Dinv, X, y, t, x);
+    help::launch_kernel(k_bwd_level<T>, dim3(grid, nrhs), 256, 0, st, list, ntiny, nrest, xt, sym, big, L, Dinv, X,
+                        y, t, x, rs);
 }
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* y,
-                     double* W, hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_fwd_gemv<T>, count, 256, 0, st, tasks, sym, L, y, W);
+                     double* W, int nrhs, RhsStride rs, hipStream_t st) {
+    if (count > 0) help::launch_kernel(k_fwd_gemv<T>, dim3(count, nrhs), 256, 0, st, tasks, sym, L, y, W, rs);
 }
 template <typename T>
 void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* x,
-                     double* t, hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_bwd_gemv<T>, count, 256, 0, st, tasks, sym, L, x, t);
+                     double* t, int nrhs, RhsStride rs, hipStream_t st) {
+    if (count > 0) help::launch_kernel(k_bwd_gemv<T>, dim3(count, nrhs), 256, 0, st, tasks, sym, L, x, t, rs);
 }
 
 #define HELP_INST(T)                                                                                   \
     template void launch_fwd_level<T>(const int32_t*, int32_t, int32_t, const SolveTask*, int32_t,     \
                                       DevSym, const int8_t*, const T*, const T*, const T*,              \
-                                      const double*, double*, double*, hipStream_t);                    \
+                                      const double*, double*, double*, int, RhsStride, hipStream_t);    \
     template void launch_bwd_level<T>(const int32_t*, int32_t, int32_t, const SolveTask*, int32_t,     \
                                       DevSym, const int8_t*, const T*, const T*, const T*,              \
-                                      const double*, const double*, double*, hipStream_t);              \
+                                      const double*, const double*, double*, int, RhsStride,            \
+                                      hipStream_t);                                                     \
     template void launch_fwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,       \
-                                     double*, hipStream_t);                                            \
+                                     double*, int, RhsStride, hipStream_t);                             \
     template void launch_bwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,       \
-                                     double*, hipStream_t);
+                                     double*, int, RhsStride, hipStream_t);
 HELP_INST(double)
 HELP_INST(float)
 #undef HELP_INST

@@ -123,6 +123,19 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1,
                    const double* b2, double* x1, double* x2,
                    help_solve_stats* st);
 
+/* Two SolveLinearEquation calls (UseCholmod.cpp:157-238) with the same d
+ * in one pass: (b1a, b2a) -> (x1a, x2a) and (b1b, b2b) -> (x1b, x2b).  In
+ * the plain FP64 direct mode (refine_fp64 = 0) both right-hand sides go
+ * through each triangular-solve launch together, with results identical to
+ * two help_gpu_solve calls; the other modes make two help_gpu_solve calls.
+ * HELP's driver calls the backend once per solve, so the reference has no
+ * counterpart: this serves drivers that know two right-hand sides at once
+ * (help_hsd_solve_gpu's bottom and predictor solves). */
+int help_gpu_solve2(help_gpu* h, const double* d,
+                    const double* b1a, const double* b2a, double* x1a, double* x2a,
+                    const double* b1b, const double* b2b, double* x1b, double* x2b,
+                    help_solve_stats* sta, help_solve_stats* stb);
+
 /* Text of the last error on this handle ("" when none). */
 const char* help_gpu_last_error(const help_gpu* h);
 /* Text of the last error of help_gpu_create when it returned no handle. */
@@ -355,6 +368,11 @@ int help_gpu_backend_kernel_stats(const help_backend* be, int32_t kind, double* 
  * residual ||r - (A D^-1 A^T + beta I) x2|| / ||r||. */
 int help_gpu_backend_solve_stats(const help_backend* be, int64_t* n_solve, int64_t* refine_steps,
                                  double* max_rel_residual);
+/* help_gpu_solve2 on the backend's live handle, counted as two solves in
+ * help_gpu_backend_solve_stats. */
+int help_gpu_backend_solve2(help_backend* be, const double* d,
+                            const double* b1a, const double* b2a, double* x1a, double* x2a,
+                            const double* b1b, const double* b2b, double* x1b, double* x2b);
 /* PCG iterations summed over the backend's solves (PCG modes) */
 int64_t help_gpu_backend_pcg_iterations(const help_backend* be);
 /* HELP_MODE_PCG_LEARNED: the network every handle of this backend starts from */

@@ -227,6 +227,41 @@ def test_solve_with_explicit_inverse(mode, inverse):
     be.destruct()
 
 
+@pytest.mark.parametrize("mode,inverse", [(L.MODE_DIRECT_FP64, 1), (L.MODE_DIRECT_FP64, -1),
+                                          (L.MODE_DIRECT_FP32, 1)])
+def test_solve2_matches_two_solves(mode, inverse):
+    """help_gpu_solve2 (two right-hand sides through each triangular-solve
+    launch in the FP64 mode, two single solves otherwise) returns exactly the
+    two single solves' results, on an instance with wide supernodes (explicit
+    inverse and block substitution) and many tiny ones; the first solution
+    also within 1e-10 of the oracle."""
+    inst = instances.mcf(side=60, commodities=4, pairs=10, seed=7)
+    be = LinearEquation(mode=mode, inverse_solve=inverse)
+    be.construct(inst)
+    rng = np.random.default_rng(21)
+    d = log_uniform(rng, inst.n, -2, 2)
+    b1a, b1b = rng.standard_normal(inst.n), rng.standard_normal(inst.n)
+    b2a, b2b = rng.standard_normal(inst.m), rng.standard_normal(inst.m)
+    be.renew(d)
+    for rep in range(2):   # the second pass reuses the update buffers left by the first
+        (x1a, x2a), (x1b, x2b) = be.solve2(d, b1a, b2a, b1b, b2b)
+        y1a, y2a = be.solve(d, b1a, b2a)
+        y1b, y2b = be.solve(d, b1b, b2b)
+        _flush(f"solve2 mode={mode} inverse={inverse} pass {rep}: max |diff| "
+               f"{max(np.abs(x2a - y2a).max(), np.abs(x2b - y2b).max()):.3e}")
+        for u, v in ((x1a, y1a), (x2a, y2a), (x1b, y1b), (x2b, y2b)):
+            assert np.array_equal(u, v)
+        b1a, b1b = b1b, b1a   # swap the roles for the second pass
+        b2a, b2b = b2b, b2a
+    perm, _, _ = be.symbolic_arrays()
+    Lp, Li, Lx = _oracle_factor(inst, perm, d)
+    o1, o2 = O.solve_kkt(inst, perm, Lp, Li, Lx, d, b1a, b2a)
+    e2 = np.linalg.norm(x2b - o2) / np.linalg.norm(o2)
+    _flush(f"  dy err {e2:.3e}")
+    assert e2 <= 1e-10
+    be.destruct()
+
+
 @pytest.mark.parametrize("mode,device", [(L.MODE_DIRECT_FP64, False), (L.MODE_DIRECT_FP32, False),
                                          (L.MODE_DIRECT_FP64, True), (L.MODE_DIRECT_FP32, True)])
 def test_hsd_end_to_end(cfg0, mode, device):
