@@ -134,6 +134,10 @@ struct HsdGpu {
     int backend_error = 0;
     double t_renew = 0, t_solve = 0, t_newton = 0, t_direction = 0, t_step = 0;
     int n_renew = 0, n_solve = 0;
+    static constexpr int kOutSlots = 128;
+    double* hout = nullptr;   // pinned host copy of `out`
+    int used = 0;             // pending slots of `out`
+    std::vector<std::pair<int, double>> empty_slots;
 
     struct Timer {
         double* acc;
@@ -146,22 +150,42 @@ struct HsdGpu {
     void map(int64_t k, F f) {
         if (k > 0) help::launch_kernel(k_map<F>, (unsigned)((k + 255) / 256), 256, 0, st, k, f);
     }
-    // returns {sum_0..sum_{K-1}, min of output 0}
+    // Deferred reductions: reduce_late launches the reduction into its own
+    // slot of `out` and returns the slot; flush() brings every pending slot
+    // to pinned host memory with one copy and one synchronization, after
+    // which val(slot, k) reads it (k < K: sum of output k; k = K: minimum of
+    // output 0).  Independent reductions thus share one host round trip.
     template <int K, class F>
-    std::array<double, K + 1> reduce(int64_t k, F f) {
-        std::array<double, K + 1> r;
-        for (int i = 0; i < K; ++i) r[i] = 0.0;
-        r[K] = HUGE_VAL;
-        if (k <= 0) return r;
+    int reduce_late(int64_t k, F f) {
+        const int slot = used;
+        if (used + K + 1 > kOutSlots) throw std::runtime_error("hsd_gpu: too many pending reductions");
+        used += K + 1;
+        if (k <= 0) {   // identities, written into the host copy by flush()
+            for (int i = 0; i <= K; ++i) empty_slots.push_back({slot + i, i < K ? 0.0 : HUGE_VAL});
+            return slot;
+        }
         int nb = (int)std::min<int64_t>(kRedBlocks, (k + 255) / 256);
         help::launch_kernel(k_reduce<K, F>, nb, 256, 0, st, k, f, part.p);
-        help::launch_kernel(k_reduce_final<K>, 1, 256, 0, st, part.p, nb, out.p);
-        HIPCHK(hipMemcpyAsync(r.data(), out.p, sizeof(double) * (K + 1), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        return r;
+        help::launch_kernel(k_reduce_final<K>, 1, 256, 0, st, part.p, nb, out.p + slot);
+        return slot;
     }
+    int dot_late(int64_t k, const double* a, const double* b) {
+        return reduce_late<1>(k, [=] __device__(int64_t i, double* v) { v[0] = a[i] * b[i]; });
+    }
+    void flush() {
+        if (used > 0) {
+            HIPCHK(hipMemcpyAsync(hout, out.p, sizeof(double) * used, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        for (const auto& e : empty_slots) hout[e.first] = e.second;
+        empty_slots.clear();
+        used = 0;
+    }
+    double val(int slot, int k = 0) const { return hout[slot + k]; }
     double dot(int64_t k, const double* a, const double* b) {
-        return reduce<1>(k, [=] __device__(int64_t i, double* v) { v[0] = a[i] * b[i]; })[0];
+        const int slot = dot_late(k, a, b);
+        flush();
+        return val(slot);
     }
 
     explicit HsdGpu(const help_lp* l, const help_backend* b, help_gpu* hh) : lp(l), be(b), h(hh) {
@@ -184,9 +208,15 @@ struct HsdGpu {
         tmpm.alloc(std::max(m, 1));
         tmpn.alloc(std::max(n, 1));
         part.alloc((size_t)kRedBlocks * 8);
-        out.alloc(8);
+        out.alloc(kOutSlots);
+        HIPCHK(hipHostMalloc((void**)&hout, sizeof(double) * kOutSlots, hipHostMallocDefault));
         HIPCHK(hipStreamSynchronize(st));
     }
+    ~HsdGpu() {
+        if (hout) (void)hipHostFree(hout);
+    }
+    HsdGpu(const HsdGpu&) = delete;
+    HsdGpu& operator=(const HsdGpu&) = delete;
 
     void renew() {
         Timer t(&t_renew);
@@ -210,7 +240,10 @@ struct HsdGpu {
         if (s && !backend_error) backend_error = s;
     }
 
-    // HSDSolver.cpp:53-150
+    // HSDSolver.cpp:53-150.  Every reduction is launched first and read
+    // after one flush; the D updates and residual products that follow a
+    // failed positivity check in the reference are harmless here, since the
+    // driver stops on that status without reading D, r_p or r_d.
     void newton_parameters() {
         Timer tm(&t_newton);
         double* Dp = D.p; double* Dup = D_u.p;
@@ -218,31 +251,21 @@ struct HsdGpu {
         const double *cp = c_.p, *bp = b_.p, *upp = u_.p;
         const double tau_ = tau, rho_ = rho;
         const int nlb = n_lb, nub = n_ub, nfr = n_fr;
-        double xz_min = tau * kappa, xz_sum = xz_min;
         map(n_lb, [=] __device__(int64_t i) { Dp[i] = zp[i] / xp[i]; });
-        auto r1 = reduce<1>(n_lb, [=] __device__(int64_t i, double* v) { v[0] = zp[i] * xp[i]; });
-        auto rd = reduce<1>(n_lb, [=] __device__(int64_t i, double* v) { v[0] = Dp[i]; });
-        if (n_lb > 0 && rd[1] <= 0.0) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_NEGATIVE; return; }
-        xz_min = std::min(xz_min, r1[1]);
-        xz_sum += r1[0];
+        const int o_r1 = reduce_late<1>(n_lb, [=] __device__(int64_t i, double* v) { v[0] = zp[i] * xp[i]; });
+        const int o_rd = reduce_late<1>(n_lb, [=] __device__(int64_t i, double* v) { v[0] = Dp[i]; });
+        int o_ru = -1, o_rdu = -1, o_tmp = -1, o_rp2u = -1, o_rd2u = -1;
         if (n_ub > 0) {
             map(n_ub, [=] __device__(int64_t i) { Dup[i] = zup[i] / xup[i]; Dp[i] += Dup[i]; });
-            auto ru = reduce<1>(n_ub, [=] __device__(int64_t i, double* v) { v[0] = zup[i] * xup[i]; });
-            auto rdu = reduce<1>(n_ub, [=] __device__(int64_t i, double* v) { v[0] = Dup[i]; });
-            if (rdu[1] <= 0.0) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_NEGATIVE; return; }
-            xz_min = std::min(xz_min, ru[1]);
-            xz_sum += ru[0];
+            o_ru = reduce_late<1>(n_ub, [=] __device__(int64_t i, double* v) { v[0] = zup[i] * xup[i]; });
+            o_rdu = reduce_late<1>(n_ub, [=] __device__(int64_t i, double* v) { v[0] = Dup[i]; });
         }
-        double tmp = 0;
         if (n_fr > 0) {
             map(n_fr, [=] __device__(int64_t i) { Dp[nlb + i] = rho_ * xfp[i]; });
-            tmp = reduce<1>(n_fr, [=] __device__(int64_t i, double* v) {
-                      v[0] = xfp[i] * xp[nlb + i] * xp[nlb + i];
-                  })[0];
+            o_tmp = reduce_late<1>(n_fr, [=] __device__(int64_t i, double* v) {
+                v[0] = xfp[i] * xp[nlb + i] * xp[nlb + i];
+            });
         }
-        D_g = kappa / tau + rho * tmp / (tau * tau);
-        mu = xz_sum / (1.0 + n_lb + n_ub);
-        rfval = (1.0 + n_lb + n_ub) * xz_min / xz_sum;
         // r_p = tau b - A x  (A x by the row-split product: heavy rows in chunks)
         double* rpp = r_p.p;
         const double* ax = tmpm.p;
@@ -259,29 +282,52 @@ struct HsdGpu {
             if (j < nub) acc += yup[j];
             rdp[j] = col_dot_acc(cptr, crow, cval, yp, j, acc);
         });
-        double cTx = dot(n_lb + n_fr, cp, xp);
-        double bTy = dot(m, bp, yp) + dot(n_ub, upp, yup);
+        const int o_ctx = dot_late(n_lb + n_fr, cp, xp);
+        const int o_by = dot_late(m, bp, yp), o_uyu = dot_late(n_ub, upp, yup);
+        const int o_rp2 = dot_late(m, rpp, rpp);
+        if (n_ub > 0)
+            o_rp2u = reduce_late<1>(n_ub, [=] __device__(int64_t i, double* v) {
+                double t = tau_ * upp[i] - xup[i] - xp[i];
+                v[0] = t * t;
+            });
+        const int o_rd2 = dot_late(n_lb + n_fr, rdp, rdp);
+        if (n_ub > 0)
+            o_rd2u = reduce_late<1>(n_ub, [=] __device__(int64_t i, double* v) {
+                double t = zup[i] + yup[i];
+                v[0] = t * t;
+            });
+        const int o_xx = dot_late(n, xp, xp);
+        const int o_zz = dot_late(n_lb, zp, zp), o_zuzu = dot_late(n_ub, zup, zup);
+        const int o_yuyu = dot_late(n_ub, yup, yup), o_yy = dot_late(m, yp, yp);
+        flush();
+        double xz_min = tau * kappa, xz_sum = xz_min;
+        if (n_lb > 0 && val(o_rd, 1) <= 0.0) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_NEGATIVE; return; }
+        xz_min = std::min(xz_min, val(o_r1, 1));
+        xz_sum += val(o_r1);
+        if (n_ub > 0) {
+            if (val(o_rdu, 1) <= 0.0) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_NEGATIVE; return; }
+            xz_min = std::min(xz_min, val(o_ru, 1));
+            xz_sum += val(o_ru);
+        }
+        double tmp = 0;
+        if (n_fr > 0) tmp = val(o_tmp);
+        D_g = kappa / tau + rho * tmp / (tau * tau);
+        mu = xz_sum / (1.0 + n_lb + n_ub);
+        rfval = (1.0 + n_lb + n_ub) * xz_min / xz_sum;
+        double cTx = val(o_ctx);
+        double bTy = val(o_by) + val(o_uyu);
         r_g = kappa + cTx - bTy;
-        double rp2 = dot(m, rpp, rpp);
-        if (n_ub > 0)
-            rp2 += reduce<1>(n_ub, [=] __device__(int64_t i, double* v) {
-                       double t = tau_ * upp[i] - xup[i] - xp[i];
-                       v[0] = t * t;
-                   })[0];
-        double rd2 = dot(n_lb + n_fr, rdp, rdp);
-        if (n_ub > 0)
-            rd2 += reduce<1>(n_ub, [=] __device__(int64_t i, double* v) {
-                       double t = zup[i] + yup[i];
-                       v[0] = t * t;
-                   })[0];
+        double rp2 = val(o_rp2);
+        if (n_ub > 0) rp2 += val(o_rp2u);
+        double rd2 = val(o_rd2);
+        if (n_ub > 0) rd2 += val(o_rd2u);
         double rg2 = r_g * r_g;
         sum_r_norm = rp2 + rd2 + rg2;
         pobj = lp->c_intercept + cTx / tau;
         dobj = lp->c_intercept + bTy / tau;
         rel_gap = std::fabs(pobj - dobj) / (1.0 + std::fabs(dobj));
-        pinf = std::sqrt(rp2) / (tau + std::sqrt(dot(n, xp, xp)));
-        dinf = std::sqrt(rd2) / (tau + std::sqrt(dot(n_lb, zp, zp) + dot(n_ub, zup, zup) + dot(n_ub, yup, yup) +
-                                                  dot(m, yp, yp)));
+        pinf = std::sqrt(rp2) / (tau + std::sqrt(val(o_xx)));
+        dinf = std::sqrt(rd2) / (tau + std::sqrt(val(o_zz) + val(o_zuzu) + val(o_yuyu) + val(o_yy)));
         infe = tau / (kappa * inf0);
     }
 
@@ -306,18 +352,28 @@ struct HsdGpu {
         });
     }
 
-    // HSDSolver.cpp:212-220
-    double mix_dots(const double* p, const double* q, const double* q_u) {
-        double r = -dot(n_lb + n_fr, c_.p, p);
+    // HSDSolver.cpp:212-220: mix_late launches the reductions, mix_value
+    // combines them after a flush
+    struct Mix { int cp, fr, bq, uq; };
+    Mix mix_late(const double* p, const double* q, const double* q_u) {
+        Mix r;
+        r.cp = dot_late(n_lb + n_fr, c_.p, p);
+        r.fr = -1;
         if (n_fr > 0) {
             const double* xfp = xf.p; const double* xp = x.p;
             const int nlb = n_lb;
-            double s = reduce<1>(n_fr, [=] __device__(int64_t i, double* v) {
-                           v[0] = xfp[i] * xp[i + nlb] * p[i + nlb];
-                       })[0];
-            r -= (rho / tau) * s;
+            r.fr = reduce_late<1>(n_fr, [=] __device__(int64_t i, double* v) {
+                v[0] = xfp[i] * xp[i + nlb] * p[i + nlb];
+            });
         }
-        r += dot(m, b_.p, q) + dot(n_ub, u_.p, q_u);
+        r.bq = dot_late(m, b_.p, q);
+        r.uq = dot_late(n_ub, u_.p, q_u);
+        return r;
+    }
+    double mix_value(const Mix& x) const {
+        double r = -val(x.cp);
+        if (x.fr >= 0) r -= (rho / tau) * val(x.fr);
+        r += val(x.bq) + val(x.uq);
         return r;
     }
 
@@ -334,9 +390,9 @@ struct HsdGpu {
         map(n_ub, [=] __device__(int64_t i) { b1u[i] = 0; b2u[i] = -upp[i]; });
         augmented_rhs(b1, b1u, b2, b2u, sle1b.p, sle2b.p);
     }
-    double bottom_finish() {
+    Mix bottom_finish() {
         augmented_finish(bot1u.p, bot2u.p, pp.p, ppu.p, qq.p, qqu.p);
-        return mix_dots(pp.p, qq.p, qqu.p);
+        return mix_late(pp.p, qq.p, qqu.p);
     }
 
     // The bottom system and the first (affine) direction share D, and the
@@ -346,10 +402,10 @@ struct HsdGpu {
     int bottom_and_direction(double eta, double& DOT, double* hrxz, double* hrxuzu, double hrtk, double* odx,
                              double* odxu, double& odtau, double* ody, double* odyu, double* odz, double* odzu,
                              double& odkappa, double& rpn, double& rdn, double& rgn) {
-        const double UP = direction_rhs(eta, hrxz, hrxuzu, hrtk);
+        const int o_up = direction_rhs(eta, hrxz, hrxuzu);
         solve2(sle1b.p, sle2b.p, pp.p, qq.p, sle1.p, sle2.p, uu.p, vv.p);
-        DOT = bottom_finish();
-        return direction_finish(eta, DOT, UP, hrxz, hrxuzu, hrtk, odx, odxu, odtau, ody, odyu, odz, odzu,
+        const Mix bot = bottom_finish();
+        return direction_finish(eta, &bot, DOT, o_up, hrxz, hrxuzu, hrtk, odx, odxu, odtau, ody, odyu, odz, odzu,
                                 odkappa, rpn, rdn, rgn);
     }
 
@@ -357,13 +413,14 @@ struct HsdGpu {
     int direction(double eta, double DOT, double* hrxz, double* hrxuzu, double hrtk, double* odx, double* odxu,
                   double& odtau, double* ody, double* odyu, double* odz, double* odzu, double& odkappa,
                   double& rpn, double& rdn, double& rgn) {
-        const double UP = direction_rhs(eta, hrxz, hrxuzu, hrtk);
+        const int o_up = direction_rhs(eta, hrxz, hrxuzu);
         solve(sle1.p, sle2.p, uu.p, vv.p);
-        return direction_finish(eta, DOT, UP, hrxz, hrxuzu, hrtk, odx, odxu, odtau, ody, odyu, odz, odzu,
-                                odkappa, rpn, rdn, rgn);
+        return direction_finish(eta, nullptr, DOT, o_up, hrxz, hrxuzu, hrtk, odx, odxu, odtau, ody, odyu, odz,
+                                odzu, odkappa, rpn, rdn, rgn);
     }
-    // the direction's right-hand side into (sle1, sle2); returns UP
-    double direction_rhs(double eta, double* hrxz, double* hrxuzu, double hrtk) {
+    // the direction's right-hand side into (sle1, sle2); returns the slot of
+    // the reduction that completes UP
+    int direction_rhs(double eta, double* hrxz, double* hrxuzu) {
         Timer tm(&t_direction);
         double *d1 = dir1.p, *d1u = dir1u.p, *d2 = dir2.p, *d2u = dir2u.p;
         const double *rdp = r_d.p, *rpp = r_p.p, *xp = x.p, *zp = z.p, *xup = xu.p, *zup = zu.p,
@@ -395,17 +452,18 @@ struct HsdGpu {
             if (xp[i] < zp[i]) d2u[i] -= hrxz[i];
             if (xup[i] < zup[i]) d2u[i] -= hrxuzu[i];
         });
-        double UP = eta * r_g + hrtk / tau;
-        UP += reduce<1>(n_lb, [=] __device__(int64_t i, double* v) {
-                  v[0] = xp[i] < zp[i] ? cp[i] * hrxz[i] : 0.0;
-              })[0];
+        const int o_up = reduce_late<1>(n_lb, [=] __device__(int64_t i, double* v) {
+            v[0] = xp[i] < zp[i] ? cp[i] * hrxz[i] : 0.0;
+        });
         augmented_rhs(d1, d1u, d2, d2u, sle1.p, sle2.p);
-        return UP;
+        return o_up;
     }
-    // the rest of the direction from the solution (uu, vv) of that system
-    int direction_finish(double eta, double DOT, double UP, double* hrxz, double* hrxuzu, double hrtk,
-                         double* odx, double* odxu, double& odtau, double* ody, double* odyu, double* odz,
-                         double* odzu, double& odkappa, double& rpn, double& rdn, double& rgn) {
+    // the rest of the direction from the solution (uu, vv) of that system;
+    // with `bot`, DOT is first completed from the bottom system's pending
+    // reductions (same flush as UP and the direction's own mix)
+    int direction_finish(double eta, const Mix* bot, double& DOT, int o_up, double* hrxz, double* hrxuzu,
+                         double hrtk, double* odx, double* odxu, double& odtau, double* ody, double* odyu,
+                         double* odz, double* odzu, double& odkappa, double& rpn, double& rdn, double& rgn) {
         Timer tm(&t_direction);
         double *d1 = dir1.p, *d1u = dir1u.p, *d2 = dir2.p, *d2u = dir2u.p;
         const double *rdp = r_d.p, *rpp = r_p.p, *xp = x.p, *zp = z.p, *xup = xu.p, *zup = zu.p,
@@ -413,10 +471,15 @@ struct HsdGpu {
         const double tau_ = tau, rho_ = rho;
         const int nlb = n_lb, nfr = n_fr;
         const int64_t* cptr = colptr; const int32_t* crow = rowind; const double* cval = cscval;
-        double DOWN = D_g - DOT;
         augmented_finish(d1u, d2u, uu.p, uuu.p, vv.p, vvu.p);
+        const Mix mu_v = mix_late(uu.p, vv.p, vvu.p);
+        flush();
+        if (bot) DOT = mix_value(*bot);
+        double UP = eta * r_g + hrtk / tau;
+        UP += val(o_up);
+        double DOWN = D_g - DOT;
         if (DOWN <= 1e-13) odtau = 0;
-        else odtau = (UP - mix_dots(uu.p, vv.p, vvu.p)) / std::max(D_g, DOWN);
+        else odtau = (UP - mix_value(mu_v)) / std::max(D_g, DOWN);
         const double dt = odtau;
         const double *up_ = uu.p, *pp_ = pp.p, *uuu_ = uuu.p, *ppu_ = ppu.p, *vv_ = vv.p, *qq_ = qq.p,
                      *vvu_ = vvu.p, *qqu_ = qqu.p;
@@ -459,32 +522,42 @@ struct HsdGpu {
             d1u[i] = eta * (zup[i] + yup[i]) + odyu[i] + odzu[i];
             d2u[i] = eta * (tau_ * upp[i] - xp[i] - xup[i]) + upp[i] * dt - odxu[i] - odx[i];
         });
-        rpn = dot(m, d2, d2) + dot(n_ub, d2u, d2u);
-        rdn = dot(n, d1, d1) + dot(n_ub, d1u, d1u);
-        double tmp = 0;
+        const int o_d2 = dot_late(m, d2, d2), o_d2u = dot_late(n_ub, d2u, d2u);
+        const int o_d1 = dot_late(n, d1, d1), o_d1u = dot_late(n_ub, d1u, d1u);
+        int o_tmp = -1;
         if (n_fr > 0)
-            tmp = reduce<1>(n_fr, [=] __device__(int64_t i, double* v) {
-                      v[0] = xfp[i] * xp[nlb + i] * xp[nlb + i];
-                  })[0];
-        rgn = eta * r_g + odkappa - mix_dots(odx, ody, odyu) - rho * tmp * dt / (tau * tau);
+            o_tmp = reduce_late<1>(n_fr, [=] __device__(int64_t i, double* v) {
+                v[0] = xfp[i] * xp[nlb + i] * xp[nlb + i];
+            });
+        const Mix mo = mix_late(odx, ody, odyu);
+        flush();
+        rpn = val(o_d2) + val(o_d2u);
+        rdn = val(o_d1) + val(o_d1u);
+        double tmp = 0;
+        if (n_fr > 0) tmp = val(o_tmp);
+        rgn = eta * r_g + odkappa - mix_value(mo) - rho * tmp * dt / (tau * tau);
         rgn = rgn * rgn;
         if (rpn + rdn <= 1e-10 || rpn + rdn + rgn <= sum_r_norm * 0.1) return 1;
         return 0;
     }
 
     // HSDSolver.cpp:478-488: the smallest ratio -v/dv below 1 over dv < -tol
-    double stepsize(const double* v, const double* vu, const double* dv, const double* dvu) {
-        double r = 1.0;
-        auto a = reduce<1>(n_lb, [=] __device__(int64_t i, double* o) {
+    // (stepsize_late launches the reductions, stepsize_value reads them)
+    std::pair<int, int> stepsize_late(const double* v, const double* vu, const double* dv, const double* dvu) {
+        const int a = reduce_late<1>(n_lb, [=] __device__(int64_t i, double* o) {
             o[0] = (dv[i] < -kVarTol && v[i] + dv[i] < 0.0) ? -v[i] / dv[i] : HUGE_VAL;
         });
-        r = std::min(r, a[1]);
-        if (n_ub > 0) {
-            auto b = reduce<1>(n_ub, [=] __device__(int64_t i, double* o) {
+        int b = -1;
+        if (n_ub > 0)
+            b = reduce_late<1>(n_ub, [=] __device__(int64_t i, double* o) {
                 o[0] = (dvu[i] < -kVarTol && vu[i] + dvu[i] < 0.0) ? -vu[i] / dvu[i] : HUGE_VAL;
             });
-            r = std::min(r, b[1]);
-        }
+        return {a, b};
+    }
+    double stepsize_value(std::pair<int, int> o) const {
+        double r = 1.0;
+        r = std::min(r, val(o.first, 1));
+        if (o.second >= 0) r = std::min(r, val(o.second, 1));
         return r;
     }
 
@@ -493,18 +566,22 @@ struct HsdGpu {
         double xz_min = (tau + at * dtau) * (kappa + ak * dkappa);
         double xz_sum = xz_min;
         const double *xp = x.p, *dxp = dx.p, *zp = z.p, *dzp = dz.p;
-        auto a = reduce<1>(n_lb, [=] __device__(int64_t i, double* o) {
+        const int a = reduce_late<1>(n_lb, [=] __device__(int64_t i, double* o) {
             o[0] = (xp[i] + ap * dxp[i]) * (zp[i] + ad * dzp[i]);
         });
-        xz_min = std::min(xz_min, a[1]);
-        xz_sum += a[0];
+        int b = -1;
         if (n_ub > 0) {
             const double *xup = xu.p, *dxup = dxu.p, *zup = zu.p, *dzup = dzu.p;
-            auto b = reduce<1>(n_ub, [=] __device__(int64_t i, double* o) {
+            b = reduce_late<1>(n_ub, [=] __device__(int64_t i, double* o) {
                 o[0] = (xup[i] + ap * dxup[i]) * (zup[i] + ad * dzup[i]);
             });
-            xz_min = std::min(xz_min, b[1]);
-            xz_sum += b[0];
+        }
+        flush();
+        xz_min = std::min(xz_min, val(a, 1));
+        xz_sum += val(a);
+        if (b >= 0) {
+            xz_min = std::min(xz_min, val(b, 1));
+            xz_sum += val(b);
         }
         double mu_ = xz_sum / (1.0 + n_lb + n_ub);
         return xz_min / mu_ >= beta;
@@ -512,8 +589,11 @@ struct HsdGpu {
 
     void get_stepsize(double& ps, double& ds) {
         Timer tm(&t_step);
-        ps = stepsize(x.p, xu.p, dx.p, dxu.p);
-        ds = stepsize(z.p, zu.p, dz.p, dzu.p);
+        const auto op = stepsize_late(x.p, xu.p, dx.p, dxu.p);
+        const auto od = stepsize_late(z.p, zu.p, dz.p, dzu.p);
+        flush();
+        ps = stepsize_value(op);
+        ds = stepsize_value(od);
         if (dtau < -kVarTol) { ps = std::min(ps, -tau / dtau); ds = std::min(ds, -tau / dtau); }
         if (dkappa < -kVarTol) { ps = std::min(ps, -kappa / dkappa); ds = std::min(ds, -kappa / dkappa); }
     }
@@ -583,7 +663,9 @@ struct HsdGpu {
     }
 
     double dmu_of(double nall) {
-        return std::max((dot(n_lb, dx.p, dz.p) + dot(n_ub, dxu.p, dzu.p) + dtau * dkappa) / nall, 0.0);
+        const int a = dot_late(n_lb, dx.p, dz.p), b = dot_late(n_ub, dxu.p, dzu.p);
+        flush();
+        return std::max((val(a) + val(b) + dtau * dkappa) / nall, 0.0);
     }
 
     // HSDSolver.cpp:596-717
@@ -656,19 +738,27 @@ struct HsdGpu {
             int newton_ok = bottom_and_direction(eta, DOT, hxz.p, hxuzu.p, htk, dx.p, dxu.p, dtau, dy.p, dyu.p,
                                                  dz.p, dzu.p, dkappa, rpn, rdn, rgn);
             double p_step = 0, d_step = 0, gamma;
+            int o_dmu[2] = {-1, -1};
             if (newton_ok) {
                 get_stepsize(p_step, d_step);
                 const double *dxp = dx.p, *zp = z.p, *dzp = dz.p;
-                double new_mu = (tau + p_step * dtau) * (kappa + d_step * dkappa);
-                new_mu += reduce<1>(n_lb, [=] __device__(int64_t i, double* o) {
-                              o[0] = (xp[i] + p_step * dxp[i]) * (zp[i] + d_step * dzp[i]);
-                          })[0];
+                const int a = reduce_late<1>(n_lb, [=] __device__(int64_t i, double* o) {
+                    o[0] = (xp[i] + p_step * dxp[i]) * (zp[i] + d_step * dzp[i]);
+                });
+                int b = -1;
                 if (n_ub > 0) {
                     const double *xup = xu.p, *dxup = dxu.p, *zup = zu.p, *dzup = dzu.p;
-                    new_mu += reduce<1>(n_ub, [=] __device__(int64_t i, double* o) {
-                                  o[0] = (xup[i] + p_step * dxup[i]) * (zup[i] + d_step * dzup[i]);
-                              })[0];
+                    b = reduce_late<1>(n_ub, [=] __device__(int64_t i, double* o) {
+                        o[0] = (xup[i] + p_step * dxup[i]) * (zup[i] + d_step * dzup[i]);
+                    });
                 }
+                // the dmu dot products (read below) share this flush
+                o_dmu[0] = dot_late(n_lb, dx.p, dz.p);
+                o_dmu[1] = dot_late(n_ub, dxu.p, dzu.p);
+                flush();
+                double new_mu = (tau + p_step * dtau) * (kappa + d_step * dkappa);
+                new_mu += val(a);
+                if (b >= 0) new_mu += val(b);
                 new_mu /= nall;
                 gamma = new_mu / mu;
                 gamma = std::min(gamma * gamma * gamma, 0.2 * std::min(gamma, 1.0));
@@ -682,7 +772,7 @@ struct HsdGpu {
             }
             eta = 1.0 - gamma;
             double alpha_p = gamma * mu;
-            double dmu = dmu_of(nall);
+            double dmu = std::max((val(o_dmu[0]) + val(o_dmu[1]) + dtau * dkappa) / nall, 0.0);   // dmu_of
             set_hat(alpha_p, phi, dmu, true);
             double dtau_b, dkappa_b, rpb, rdb, rgb;
             int imp_ok = direction(eta, DOT, hxz.p, hxuzu.p, htk, dx_b.p, dxu_b.p, dtau_b, dy_b.p, dyu_b.p,
