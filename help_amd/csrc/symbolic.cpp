@@ -240,15 +240,33 @@ std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
         for (int32_t i : Lp) bucket_remove(i);
         // --- |L_e \ L_p| for every element adjacent to L_p ---
         wflg += (int64_t)n + 2;
-        for (int32_t i : Lp)
+        const size_t nLp = Lp.size();
+        // the lists of L_p's variables are scattered over the heap and the
+        // two scans below are bound by the latency of reaching them: fetch
+        // the list headers four variables ahead and their first lines two
+        // ahead
+        auto prefetch_lists = [&](size_t x) {
+            if (x + 4 < nLp) { __builtin_prefetch(&E[Lp[x + 4]]); __builtin_prefetch(&Av[Lp[x + 4]]); }
+            if (x + 2 < nLp) {
+                const int32_t* pe = E[Lp[x + 2]].data();
+                const int32_t* pa = Av[Lp[x + 2]].data();
+                __builtin_prefetch(pe); __builtin_prefetch(pa); __builtin_prefetch(pa + 16);
+            }
+        };
+        for (size_t x = 0; x < nLp; ++x) {
+            prefetch_lists(x);
+            const int32_t i = Lp[x];
             for (int32_t e : E[i]) {
                 if (status[e] != ELEM) continue;
                 if (w[e] < wflg) w[e] = wflg + edeg[e];
                 w[e] -= nv[i];
             }
+        }
         // --- prune lists, external degrees, mass elimination, hashes ---
         Lp_live.clear();
-        for (int32_t i : Lp) {
+        for (size_t x = 0; x < nLp; ++x) {
+            prefetch_lists(x);
+            const int32_t i = Lp[x];
             int64_t ext = 0;
             uint64_t h = 0;
             size_t k = 0;
