@@ -280,6 +280,7 @@ void build_schedules(help_gpu* h) {
         if ((int32_t)h->ea.size() > off) h->fops.push_back({OP_EA, off, (int32_t)h->ea.size() - off, w});
         // the panel of every active front
         int32_t po = (int32_t)h->potrf.size(), to = (int32_t)h->trsm.size(), go = (int32_t)h->gemm.size();
+        const int32_t pa = (int32_t)h->panel.size();
         double wp = 0, wt = 0, wg = 0;
         for (int32_t s : at_active[p]) {
             const int32_t nc = ncol(s), c = S.snrow[s];
@@ -287,8 +288,10 @@ void build_schedules(help_gpu* h) {
             const int32_t nb = std::min(kTile, nc - b);
             h->potrf.push_back({s, b, 0});
             wp += (double)nb * nb * nb / 3.0;
+            if (b + nb >= c) h->panel.push_back({s, b, 0});
             for (int32_t r0 = b + nb, rb = 1; r0 < c; r0 += kTile, ++rb) {
                 h->trsm.push_back({s, b, rb});
+                h->panel.push_back({s, b, rb});
                 wt += (double)nb * nb * std::min(kTile, c - r0);
             }
             for (int32_t c0 = b + nb; c0 < nc; c0 += kTile)
@@ -312,8 +315,21 @@ void build_schedules(help_gpu* h) {
                     }
             }
         }
-        h->fops.push_back({OP_POTRF, po, (int32_t)h->potrf.size() - po, wp});
-        if ((int32_t)h->trsm.size() > to) h->fops.push_back({OP_TRSM, to, (int32_t)h->trsm.size() - to, wt});
+        // FP64: one fused launch for the phase's panels when its workgroups
+        // (one per row block, 104 KB of LDS each) fit the chip at once
+        const int32_t npa = (int32_t)h->panel.size() - pa;
+        const bool fuse = npa <= kFusedPanelMax;
+        Op op{OP_POTRF, po, (int32_t)h->potrf.size() - po, wp};
+        if (fuse) {
+            op.xoff = pa; op.xcount = npa; op.work2 = wp + wt;
+            for (int32_t s : at_active[p]) h->pcopy.push_back({s, (p - start[s]) * kTile, 0});
+        }
+        h->fops.push_back(op);
+        if ((int32_t)h->trsm.size() > to) {
+            Op ot{OP_TRSM, to, (int32_t)h->trsm.size() - to, wt};
+            ot.lds = fuse ? 1 : 0;
+            h->fops.push_back(ot);
+        }
         if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
         // U -= L21 L21^T with the whole panel as inner dimension, for the
         // fronts whose last panel is this phase's
@@ -495,6 +511,8 @@ void setup_device_factor(help_gpu* h) {
     h->d_ea.upload(h->ea, st);
     h->d_earange.upload(h->earange, st);
     h->d_potrf.upload(h->potrf, st);
+    h->d_panel.upload(h->panel, st);
+    h->d_pcopy.upload(h->pcopy, st);
     h->d_trsm.upload(h->trsm, st);
     h->d_gemm.upload(h->gemm, st);
     h->d_fgemv.upload(h->fgemv, st);
@@ -526,6 +544,7 @@ void setup_device_factor(help_gpu* h) {
         h->d_U64.alloc(std::max<int64_t>(1, nu));
         if (h->use_uacc) h->d_Uacc64.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv64.alloc(std::max<int64_t>(1, nd));
+        if (!h->pcopy.empty()) h->d_Lstage64.alloc(std::max<int64_t>(1, nd));
         if (h->use_xinv) {
             h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
             h->d_S64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
@@ -582,16 +601,34 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
             side_used = true;
         }
         forked = side;
-        ProfScope ps(h, o.kind, o.work);
+        const bool fused = std::is_same<T, double>::value &&
+                           ((o.kind == OP_POTRF && o.xcount > 0) || (o.kind == OP_TRSM && o.lds == 1));
+        if (fused && o.kind == OP_TRSM) continue;   // done by the fused panel launch
+        ProfScope ps(h, o.kind, fused ? o.work2 : o.work);
         switch (o.kind) {
             case OP_SMALL: launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st); break;
             case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->d_earange.p, h->sym, L, U, st); break;
-            case OP_POTRF: launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, Dinv, h->d_err.p, st); break;
+            case OP_POTRF:
+                if constexpr (std::is_same<T, double>::value) {
+                    if (fused) {
+                        launch_panel_fp64(h->d_panel.p + o.xoff, o.xcount, h->sym, L, Dinv, h->d_Lstage64.p,
+                                          h->d_err.p, st);
+                        break;
+                    }
+                }
+                launch_potrf_diag<T>(h->d_potrf.p + o.off, o.count, h->sym, L, Dinv, h->d_err.p, st);
+                break;
             case OP_TRSM: launch_trsm_rows<T>(h->d_trsm.p + o.off, o.count, h->sym, L, Dinv, st); break;
             case OP_GEMM: launch_gemm<T>(h->d_gemm.p + o.off, o.count, h->sym, L, U, uacc_store<T>(h), st); break;
             case OP_XINV: launch_xinv<T>(h->d_xtask.p + o.off, o.count, h->sym, L, Dinv, xinv_store<T>(h), xinv_partial<T>(h), h->st_side); break;
             default: break;
         }
+    }
+    if constexpr (std::is_same<T, double>::value) {
+        // the fused panels' L11 blocks into L (nothing reads them from L
+        // during the factorization)
+        if (!h->pcopy.empty())
+            launch_copy_diag_fp64(h->d_pcopy.p, (int32_t)h->pcopy.size(), h->sym, h->d_Lstage64.p, L, st);
     }
     if (side_used) {
         HIPCHK(hipEventRecord(h->ev_side[1], h->st_side));
@@ -730,6 +767,7 @@ void ensure_fp64_factor(help_gpu* h) {
     h->d_U64.alloc(std::max<int64_t>(1, S.uoff[S.nsuper]));
     if (h->use_uacc) h->d_Uacc64.alloc(std::max<int64_t>(1, S.uoff[S.nsuper]));
     h->d_Dinv64.alloc(std::max<int64_t>(1, h->dinvoff[S.nsuper]));
+    if (!h->pcopy.empty()) h->d_Lstage64.alloc(std::max<int64_t>(1, h->dinvoff[S.nsuper]));
     if (h->use_xinv) {
         h->d_X64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
         h->d_S64.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));

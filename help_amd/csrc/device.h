@@ -72,6 +72,8 @@ struct EaTask {
 };
 
 constexpr int kTile = 64;      // GEMM tile and panel width
+// most workgroups of a fused FP64 panel launch (one per CU: 104 KB of LDS)
+constexpr int kFusedPanelMax = 240;
 constexpr int kSmallRows = 256;
 constexpr int kSmallElems = 4032;  // panel elements of a small front; with the 64x65
                                    // inverse scratch at most 8192 (64 KB FP64) in LDS
@@ -133,6 +135,16 @@ void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges
 template <typename T>
 void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv,
                        int32_t* err, hipStream_t st);
+// FP64: the diagonal block and one row block per task (PanelTask.rb >= 1;
+// rb = 0 for a panel with no rows below), each task factoring the block
+// redundantly; the task with rb <= 1 stores Dinv and L11, the latter into
+// Lstage (Dinv layout), because the other tasks of the launch may still be
+// reading the unfactored block from L
+void launch_panel_fp64(const PanelTask* tasks, int32_t count, DevSym sym, double* L, double* Dinv,
+                       double* Lstage, int32_t* err, hipStream_t st);
+// L11 blocks (s, b) from Lstage into L
+void launch_copy_diag_fp64(const PanelTask* tasks, int32_t count, DevSym sym, const double* Lstage,
+                           double* L, hipStream_t st);
 // panel rows below the diagonal block times the stored inverse (MFMA)
 template <typename T>
 void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, const T* Dinv,

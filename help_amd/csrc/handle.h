@@ -65,7 +65,10 @@ struct Op {
     double work;       // algorithmic flops (factor classes) or bytes (the others)
     int32_t lds = 0;   // dynamic LDS elements per workgroup (small fronts)
     int32_t ntiny = 0; // solves: leading supernodes of the list solved one per wave
-    int32_t xoff = 0, xcount = 0;   // solves: the level's wide-supernode block tasks
+    int32_t xoff = 0, xcount = 0;   // solves: the level's wide-supernode block tasks;
+                                    // FP64 factor: the phase's fused panel tasks
+                                    // (POTRF; its TRSM op then has lds = 1)
+    double work2 = 0;  // flops of the fused panel launch
 };
 
 inline const char* const kKindName[HELP_K_COUNT] = {
@@ -110,7 +113,8 @@ struct help_gpu {
     std::vector<int32_t> list_small, list_fwd, list_bwd;
     std::vector<EaTask> ea;
     std::vector<int32_t> earange;
-    std::vector<PanelTask> potrf, trsm;
+    std::vector<PanelTask> potrf, trsm, panel;
+    std::vector<PanelTask> pcopy;   // diagonal blocks factored by fused panel launches
     std::vector<GemmTask> gemm;
     std::vector<SolveTask> fgemv, bgemv;
     // explicit inverse of L11 for supernodes wider than one 64-column block
@@ -137,6 +141,7 @@ struct help_gpu {
     DBuf<double> d_val, d_csrval, d_d, d_dinv, d_b1, d_b2, d_x1, d_x2, d_rp, d_W, d_t, d_tn, d_r,
         d_c, d_z, d_p, d_q, d_part, d_scal, d_slotvals;
     DBuf<double> d_L64, d_U64, d_Dinv64, d_Uacc64;
+    DBuf<double> d_Lstage64;   // L11 of the fused panels (Dinv layout) until the final copy
     // FP32 mode: symmetric diagonal scaling of the factored matrix
     std::vector<int32_t> slot_row;
     DBuf<int32_t> d_slot_row, d_slot_col;
@@ -145,7 +150,7 @@ struct help_gpu {
     DBuf<int8_t> d_big;
     DBuf<EaTask> d_ea;
     DBuf<int32_t> d_earange;
-    DBuf<PanelTask> d_potrf, d_trsm;
+    DBuf<PanelTask> d_potrf, d_trsm, d_panel, d_pcopy;
     DBuf<GemmTask> d_gemm;
     DBuf<SolveTask> d_fgemv, d_bgemv, d_fxs, d_bxs;
     DBuf<XTask> d_xtask;

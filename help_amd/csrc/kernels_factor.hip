@@ -547,6 +547,12 @@ __device__ inline void mfma_tile(const T* As, const T* Bs, int kc, int lane, int
 //    columns) and stores its part of column block kb+1 to S; barrier.
 // Two barriers per 4 pivots instead of per-pivot publishing, and the
 // trailing updates run on the matrix cores.
+// kTrsm: the task also carries row block rb - 1 below the diagonal block
+// (every task of a panel factors the block redundantly; the task with
+// rb <= 1 writes L11, Dinv and the pivot count).  Its 64 rows are loaded
+// before the factorization and solved at the end as k_trsm_gemm does,
+// X * Dinv^T on MFMA with Dinv taken from LDS: the same products and
+// order, so the same values, without a launch of its own.
 __device__ inline double read_lane(double v, int l) {
     const long long b = __builtin_bit_cast(long long, v);
     const int lo = __builtin_amdgcn_readlane((int)b, l);
@@ -554,9 +560,10 @@ __device__ inline double read_lane(double v, int l) {
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
 
+template <bool kTrsm>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_potrf_mfma(const PanelTask* __restrict__ tasks, DevSym sym, double* __restrict__ L,
-                  double* __restrict__ Dinv, int32_t* __restrict__ err) {
+                  double* __restrict__ Dinv, double* __restrict__ Lstage, int32_t* __restrict__ err) {
     typedef double T;
     typedef Mfma<double>::acc_t acc_t;
     constexpr int kS = kTile + 1;
@@ -575,6 +582,21 @@ void k_potrf_mfma(const PanelTask* __restrict__ tasks, DevSym sym, double* __res
     T* Db = Dinv + dinv_block_offset(sym, s, b);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int lc = lane & 15, lk = lane >> 4;
+    const bool owner = !kTrsm || tk.rb <= 1;
+    const bool rows = kTrsm && tk.rb >= 1;
+    const int32_t r0 = b + nb + (tk.rb - 1) * kTile;
+    T* Lp = L + sym.loff[s];
+    // the row block's entries (as k_trsm_gemm loads them), in flight
+    // during the factorization
+    T va[kTile * kTile / 256];
+    if (rows) {
+        const int32_t ri = min(r0 + lane, c - 1);
+#pragma unroll
+        for (int q = 0; q < kTile * kTile / 256; ++q) {
+            const int kc = min(wave + 4 * q, nb - 1);
+            va[q] = Lp[(int64_t)(b + kc) * c + ri];
+        }
+    }
     {
         // lower triangle, identity outside nb, zero above; clamped loads
         const int r = lane, rr = min(r, nb - 1);
@@ -675,7 +697,7 @@ void k_potrf_mfma(const PanelTask* __restrict__ tasks, DevSym sym, double* __res
         }
         __syncthreads();
     }
-    if (t == 0 && nbad) atomicAdd(err, nbad);
+    if (owner && t == 0 && nbad) atomicAdd(err, nbad);
     if (t < kTile) sdv[t] = sqrt(pivs[t]);
     __syncthreads();
     // L11[r][k] = raw_r(step k) / d_k * sqrt(d_k); Dinv[r][k] = X[r][k] / sqrt(d_r)
@@ -688,9 +710,57 @@ void k_potrf_mfma(const PanelTask* __restrict__ tasks, DevSym sym, double* __res
         const T f = (r > k) ? rps[k] * sk : T(0);
         T v = Uf[k * kTile + r] * f;
         v = (r == k) ? sk : v;
-        if (r < nb && k < nb) P[(int64_t)k * c + r] = v;
+        if (kTrsm) {
+            if (owner && r < nb && k < nb) Lstage[dinv_block_offset(sym, s, b) + k * nb + r] = v;
+        } else {
+            if (r < nb && k < nb) P[(int64_t)k * c + r] = v;
+        }
         const T xv = XF[k * kTile + r] * ((r >= k) ? isd : T(0));
-        if (r < nb && k < nb) Db[k * nb + r] = xv;
+        if (owner && r < nb && k < nb) Db[k * nb + r] = xv;
+    }
+    if (!rows) return;   // uniform over the workgroup
+    // row block: As[k][i] = X[i][k] in S's storage, Bs[k][j] = Dinv[j][k]
+    // (the values just written to Db) in Uf's storage
+    __syncthreads();
+    T* As = S;
+    T* Bs = Uf;
+#pragma unroll
+    for (int q = 0; q < kTile * kTile / 256; ++q) {
+        const int k = wave + 4 * q;
+        As[k * kTile + lane] = (k < nb && r0 + lane < c) ? va[q] : T(0);
+        const T xv = XF[k * kTile + r] * ((r >= k) ? isd : T(0));
+        Bs[k * kTile + lane] = (k < nb && lane < nb) ? xv : T(0);
+    }
+    __syncthreads();
+    acc_t tacc[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < 2; ++q) tacc[a][q] = acc_t{0, 0, 0, 0};
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    mfma_tile<T>(As, Bs, kTile, lane, wr, wc, tacc);
+    for (int a = 0; a < 2; ++a)
+        for (int q = 0; q < 2; ++q)
+            for (int rr = 0; rr < 4; ++rr) {
+                const int32_t i = wr + a * 16 + Mfma<T>::row(lane, rr);
+                const int32_t j = wc + q * 16 + (lane & 15);
+                if (r0 + i >= c || j >= nb) continue;
+                Lp[(int64_t)(b + j) * c + r0 + i] = tacc[a][q][rr];
+            }
+}
+
+// L11 block (s, b), nb x nb column-major at the Dinv offset of the block,
+// back into the front's panel
+__global__ __launch_bounds__(256) void k_copy_diag(const PanelTask* __restrict__ tasks, DevSym sym,
+                                                   const double* __restrict__ Lstage, double* __restrict__ L) {
+    const PanelTask tk = tasks[blockIdx.x];
+    const int32_t s = tk.s, b = tk.b;
+    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
+    const int32_t c = sym.snrow[s];
+    const int nb = min(kTile, n - b);
+    const double* src = Lstage + dinv_block_offset(sym, s, b);
+    double* P = L + sym.loff[s] + (int64_t)b * c + b;
+    for (int idx = threadIdx.x; idx < nb * nb; idx += 256) {
+        const int k = idx / nb, r = idx - k * nb;
+        P[(int64_t)k * c + r] = src[idx];
     }
 }
 
@@ -1033,9 +1103,18 @@ void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, 
     if (count <= 0) return;
     // FP64: the blocked MFMA form; FP32 keeps the two-wave register-blocked form
     if constexpr (std::is_same<T, double>::value)
-        help::launch_kernel(k_potrf_mfma, count, 256, 0, st, tasks, sym, L, Dinv, err);
+        help::launch_kernel(k_potrf_mfma<false>, count, 256, 0, st, tasks, sym, L, Dinv, (double*)nullptr, err);
     else
         help::launch_kernel(k_potrf_diag<T>, count, 128, 0, st, tasks, sym, L, Dinv, err);
+}
+void launch_panel_fp64(const PanelTask* tasks, int32_t count, DevSym sym, double* L, double* Dinv,
+                       double* Lstage, int32_t* err, hipStream_t st) {
+    if (count > 0)
+        help::launch_kernel(k_potrf_mfma<true>, count, 256, 0, st, tasks, sym, L, Dinv, Lstage, err);
+}
+void launch_copy_diag_fp64(const PanelTask* tasks, int32_t count, DevSym sym, const double* Lstage,
+                           double* L, hipStream_t st) {
+    if (count > 0) help::launch_kernel(k_copy_diag, count, 256, 0, st, tasks, sym, Lstage, L);
 }
 template <typename T>
 void launch_trsm_rows(const PanelTask* tasks, int32_t count, DevSym sym, T* L, const T* Dinv,
