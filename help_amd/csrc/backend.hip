@@ -243,17 +243,39 @@ void build_schedules(help_gpu* h) {
     }
     for (int32_t p = 0; p < nphase; ++p) {
         if (!at_small[p].empty()) {
-            int32_t off = (int32_t)h->list_small.size();
-            double w = 0;
-            int32_t lds = 0;
-            for (int32_t s : at_small[p]) {
+            // fronts of at most 64 rows and 16 columns whose panel and
+            // inverse scratch fit kSmallWaveLds take one wave each, the others
+            // a workgroup; only in phases with enough such fronts to fill the
+            // chip (a lone one-wave front walks its columns serially per lane
+            // and is slower than the workgroup form)
+            auto wave_ok = [&](int32_t s) {
                 const int32_t nc = ncol(s), c = S.snrow[s];
-                h->list_small.push_back(s);
-                w += sn_flops(s);
                 const int32_t nbm = std::min(kTile, nc);
-                lds = std::max(lds, c * nc + nbm * (nbm + 1));
+                return c <= 64 && nc <= 16 && c * nc + nbm * (nbm + 1) <= kSmallWaveLds;
+            };
+            int32_t nwave = 0;
+            for (int32_t s : at_small[p]) nwave += wave_ok(s) ? 1 : 0;
+            const bool use_wave = nwave >= 2048;
+            for (int pass = 0; pass < 2; ++pass) {
+                int32_t off = (int32_t)h->list_small.size();
+                double w = 0;
+                int32_t lds = 0;
+                for (int32_t s : at_small[p]) {
+                    const int32_t nc = ncol(s), c = S.snrow[s];
+                    const int32_t nbm = std::min(kTile, nc);
+                    const int32_t need = c * nc + nbm * (nbm + 1);
+                    const bool one_wave = use_wave && wave_ok(s);
+                    if (one_wave != (pass == 0)) continue;
+                    h->list_small.push_back(s);
+                    w += sn_flops(s);
+                    lds = std::max(lds, need);
+                }
+                if ((int32_t)h->list_small.size() > off) {
+                    Op o{OP_SMALL, off, (int32_t)h->list_small.size() - off, w, lds};
+                    o.wave = pass == 0 ? 1 : 0;
+                    h->fops.push_back(o);
+                }
             }
-            h->fops.push_back({OP_SMALL, off, (int32_t)h->list_small.size() - off, w, lds});
         }
         if (at_active[p].empty()) continue;
         // extend-add by kEaCols-column chunks of each starting front
@@ -606,7 +628,12 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
         if (fused && o.kind == OP_TRSM) continue;   // done by the fused panel launch
         ProfScope ps(h, o.kind, fused ? o.work2 : o.work);
         switch (o.kind) {
-            case OP_SMALL: launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st); break;
+            case OP_SMALL:
+                if (o.wave)
+                    launch_small_wave<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st);
+                else
+                    launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st);
+                break;
             case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->d_earange.p, h->sym, L, U, st); break;
             case OP_POTRF:
                 if constexpr (std::is_same<T, double>::value) {

@@ -41,6 +41,13 @@ __device__ inline T safe_sqrt_pivot(T d, int32_t* err, bool count) {
     return sqrt(d);
 }
 
+// orders the LDS accesses of one wave (no other wave takes part)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline int64_t dinv_block_offset(const DevSym& sym, int32_t s, int32_t b) {
     return sym.dinvoff[s] + (int64_t)(b / kTile) * kTile * kTile;
 }
@@ -193,6 +200,144 @@ __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict_
     }
 }
 
+// ------------------------------------------------------------ small, wave
+// k_small_fronts for fronts of at most 64 rows: one wave per front (a row
+// per lane), four fronts per workgroup, each in its own lds_elems slice of
+// the dynamic LDS.  Every entry goes through the same operations in the same
+// order as in k_small_fronts (extend-add child by child, right-looking
+// columns, the inverse by row operations, U -= L21 L21^T), with wave-level
+// ordering in place of the workgroup barriers, so the results are the same.
+__device__ inline void wave_fence_all() {
+    __threadfence_block();   // global read-modify-writes of one child before the next child's
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ list, int32_t count,
+                                                    int32_t lds_elems, DevSym sym, T* __restrict__ L,
+                                                    T* __restrict__ U, T* __restrict__ Dinv,
+                                                    int32_t* __restrict__ err) {
+    extern __shared__ unsigned char smem_raw[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t task = blockIdx.x * 4 + wave;
+    if (task >= count) return;   // the waves are independent: no barrier below
+    T* F = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * lds_elems;
+    const int32_t s = list[task];
+    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
+    const int32_t c = sym.snrow[s];
+    const int32_t cu = c - n;
+    T* Lp = L + sym.loff[s];
+    T* Up = U + sym.uoff[s];
+    const int32_t cn = c * n;
+    for (int32_t idx = lane; idx < cn; idx += 64) F[idx] = Lp[idx];
+    wave_sync();
+    for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
+        const int32_t ch = sym.child[t];
+        const int32_t chn = sym.sfirst[ch + 1] - sym.sfirst[ch];
+        const int32_t chc = sym.snrow[ch] - chn;
+        const int32_t* rel = sym.relmap + sym.reloff[ch];
+        const T* Uc = U + sym.uoff[ch];
+        const int64_t tot = (int64_t)chc * chc;
+        for (int64_t idx0 = lane; idx0 < tot; idx0 += 4 * 64) {
+            bool ok[4];
+            int32_t rr[4], cc[4];
+            T v[4], old[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t idx = idx0 + (int64_t)q * 64;
+                const int64_t id = idx < tot ? idx : 0;
+                const int32_t jj = (int32_t)(id / chc), ii = (int32_t)(id - (int64_t)jj * chc);
+                ok[q] = idx < tot && ii >= jj;
+                rr[q] = rel[ii];
+                cc[q] = rel[jj];
+                v[q] = Uc[id];
+            }
+            if (cu > 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool up = ok[q] && cc[q] >= n;
+                    old[q] = Up[up ? (int64_t)(cc[q] - n) * cu + (rr[q] - n) : 0];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (!ok[q]) continue;
+                if (cc[q] < n) F[cc[q] * c + rr[q]] += v[q];
+                else Up[(int64_t)(cc[q] - n) * cu + (rr[q] - n)] = old[q] + v[q];
+            }
+        }
+        wave_fence_all();
+    }
+    // dense partial Cholesky of the panel, right-looking by columns
+    for (int32_t j = 0; j < n; ++j) {
+        const T p = safe_sqrt_pivot<T>(F[j * c + j], err, lane == 0);
+        wave_sync();
+        if (lane == 0) F[j * c + j] = p;
+        if (lane > j && lane < c) F[j * c + lane] /= p;
+        wave_sync();
+        if (lane > j && lane < c) {
+            for (int32_t k = j + 1; k < n && k <= lane; ++k)
+                F[k * c + lane] -= F[j * c + lane] * F[j * c + k];
+        }
+        wave_sync();
+    }
+    for (int32_t idx = lane; idx < cn; idx += 64) Lp[idx] = F[idx];
+    // inverse of the (single, n <= 64) diagonal block, as invert_lower_block
+    {
+        const int32_t nb = n, ldx = nb + 1;
+        T* X = F + cn;
+        T* Db = Dinv + dinv_block_offset(sym, s, 0);
+        for (int idx = lane; idx < nb * nb; idx += 64) {
+            int j = idx / nb, i = idx - j * nb;
+            X[j * ldx + i] = (i == j) ? T(1) : T(0);
+        }
+        wave_sync();
+        for (int j = 0; j < nb; ++j) {
+            const T djj = F[j * c + j];
+            if (lane <= j) X[lane * ldx + j] /= djj;
+            wave_sync();
+            const int cnt = (nb - j - 1) * (j + 1);
+            for (int idx = lane; idx < cnt; idx += 64) {
+                int k = idx / (nb - j - 1), i = j + 1 + idx - k * (nb - j - 1);
+                X[k * ldx + i] -= F[j * c + i] * X[k * ldx + j];
+            }
+            wave_sync();
+        }
+        for (int idx = lane; idx < nb * nb; idx += 64) {
+            int j = idx / nb, i = idx - j * nb;
+            Db[j * nb + i] = (i >= j) ? X[j * ldx + i] : T(0);
+        }
+    }
+    // U -= L21 L21^T, four entries per lane with their loads first
+    const int32_t tot = cu * cu;
+    for (int32_t idx0 = lane; idx0 < tot; idx0 += 4 * 64) {
+        bool ok[4];
+        int32_t pos[4];
+        T old[4], acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t idx = idx0 + q * 64;
+            const int32_t id = idx < tot ? idx : 0;
+            const int32_t jj = id / cu, ii = id - jj * cu;
+            ok[q] = idx < tot && ii >= jj;
+            pos[q] = id;
+            old[q] = Up[id];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            T a = 0;
+            if (ok[q]) {
+                const int32_t jj = pos[q] / cu, ii = pos[q] - jj * cu;
+                for (int32_t j = 0; j < n; ++j) a += F[j * c + n + ii] * F[j * c + n + jj];
+            }
+            acc[q] = a;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ok[q]) Up[pos[q]] = old[q] - acc[q];
+    }
+}
+
 // ------------------------------------------------------------ extend-add
 // Task = kEaCols columns [c0, c1) of a front.  For each child in order, the
 // child columns that land in the range are spread over the 4 waves (two
@@ -313,12 +458,6 @@ __device__ void factor_invert_block(const T* __restrict__ src, int64_t lds, int3
         if (i >= j) Ldst[(int64_t)j * lds + i] = (i == j) ? sdj : A[j * ld + i] * sdj;
         Db[j * nb + i] = (i >= j) ? X[j * ld + i] / sqrt(A[i * ld + i]) : T(0);
     }
-}
-
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Diagonal block of a panel: LDL^T of the nb x nb block (identity padding to
@@ -1089,6 +1228,13 @@ void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T
         help::launch_kernel(k_small_fronts<T>, count, 256, (size_t)lds_elems * sizeof(T), st, list, sym, L, U, Dinv, err);
 }
 template <typename T>
+void launch_small_wave(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, T* Dinv, int32_t* err,
+                       int32_t lds_elems, hipStream_t st) {
+    if (count > 0)
+        help::launch_kernel(k_small_wave<T>, (count + 3) / 4, 256, (size_t)4 * lds_elems * sizeof(T), st, list,
+                            count, lds_elems, sym, L, U, Dinv, err);
+}
+template <typename T>
 void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges, DevSym sym, T* L, T* U,
                        hipStream_t st) {
     if (count > 0) help::launch_kernel(k_extend_add<T>, count, 256, 0, st, tasks, ranges, sym, L, U);
@@ -1130,6 +1276,8 @@ void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, cons
 #define HELP_INST(T)                                                                                \
     template void launch_small_fronts<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,      \
                                          int32_t, hipStream_t);                                     \
+    template void launch_small_wave<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,        \
+                                       int32_t, hipStream_t);                                       \
     template void launch_extend_add<T>(const EaTask*, int32_t, const int32_t*, DevSym, T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, T*, hipStream_t);           \
     template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, T*, int32_t*,        \
