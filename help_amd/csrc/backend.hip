@@ -169,29 +169,6 @@ void build_schedules(help_gpu* h) {
         }
         return b;
     };
-    // inverse blocks of the given wide supernodes, right-looking: the launch
-    // for step t finalises X_kj with k = j + t and adds its terms to the
-    // partial sums of the blocks below
-    auto push_xinv = [&](const std::vector<int32_t>& sns) {
-        int32_t maxnb = 0;
-        for (int32_t s : sns) maxnb = std::max(maxnb, nblk_of(s));
-        for (int32_t step = 0; step < maxnb; ++step) {
-            int32_t xo = (int32_t)h->xtask.size();
-            double wx = 0;
-            for (int32_t s : sns) {
-                const int32_t nb = nblk_of(s);
-                for (int32_t j = 0; j + step < nb; ++j) {
-                    const int32_t k = j + step;
-                    for (int32_t i = k; i < nb; ++i) {
-                        if (step == 0 && i == j) continue;
-                        h->xtask.push_back({s, i, j, k});
-                        wx += 2.0 * kTile * kTile * kTile * ((step > 0 ? 1 : 0) + (i > k ? 1 : 0));
-                    }
-                }
-            }
-            if ((int32_t)h->xtask.size() > xo) h->fops.push_back({OP_XINV, xo, (int32_t)h->xtask.size() - xo, wx});
-        }
-    };
     // Phases, as soon as possible.  A small front is one SMALL launch; a
     // large front with P panels takes phases start .. start+P-1: extend-add
     // and panel 0 in the first, panel b in phase start+b, and the update
@@ -202,7 +179,7 @@ void build_schedules(help_gpu* h) {
     // Supernodes are postordered, so children come before their parent.
     // Within a phase the launches run SMALL, EA, POTRF, TRSM, GEMM (in-panel
     // trailing updates), GEMM (update matrices), then the side-stream XINV
-    // steps of the wide fronts finished in the phase.
+    // launches of the wide fronts (see below).
     const int32_t ns = S.nsuper;
     // A front whose update matrix is small takes its U -= L21 L21^T in the
     // GEMM launch of every panel (the accumulator parked in Uacc between
@@ -357,10 +334,8 @@ void build_schedules(help_gpu* h) {
         // fronts whose last panel is this phase's
         go = (int32_t)h->gemm.size();
         wg = 0;
-        std::vector<int32_t> wide_done;
         for (int32_t s : at_done[p]) {
             const int32_t nc = ncol(s), c = S.snrow[s];
-            if (wide(s)) wide_done.push_back(s);
             if (folded(s, npan[s])) continue;
             for (int32_t c0 = nc; c0 < c; c0 += kTile)
                 for (int32_t r0 = c0; r0 < c; r0 += kTile) {
@@ -369,7 +344,56 @@ void build_schedules(help_gpu* h) {
                 }
         }
         if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
-        push_xinv(wide_done);
+        // Inverse blocks of the wide fronts, right-looking: task (i, j, k)
+        // finalises X_kj (i = k) or adds L_ik X_kj to the partial sum of X_ij
+        // (i > k), and every partial sum takes its terms in increasing k.
+        // Fronts near the root, whose chain of nb launches after the last
+        // panel (about nb/3 phases' time) would outlast the phases left, get
+        // the tasks of panel k = p - start[s] as soon as that panel is
+        // factored; the others get all their tasks after their last panel,
+        // one launch per k - j: fewer launches and fewer forks.
+        auto early = [&](int32_t s) { return done[s] + nblk_of(s) / 3 >= nphase - 1; };
+        auto task_work = [](int32_t i, int32_t j, int32_t k) {
+            return 2.0 * kTile * kTile * kTile * ((k > j ? 1 : 0) + (i > k ? 1 : 0));
+        };
+        {
+            const int32_t xo = (int32_t)h->xtask.size();
+            double wx = 0;
+            for (int32_t s : at_active[p]) {
+                if (!wide(s) || !early(s)) continue;
+                const int32_t nb = nblk_of(s), k = p - start[s];
+                for (int32_t j = 0; j <= k; ++j)
+                    for (int32_t i = k; i < nb; ++i) {
+                        if (i == j) continue;   // X_kk is Dinv_k
+                        h->xtask.push_back({s, i, j, k});
+                        wx += task_work(i, j, k);
+                    }
+            }
+            if ((int32_t)h->xtask.size() > xo)
+                h->fops.push_back({OP_XINV, xo, (int32_t)h->xtask.size() - xo, wx});
+        }
+        std::vector<int32_t> wide_done;
+        for (int32_t s : at_done[p])
+            if (wide(s) && !early(s)) wide_done.push_back(s);
+        int32_t maxnb = 0;
+        for (int32_t s : wide_done) maxnb = std::max(maxnb, nblk_of(s));
+        for (int32_t step = 0; step < maxnb; ++step) {
+            const int32_t xo = (int32_t)h->xtask.size();
+            double wx = 0;
+            for (int32_t s : wide_done) {
+                const int32_t nb = nblk_of(s);
+                for (int32_t j = 0; j + step < nb; ++j) {
+                    const int32_t k = j + step;
+                    for (int32_t i = k; i < nb; ++i) {
+                        if (i == j) continue;
+                        h->xtask.push_back({s, i, j, k});
+                        wx += task_work(i, j, k);
+                    }
+                }
+            }
+            if ((int32_t)h->xtask.size() > xo)
+                h->fops.push_back({OP_XINV, xo, (int32_t)h->xtask.size() - xo, wx});
+        }
     }
     // triangular solves
     // algorithmic bytes of the solves: each stored factor entry is read once
