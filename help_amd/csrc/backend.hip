@@ -424,6 +424,15 @@ void build_schedules(help_gpu* h) {
         double nb = nblk_of(s), n = ncol(s);
         return wsz * (nb * (nb - 1) / 2 * kTile * kTile + n * kTile);
     };
+    // lanes per tiny supernode for a level of tiny supernodes only: 32 (two
+    // per wave) when every one has at most 32 rows, else 64
+    auto tiny_lanes = [&](const std::vector<int32_t>& lst, int32_t off, int32_t count, int32_t ntiny,
+                          int32_t nx) {
+        if (ntiny != count || nx != 0) return 64;
+        for (int32_t t = off; t < off + count; ++t)
+            if (S.snrow[lst[t]] > 32) return 64;
+        return 32;
+    };
     for (int32_t l = 0; l < S.nlevels; ++l) {
         int32_t off = (int32_t)h->list_fwd.size(), to = (int32_t)h->fgemv.size(), xo = (int32_t)h->fxs.size();
         double w = 0, wg = 0;
@@ -447,6 +456,7 @@ void build_schedules(help_gpu* h) {
         Op o{OP_FWD, off, (int32_t)h->list_fwd.size() - off, w, 0, ntiny};
         o.xoff = xo;
         o.xcount = (int32_t)h->fxs.size() - xo;
+        o.lds = tiny_lanes(h->list_fwd, off, o.count, ntiny, o.xcount);
         h->sops_fwd.push_back(o);
         if ((int32_t)h->fgemv.size() > to) h->sops_fwd.push_back({OP_FWD_GEMV, to, (int32_t)h->fgemv.size() - to, wg});
     }
@@ -472,6 +482,7 @@ void build_schedules(help_gpu* h) {
         Op o{OP_BWD, off, (int32_t)h->list_bwd.size() - off, w, 0, ntiny};
         o.xoff = xo;
         o.xcount = (int32_t)h->bxs.size() - xo;
+        o.lds = tiny_lanes(h->list_bwd, off, o.count, ntiny, o.xcount);
         h->sops_bwd.push_back(o);
     }
     // SpMV-class kernels: 12 bytes per nonzero (value + index) plus vectors
@@ -704,7 +715,7 @@ void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x, int nrhs = 
         ProfScope ps(h, o.kind, o.work * nrhs);
         if (o.kind == OP_FWD)
             launch_fwd_level<T>(h->d_list_fwd.p + o.off, o.count, o.ntiny, h->d_fxs.p + o.xoff, o.xcount, h->sym,
-                                h->d_big.p, L, Dinv, X, x, y, h->d_W.p, nrhs, rs, st);
+                                h->d_big.p, L, Dinv, X, x, y, h->d_W.p, nrhs, rs, st, o.lds);
         else
             launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, y, h->d_W.p, nrhs, rs, st);
     }
@@ -712,7 +723,7 @@ void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x, int nrhs = 
         ProfScope ps(h, o.kind, o.work * nrhs);
         if (o.kind == OP_BWD)
             launch_bwd_level<T>(h->d_list_bwd.p + o.off, o.count, o.ntiny, h->d_bxs.p + o.xoff, o.xcount, h->sym,
-                                h->d_big.p, L, Dinv, X, y, h->d_t.p, x, nrhs, rs, st);
+                                h->d_big.p, L, Dinv, X, y, h->d_t.p, x, nrhs, rs, st, o.lds);
         else
             launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, nrhs, rs, st);
     }

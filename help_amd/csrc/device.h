@@ -188,6 +188,8 @@ constexpr int kTinyRows = 64;      // supernodes with at most this many rows tak
 //            children's update vectors;
 //   backward x_s(block j) = sum_{i>=j} X_ij^T (y_s - t_s)(block i).
 // Supernodes with big[s] != 0 leave their L21 products to the gemv kernels.
+// tiny_lanes = 32 (a level of tiny supernodes only, all with at most 32
+// rows): two supernodes per wave, 32 lanes each, same per-lane arithmetic.
 // Every launch solves nrhs right-hand sides at once (blockIdx.y): right-hand
 // side r uses the vectors at offset r * rs.v and the update vectors at
 // offset r * rs.w, with the same operations as a single solve.
@@ -198,11 +200,11 @@ struct RhsStride {
 template <typename T>
 void launch_fwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
                       DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* b,
-                      double* y, double* W, int nrhs, RhsStride rs, hipStream_t st);
+                      double* y, double* W, int nrhs, RhsStride rs, hipStream_t st, int tiny_lanes = 64);
 template <typename T>
 void launch_bwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
                       DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* y,
-                      const double* t, double* x, int nrhs, RhsStride rs, hipStream_t st);
+                      const double* t, double* x, int nrhs, RhsStride rs, hipStream_t st, int tiny_lanes = 64);
 // Forward, big supernodes: W_s += L21 y_s by 64-row blocks.
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L,

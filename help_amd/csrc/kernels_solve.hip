@@ -112,11 +112,11 @@ __device__ inline void gather_child(double* g, int32_t lim, double* __restrict__
 // solve reads y and writes x (x may be b's storage).  Keeping input and
 // output apart lets the blocks of one wide supernode run in parallel
 // without reading entries another block is writing.
-template <typename T>
+template <typename T, int G>
 __device__ void fwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, const T* __restrict__ Dinv,
                          const double* __restrict__ b, double* __restrict__ y, double* __restrict__ W,
                          double* xw) {
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & (G - 1);   // G lanes per supernode (c <= G)
     const int32_t f = sym.sfirst[s];
     const int32_t n = sym.sfirst[s + 1] - f;
     const int32_t c = sym.snrow[s];
@@ -148,10 +148,10 @@ __device__ void fwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, 
     }
 }
 
-template <typename T>
+template <typename T, int G>
 __device__ void bwd_tiny(int32_t s, const DevSym& sym, const T* __restrict__ L, const T* __restrict__ Dinv,
                          const double* __restrict__ y, double* __restrict__ x, double* xw) {
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & (G - 1);   // G lanes per supernode (c <= G)
     const int32_t f = sym.sfirst[s];
     const int32_t n = sym.sfirst[s + 1] - f;
     const int32_t c = sym.snrow[s];
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(const int32_t* __restrict__ l
     if (bid < ntb) {
         const int wave = threadIdx.x >> 6;
         const int32_t idx = bid * 4 + wave;
-        if (idx < ntiny) fwd_tiny<T>(list[idx], sym, L, Dinv, b, y, W, red[wave]);
+        if (idx < ntiny) fwd_tiny<T, 64>(list[idx], sym, L, Dinv, b, y, W, red[wave]);
     } else if (bid < ntb + nrest) {
         fwd_sn<T>(list[ntiny + bid - ntb], sym, big, L, Dinv, b, y, W, xs, red);
     } else {
@@ -368,16 +368,16 @@ __global__ __launch_bounds__(256) void k_fwd_level(const int32_t* __restrict__ l
 
 // levels made of tiny supernodes only (the leaves: many of them, and a
 // kernel with a small register and LDS footprint)
-template <typename T>
+template <typename T, int G>
 __global__ __launch_bounds__(256) void k_fwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
                                                   const T* __restrict__ L, const T* __restrict__ Dinv,
                                                   const double* __restrict__ b, double* __restrict__ y,
                                                   double* __restrict__ W, RhsStride rs) {
-    __shared__ double xw[4][64];
+    __shared__ double xw[256 / G][G];
     b += blockIdx.y * rs.v; y += blockIdx.y * rs.v; W += blockIdx.y * rs.w;
-    const int wave = threadIdx.x >> 6;
-    const int32_t idx = blockIdx.x * 4 + wave;
-    if (idx < count) fwd_tiny<T>(list[idx], sym, L, Dinv, b, y, W, xw[wave]);
+    const int g = threadIdx.x / G;
+    const int32_t idx = blockIdx.x * (256 / G) + g;
+    if (idx < count) fwd_tiny<T, G>(list[idx], sym, L, Dinv, b, y, W, xw[g]);
 }
 
 template <typename T>
@@ -652,7 +652,7 @@ __global__ __launch_bounds__(256) void k_bwd_level(const int32_t* __restrict__ l
     if (bid < ntb) {
         const int wave = threadIdx.x >> 6;
         const int32_t idx = bid * 4 + wave;
-        if (idx < ntiny) bwd_tiny<T>(list[idx], sym, L, Dinv, y, x, red[wave]);
+        if (idx < ntiny) bwd_tiny<T, 64>(list[idx], sym, L, Dinv, y, x, red[wave]);
     } else if (bid < ntb + nrest) {
         bwd_sn<T>(list[ntiny + bid - ntb], sym, big, L, Dinv, y, tin, x, xs, xr, red);
     } else {
@@ -660,16 +660,16 @@ __global__ __launch_bounds__(256) void k_bwd_level(const int32_t* __restrict__ l
     }
 }
 
-template <typename T>
+template <typename T, int G>
 __global__ __launch_bounds__(256) void k_bwd_tiny(const int32_t* __restrict__ list, int32_t count, DevSym sym,
                                                   const T* __restrict__ L, const T* __restrict__ Dinv,
                                                   const double* __restrict__ y, double* __restrict__ x,
                                                   RhsStride rs) {
-    __shared__ double xw[4][64];
+    __shared__ double xw[256 / G][G];
     y += blockIdx.y * rs.v; x += blockIdx.y * rs.v;
-    const int wave = threadIdx.x >> 6;
-    const int32_t idx = blockIdx.x * 4 + wave;
-    if (idx < count) bwd_tiny<T>(list[idx], sym, L, Dinv, y, x, xw[wave]);
+    const int g = threadIdx.x / G;
+    const int32_t idx = blockIdx.x * (256 / G) + g;
+    if (idx < count) bwd_tiny<T, G>(list[idx], sym, L, Dinv, y, x, xw[g]);
 }
 
 }  // namespace
@@ -677,12 +677,15 @@ __global__ __launch_bounds__(256) void k_bwd_tiny(const int32_t* __restrict__ li
 template <typename T>
 void launch_fwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
                       DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* b,
-                      double* y, double* W, int nrhs, RhsStride rs, hipStream_t st) {
+                      double* y, double* W, int nrhs, RhsStride rs, hipStream_t st, int tiny_lanes) {
     const int32_t nrest = count - ntiny;
     if (nrest == 0 && nx == 0) {
-        if (ntiny > 0)
-            help::launch_kernel(k_fwd_tiny<T>, dim3((ntiny + 3) / 4, nrhs), 256, 0, st, list, ntiny, sym, L, Dinv, b,
-                                y, W, rs);
+        if (ntiny > 0 && tiny_lanes == 32)
+            help::launch_kernel(k_fwd_tiny<T, 32>, dim3((ntiny + 7) / 8, nrhs), 256, 0, st, list, ntiny, sym, L,
+                                Dinv, b, y, W, rs);
+        else if (ntiny > 0)
+            help::launch_kernel(k_fwd_tiny<T, 64>, dim3((ntiny + 3) / 4, nrhs), 256, 0, st, list, ntiny, sym, L,
+                                Dinv, b, y, W, rs);
         return;
     }
     const int32_t grid = (ntiny + 3) / 4 + nrest + nx;
@@ -692,12 +695,15 @@ void launch_fwd_level(const int32_t* list, int32_t count, int32_t ntiny, const S
 template <typename T>
 void launch_bwd_level(const int32_t* list, int32_t count, int32_t ntiny, const SolveTask* xt, int32_t nx,
                       DevSym sym, const int8_t* big, const T* L, const T* Dinv, const T* X, const double* y,
-                      const double* t, double* x, int nrhs, RhsStride rs, hipStream_t st) {
+                      const double* t, double* x, int nrhs, RhsStride rs, hipStream_t st, int tiny_lanes) {
     const int32_t nrest = count - ntiny;
     if (nrest == 0 && nx == 0) {
-        if (ntiny > 0)
-            help::launch_kernel(k_bwd_tiny<T>, dim3((ntiny + 3) / 4, nrhs), 256, 0, st, list, ntiny, sym, L, Dinv, y,
-                                x, rs);
+        if (ntiny > 0 && tiny_lanes == 32)
+            help::launch_kernel(k_bwd_tiny<T, 32>, dim3((ntiny + 7) / 8, nrhs), 256, 0, st, list, ntiny, sym, L,
+                                Dinv, y, x, rs);
+        else if (ntiny > 0)
+            help::launch_kernel(k_bwd_tiny<T, 64>, dim3((ntiny + 3) / 4, nrhs), 256, 0, st, list, ntiny, sym, L,
+                                Dinv, y, x, rs);
         return;
     }
     const int32_t grid = (ntiny + 3) / 4 + nrest + nx;
@@ -718,11 +724,12 @@ void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T*
 #define HELP_INST(T)                                                                                   \
     template void launch_fwd_level<T>(const int32_t*, int32_t, int32_t, const SolveTask*, int32_t,     \
                                       DevSym, const int8_t*, const T*, const T*, const T*,              \
-                                      const double*, double*, double*, int, RhsStride, hipStream_t);    \
+                                      const double*, double*, double*, int, RhsStride, hipStream_t,     \
+                                      int);                                                             \
     template void launch_bwd_level<T>(const int32_t*, int32_t, int32_t, const SolveTask*, int32_t,     \
                                       DevSym, const int8_t*, const T*, const T*, const T*,              \
                                       const double*, const double*, double*, int, RhsStride,            \
-                                      hipStream_t);                                                     \
+                                      hipStream_t, int);                                                \
     template void launch_fwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,       \
                                      double*, int, RhsStride, hipStream_t);                             \
     template void launch_bwd_gemv<T>(const SolveTask*, int32_t, DevSym, const T*, const double*,       \
