@@ -249,7 +249,11 @@ void build_schedules(help_gpu* h) {
                 }
                 if ((int32_t)h->list_small.size() > off) {
                     Op o{OP_SMALL, off, (int32_t)h->list_small.size() - off, w, lds};
-                    o.wave = pass == 0 ? 1 : 0;
+                    if (pass == 0) {   // 32 lanes per front when every front has at most 32 rows
+                        o.wave = 32;
+                        for (int32_t t = off; t < (int32_t)h->list_small.size(); ++t)
+                            if (S.snrow[h->list_small[t]] > 32) o.wave = 64;
+                    }
                     h->fops.push_back(o);
                 }
             }
@@ -665,7 +669,8 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
         switch (o.kind) {
             case OP_SMALL:
                 if (o.wave)
-                    launch_small_wave<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st);
+                    launch_small_wave<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds,
+                                         o.wave, st);
                 else
                     launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st);
                 break;

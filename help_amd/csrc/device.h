@@ -128,10 +128,11 @@ void launch_normal_matvec(int32_t m, int32_t n, const int64_t* rowptr, const int
 template <typename T>
 void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, T* Dinv,
                          int32_t* err, int32_t lds_elems, hipStream_t st);
-// the same for fronts of at most 64 rows, one wave each (lds_elems per front)
+// the same for fronts of at most 64 rows, one wave each (lds_elems per front);
+// lanes = 32: fronts of at most 32 rows, two per wave
 template <typename T>
 void launch_small_wave(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, T* Dinv,
-                       int32_t* err, int32_t lds_elems, hipStream_t st);
+                       int32_t* err, int32_t lds_elems, int32_t lanes, hipStream_t st);
 constexpr int kSmallWaveLds = 2048;   // LDS elements per front of the one-wave form
 template <typename T>
 void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges, DevSym sym, T* L, T* U,

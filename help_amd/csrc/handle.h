@@ -69,7 +69,7 @@ struct Op {
                                     // FP64 factor: the phase's fused panel tasks
                                     // (POTRF; its TRSM op then has lds = 1)
     double work2 = 0;  // flops of the fused panel launch
-    int32_t wave = 0;  // small fronts: the one-wave-per-front kernel
+    int32_t wave = 0;  // small fronts: lanes per front of the one-wave kernel (64 or 32), 0 = workgroup
 };
 
 inline const char* const kKindName[HELP_K_COUNT] = {

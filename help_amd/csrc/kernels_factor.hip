@@ -212,16 +212,17 @@ __device__ inline void wave_fence_all() {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <typename T>
+template <typename T, int G>
 __global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ list, int32_t count,
                                                     int32_t lds_elems, DevSym sym, T* __restrict__ L,
                                                     T* __restrict__ U, T* __restrict__ Dinv,
                                                     int32_t* __restrict__ err) {
     extern __shared__ unsigned char smem_raw[];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int32_t task = blockIdx.x * 4 + wave;
-    if (task >= count) return;   // the waves are independent: no barrier below
-    T* F = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * lds_elems;
+    // G lanes per front (c <= G): 64, or 32 for two fronts per wave
+    const int lane = threadIdx.x & (G - 1), grp = threadIdx.x / G;
+    const int32_t task = blockIdx.x * (256 / G) + grp;
+    if (task >= count) return;   // the fronts are independent: no barrier below
+    T* F = reinterpret_cast<T*>(smem_raw) + (int64_t)grp * lds_elems;
     const int32_t s = list[task];
     const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
     const int32_t c = sym.snrow[s];
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ 
     T* Lp = L + sym.loff[s];
     T* Up = U + sym.uoff[s];
     const int32_t cn = c * n;
-    for (int32_t idx = lane; idx < cn; idx += 64) F[idx] = Lp[idx];
+    for (int32_t idx = lane; idx < cn; idx += G) F[idx] = Lp[idx];
     wave_sync();
     for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
         const int32_t ch = sym.child[t];
@@ -238,13 +239,13 @@ __global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ 
         const int32_t* rel = sym.relmap + sym.reloff[ch];
         const T* Uc = U + sym.uoff[ch];
         const int64_t tot = (int64_t)chc * chc;
-        for (int64_t idx0 = lane; idx0 < tot; idx0 += 4 * 64) {
+        for (int64_t idx0 = lane; idx0 < tot; idx0 += 4 * G) {
             bool ok[4];
             int32_t rr[4], cc[4];
             T v[4], old[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int64_t idx = idx0 + (int64_t)q * 64;
+                const int64_t idx = idx0 + (int64_t)q * G;
                 const int64_t id = idx < tot ? idx : 0;
                 const int32_t jj = (int32_t)(id / chc), ii = (int32_t)(id - (int64_t)jj * chc);
                 ok[q] = idx < tot && ii >= jj;
@@ -281,13 +282,13 @@ __global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ 
         }
         wave_sync();
     }
-    for (int32_t idx = lane; idx < cn; idx += 64) Lp[idx] = F[idx];
+    for (int32_t idx = lane; idx < cn; idx += G) Lp[idx] = F[idx];
     // inverse of the (single, n <= 64) diagonal block, as invert_lower_block
     {
         const int32_t nb = n, ldx = nb + 1;
         T* X = F + cn;
         T* Db = Dinv + dinv_block_offset(sym, s, 0);
-        for (int idx = lane; idx < nb * nb; idx += 64) {
+        for (int idx = lane; idx < nb * nb; idx += G) {
             int j = idx / nb, i = idx - j * nb;
             X[j * ldx + i] = (i == j) ? T(1) : T(0);
         }
@@ -297,26 +298,26 @@ __global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ 
             if (lane <= j) X[lane * ldx + j] /= djj;
             wave_sync();
             const int cnt = (nb - j - 1) * (j + 1);
-            for (int idx = lane; idx < cnt; idx += 64) {
+            for (int idx = lane; idx < cnt; idx += G) {
                 int k = idx / (nb - j - 1), i = j + 1 + idx - k * (nb - j - 1);
                 X[k * ldx + i] -= F[j * c + i] * X[k * ldx + j];
             }
             wave_sync();
         }
-        for (int idx = lane; idx < nb * nb; idx += 64) {
+        for (int idx = lane; idx < nb * nb; idx += G) {
             int j = idx / nb, i = idx - j * nb;
             Db[j * nb + i] = (i >= j) ? X[j * ldx + i] : T(0);
         }
     }
     // U -= L21 L21^T, four entries per lane with their loads first
     const int32_t tot = cu * cu;
-    for (int32_t idx0 = lane; idx0 < tot; idx0 += 4 * 64) {
+    for (int32_t idx0 = lane; idx0 < tot; idx0 += 4 * G) {
         bool ok[4];
         int32_t pos[4];
         T old[4], acc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int32_t idx = idx0 + q * 64;
+            const int32_t idx = idx0 + q * G;
             const int32_t id = idx < tot ? idx : 0;
             const int32_t jj = id / cu, ii = id - jj * cu;
             ok[q] = idx < tot && ii >= jj;
@@ -1229,10 +1230,14 @@ void launch_small_fronts(const int32_t* list, int32_t count, DevSym sym, T* L, T
 }
 template <typename T>
 void launch_small_wave(const int32_t* list, int32_t count, DevSym sym, T* L, T* U, T* Dinv, int32_t* err,
-                       int32_t lds_elems, hipStream_t st) {
-    if (count > 0)
-        help::launch_kernel(k_small_wave<T>, (count + 3) / 4, 256, (size_t)4 * lds_elems * sizeof(T), st, list,
-                            count, lds_elems, sym, L, U, Dinv, err);
+                       int32_t lds_elems, int32_t lanes, hipStream_t st) {
+    if (count <= 0) return;
+    if (lanes == 32)
+        help::launch_kernel(k_small_wave<T, 32>, (count + 7) / 8, 256, (size_t)8 * lds_elems * sizeof(T), st,
+                            list, count, lds_elems, sym, L, U, Dinv, err);
+    else
+        help::launch_kernel(k_small_wave<T, 64>, (count + 3) / 4, 256, (size_t)4 * lds_elems * sizeof(T), st,
+                            list, count, lds_elems, sym, L, U, Dinv, err);
 }
 template <typename T>
 void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges, DevSym sym, T* L, T* U,
@@ -1277,7 +1282,7 @@ void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, cons
     template void launch_small_fronts<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,      \
                                          int32_t, hipStream_t);                                     \
     template void launch_small_wave<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,        \
-                                       int32_t, hipStream_t);                                       \
+                                       int32_t, int32_t, hipStream_t);                              \
     template void launch_extend_add<T>(const EaTask*, int32_t, const int32_t*, DevSym, T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, T*, hipStream_t);           \
     template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, T*, int32_t*,        \
