@@ -72,7 +72,8 @@ class GenParams(C.Structure):
     _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("m", C.c_int32), ("nb", C.c_int32),
                 ("k", C.c_int32), ("zipf_s", C.c_double), ("side", C.c_int32),
                 ("commodities", C.c_int32), ("pairs", C.c_int32), ("periods", C.c_int32),
-                ("block_rows", C.c_int32), ("block_cols", C.c_int32), ("block_k", C.c_int32)]
+                ("block_rows", C.c_int32), ("block_cols", C.c_int32), ("block_k", C.c_int32),
+                ("ub_cols", C.c_int32), ("free_cols", C.c_int32)]
 
 
 CONSTRUCT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(LP))

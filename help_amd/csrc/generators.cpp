@@ -1,7 +1,8 @@
 // Deterministic LP instance generators and a fixed-format MPS writer.
 //
-// Every instance is in HELP's final-model layout (all columns x >= 0 with no
-// upper bound, so n_lb = n, n_ub = n_fr = 0) and every value is rounded to
+// Every instance is in HELP's final-model layout (by default all columns
+// x >= 0 with no upper bound, so n_lb = n, n_ub = n_fr = 0; the random LP
+// can add bounded and free columns) and every value is rounded to
 // six decimals, so that the text written by help_write_mps parses back
 // (HELP MPSRead.cpp:150-170, atof at the fixed card positions of
 // MPSRead.cpp:18) to exactly the doubles held here.
@@ -135,8 +136,47 @@ void add_identity(Builder& B) {
     for (int32_t i = 0; i < B.m; ++i) { B.begin(); B.add(i, 1.0); B.end(); }
 }
 
+// Bounded / free variant of the random LP, in HELP's final column layout
+// (Presolve.cpp:1101-1170): the first nub columns of B carry upper bounds,
+// the rest of B and the identity a lower bound only, and nfr extra columns
+// (k nnz each) after the identity are free.  Interior point: x0 ~ U(0.5,1.5)
+// with u = x0 + U(0.5,1.5) on bounded columns, x0 ~ N(0,1) on free ones;
+// y0 ~ N(0,1); reduced costs s0 ~ U(0.5,1.5), zero on free columns (their
+// dual row is an equality).
+int finish_bounded(Rng& g, Builder& B, int32_t nub, int32_t nfr, help_lp* lp) {
+    const int32_t n = B.ncols(), m = B.m;
+    std::vector<double> x0(n), y0(m), s0(n, 0.0), u(nub);
+    for (int32_t j = 0; j < n - nfr; ++j) x0[j] = g.uniform(0.5, 1.5);
+    for (int32_t j = n - nfr; j < n; ++j) x0[j] = g.normal();
+    for (int32_t j = 0; j < nub; ++j) u[j] = r6(x0[j] + g.uniform(0.5, 1.5));
+    for (int32_t i = 0; i < m; ++i) y0[i] = g.normal();
+    for (int32_t j = 0; j < n - nfr; ++j) s0[j] = g.uniform(0.5, 1.5);
+    std::vector<double> b(m, 0.0), c(n, 0.0);
+    for (int32_t j = 0; j < n; ++j) {
+        double cj = 0.0;
+        for (int64_t p = B.colptr[j]; p < B.colptr[j + 1]; ++p) {
+            b[B.rowind[p]] += B.val[p] * x0[j];
+            cj += B.val[p] * y0[B.rowind[p]];
+        }
+        c[j] = r6(cj + s0[j]);
+    }
+    for (int32_t i = 0; i < m; ++i) b[i] = r6(b[i]);
+    int st = finish(B, b, c, lp);
+    if (st != HELP_OK) return st;
+    lp->n_ub = nub;
+    lp->n_fr = nfr;
+    lp->n_lb = n - nfr;
+    if (nub > 0) {
+        lp->ub = (double*)std::malloc(sizeof(double) * nub);
+        if (!lp->ub) return HELP_ERR_NOMEM;
+        std::memcpy(lp->ub, u.data(), sizeof(double) * nub);
+    }
+    return HELP_OK;
+}
+
 int gen_random(const help_gen_params* p, help_lp* lp) {
     if (p->m <= 0 || p->nb < 0 || p->k <= 0 || p->k > p->m) return HELP_ERR_ARG;
+    if (p->ub_cols < 0 || p->ub_cols > p->nb || p->free_cols < 0) return HELP_ERR_ARG;
     Rng g(p->seed);
     Builder B; B.m = p->m;
     std::vector<int32_t> rows;
@@ -147,6 +187,15 @@ int gen_random(const help_gen_params* p, help_lp* lp) {
         B.end();
     }
     add_identity(B);
+    if (p->ub_cols > 0 || p->free_cols > 0) {
+        for (int32_t j = 0; j < p->free_cols; ++j) {
+            distinct_rows(g, 0, p->m, p->k, rows);
+            B.begin();
+            for (int32_t r : rows) B.add(r, nz_normal(g));
+            B.end();
+        }
+        return finish_bounded(g, B, p->ub_cols, p->free_cols, lp);
+    }
     std::vector<double> b, c;
     feasible_rhs_cost(g, B, b, c);
     return finish(B, b, c, lp);
@@ -284,7 +333,9 @@ extern "C" void help_lp_free(help_lp* lp) {
 // sides are omitted; MPSRead leaves them at 0.
 extern "C" int help_write_mps(const help_lp* lp, const char* path, const char* name) {
     if (!lp || !path) return HELP_ERR_ARG;
-    if (lp->m > 9999999 || lp->n > 9999999 || lp->n_ub != 0 || lp->n_fr != 0) return HELP_ERR_ARG;
+    if (lp->m > 9999999 || lp->n > 9999999 || lp->n_ub < 0 || lp->n_fr < 0 ||
+        lp->n_ub + lp->n_fr > lp->n || (lp->n_ub > 0 && !lp->ub))
+        return HELP_ERR_ARG;
     FILE* f = std::fopen(path, "w");
     if (!f) return HELP_ERR_ARG;
     std::fprintf(f, "NAME          %s\n", name ? name : "HELPGEN");
@@ -299,6 +350,13 @@ extern "C" int help_write_mps(const help_lp* lp, const char* path, const char* n
     std::fprintf(f, "RHS\n");
     for (int32_t i = 0; i < lp->m; ++i)
         if (lp->b[i] != 0.0) std::fprintf(f, "    RHS       R%07d  %.6f\n", i, lp->b[i]);
+    // bounds at card positions 1 (type), 4 (set), 14 (column), 24 (value)
+    // as MPSRead.cpp:286-345 reads them; LB stays at its default 0
+    if (lp->n_ub > 0 || lp->n_fr > 0) {
+        std::fprintf(f, "BOUNDS\n");
+        for (int32_t j = 0; j < lp->n_ub; ++j) std::fprintf(f, " UP BND       C%07d  %.6f\n", j, lp->ub[j]);
+        for (int32_t j = lp->n - lp->n_fr; j < lp->n; ++j) std::fprintf(f, " FR BND       C%07d\n", j);
+    }
     std::fprintf(f, "ENDATA\n");
     std::fclose(f);
     return HELP_OK;

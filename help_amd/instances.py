@@ -2,7 +2,9 @@
 
 Workloads (sizes as named by the benchmark plan):
 * random     - A=[B | I_m], B m x nb with k N(0,1) nnz per column at distinct
-               uniform rows; x0/y0/s0 feasibility construction.
+               uniform rows; x0/y0/s0 feasibility construction.  Optionally
+               upper bounds on the first columns of B and free columns after
+               the identity.
 * mcf        - block-angular multicommodity flow on a side x side 4-connected
                grid with K commodities, 20 source-sink pairs each.
 * staircase  - T periods of 1000 x 3000 blocks, 5 nnz per column in-block plus
@@ -29,13 +31,17 @@ def _generate(params):
         val = np.ctypeslib.as_array(lp.val, shape=(max(nnz, 1),))[:nnz].copy()
         b = np.ctypeslib.as_array(lp.b, shape=(m,)).copy()
         c = np.ctypeslib.as_array(lp.c, shape=(n,)).copy()
+        ub = np.ctypeslib.as_array(lp.ub, shape=(lp.n_ub,)).copy() if lp.n_ub > 0 else None
+        n_ub, n_fr = lp.n_ub, lp.n_fr
     finally:
         L.lib().help_lp_free(C.byref(lp))
-    return Instance(m, n, colptr, rowind, val, b, c)
+    return Instance(m, n, colptr, rowind, val, b, c, n_ub=n_ub, n_fr=n_fr, ub=ub)
 
 
-def random_lp(m=2000, nb=4000, k=4, seed=0):
-    p = L.GenParams(kind=L.GEN_RANDOM, seed=seed, m=m, nb=nb, k=k)
+def random_lp(m=2000, nb=4000, k=4, seed=0, ub_cols=0, free_cols=0):
+    """ub_cols > 0 bounds the first ub_cols columns of B above; free_cols > 0
+    appends that many free columns (HELP's [UB | LB-only | free] layout)."""
+    p = L.GenParams(kind=L.GEN_RANDOM, seed=seed, m=m, nb=nb, k=k, ub_cols=ub_cols, free_cols=free_cols)
     return _generate(p)
 
 

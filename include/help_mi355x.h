@@ -283,12 +283,19 @@ typedef struct help_gen_params {
     double  zipf_s;          /* ZIPF exponent                                   */
     int32_t side, commodities, pairs; /* MCF                                    */
     int32_t periods, block_rows, block_cols, block_k; /* STAIRCASE              */
+    int32_t ub_cols;         /* RANDOM: the first ub_cols columns of B get an
+                                upper bound (HELP's n_UB block)                 */
+    int32_t free_cols;       /* RANDOM: free_cols extra free columns after the
+                                identity (HELP's n_FR block)                    */
 } help_gen_params;
 
 /* Allocates the arrays of *lp (free with help_lp_free). */
 int  help_generate(const help_gen_params* p, help_lp* lp);
 void help_lp_free(help_lp* lp);
-/* Write *lp in fixed-format MPS readable by HELP's MPSRead.cpp. */
+/* Write *lp in fixed-format MPS readable by HELP's MPSRead.cpp, with a
+ * BOUNDS section (UP for the n_ub bounded columns, FR for the n_fr free
+ * ones) that Crushing.cpp and Presolve_FinalizeModel turn back into the
+ * same column layout. */
 int  help_write_mps(const help_lp* lp, const char* path, const char* name);
 
 /* ------------------------------------------------------------------ */

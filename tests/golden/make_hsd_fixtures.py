@@ -22,7 +22,25 @@ CASES = {
     "mcf_12x12_k3_s1003": ("mcf", dict(side=12, commodities=3, pairs=6, seed=1003)),
     "staircase_5x40x120_s1004": ("staircase", dict(periods=5, block_rows=40, block_cols=120, block_k=5,
                                                    seed=1004)),
+    # bounded and free columns (HELP's n_UB and n_FR blocks)
+    "random_300x600_ub150_s1010": ("random_lp", dict(m=300, nb=600, k=4, seed=1010, ub_cols=150)),
+    "random_300x600_fr20_s1010": ("random_lp", dict(m=300, nb=600, k=4, seed=1010, free_cols=20)),
+    "random_300x600_ub100_fr2_s1013": ("random_lp", dict(m=300, nb=600, k=4, seed=1013, ub_cols=100,
+                                                         free_cols=2)),
 }
+
+
+def bounds_of(inst):
+    """linprog bounds of HELP's column layout [UB | LB-only | free]."""
+    out = []
+    for j in range(inst.n):
+        if j < inst.n_ub:
+            out.append((0, float(inst.ub[j])))
+        elif j >= inst.n - inst.n_fr:
+            out.append((None, None))
+        else:
+            out.append((0, None))
+    return out
 
 
 def main():
@@ -30,7 +48,7 @@ def main():
     for name, (gen, kw) in CASES.items():
         inst = getattr(instances, gen)(**kw)
         A = sp.csc_matrix((inst.val, inst.rowind, inst.colptr), shape=(inst.m, inst.n))
-        r = linprog(inst.c, A_eq=A, b_eq=inst.b, bounds=(0, None), method="highs")
+        r = linprog(inst.c, A_eq=A, b_eq=inst.b, bounds=bounds_of(inst), method="highs")
         assert r.status == 0, (name, r.message)
         out[name] = dict(generator=gen, params=kw, m=inst.m, n=inst.n, nnz=inst.nnz,
                          objective=float(r.fun))

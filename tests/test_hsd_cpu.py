@@ -54,10 +54,7 @@ def _ref_result(mps):
     return kv
 
 
-@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="reference driver not built (no reference sources)")
-@pytest.mark.parametrize("name", ["random_300x600_s1000", "mcf_8x8_k2_s1002"])
-def test_driver_bit_identical_to_reference_driver(name, tmp_path):
-    inst = _instance(FIX[name])
+def _compare_with_reference_driver(inst, tmp_path):
     mps = str(tmp_path / "inst.mps")
     instances.write_mps(inst, mps)
     ref = _ref_result(mps)
@@ -66,6 +63,30 @@ def test_driver_bit_identical_to_reference_driver(name, tmp_path):
         res, rows = solve_lp(inst, ob.table, exact=True)
     finally:
         ob.close()
+    assert int(ref["status"]) == res.status
     assert float(ref["pobj"]) == res.primal_obj
     assert float(ref["dobj"]) == res.dual_obj
     assert int(ref["m"]) == inst.m and int(ref["n"]) == inst.n
+    return res
+
+
+@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="reference driver not built (no reference sources)")
+@pytest.mark.parametrize("name", ["random_300x600_s1000", "mcf_8x8_k2_s1002", "random_300x600_ub150_s1010",
+                                  "random_300x600_fr20_s1010", "random_300x600_ub100_fr2_s1013"])
+def test_driver_bit_identical_to_reference_driver(name, tmp_path):
+    """The bounded/free cases go through MPSRead's BOUNDS section, Crushing.cpp
+    and Presolve_FinalizeModel on the reference side and exercise the n_UB
+    block (HSDSolver.cpp:70-82,99-100,160-165,203-207) and the free-column
+    terms (HSDSolver.cpp:84-90,101-102) of both drivers."""
+    res = _compare_with_reference_driver(_instance(FIX[name]), tmp_path)
+    assert L.HSD_STATUS[res.status] == "OK"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="reference driver not built (no reference sources)")
+def test_driver_matches_reference_driver_stall(tmp_path):
+    """Bounded and free columns together on this seed make HELP's own driver
+    stop on its inaccurate-Newton-direction test (HSDSolver.cpp:814-818);
+    the restatement stops at the same iterate with the same status."""
+    inst = instances.random_lp(300, 600, 4, seed=1011, ub_cols=150, free_cols=5)
+    res = _compare_with_reference_driver(inst, tmp_path)
+    assert L.HSD_STATUS[res.status] == "STALLED" and res.stall_reason == 2
