@@ -1,22 +1,34 @@
 #!/usr/bin/env python3
-"""Benchmark: HSD wall time to the 1e-8 relative-gap stop test on one MI355X.
+"""Benchmark: HSD wall time to the 1e-8 relative-gap stop test on one MI355X,
+mixed precision against plain FP64.
 
-Workload (default): block-angular multicommodity flow on a 150x150 4-connected
-grid with K=8 commodities (m=269,392, n=804,600, nnz=2,234,968), plain FP64
-direct path: every normal-equation solve of HELP's HSD driver goes through the
-MI355X backend (ADA^T assembly, supernodal Cholesky, triangular solves).
+Default workload: the staircase LP with T=200 periods (1000 x 3000 blocks,
+5 N(0,1) nnz per column in-block plus one linking nnz into the next period,
+plus slack identity: m=200,000, n=800,000, nnz=3,797,000).  Every
+normal-equation solve of HELP's HSD driver goes through the MI355X backend.
+Three paths run on the same instance, each as W untimed and K timed steps:
+
+* plain_fp64_amd  FP64 factor, AMD ordering: CHOLMOD's semantics in HELP
+                  (UseCholmod.cpp, Setting_CPU.cpp:21-26), one factor solve per
+                  system;
+* plain_fp64_nd   FP64 factor, nested-dissection ordering, one factor solve;
+* mixed_fp32_nd   FP32 factor (MFMA, FP32 accumulation) + FP64 conjugate
+                  gradients preconditioned by it, nested dissection.
 
 A "step" is one complete HSD solve (construct, iterate to the stop test,
-destruct) - the span HELP times as HSD_Main (Main.cpp:95-97).  The JSON line
-reports the mean seconds per step (lower is better), the dominant kernel's
-roofline measured with HIP events on the library's stream in one extra,
-untimed solve, and the CPU oracle timed on a bounded sample of the same
-workload.
+destruct): the span HELP times as HSD_Main (Main.cpp:95-97).  `value` is the
+mixed path's mean seconds per step; the plain times and the speedups sit
+beside it.  Also reported: the dominant kernel's roofline, measured with HIP
+events on the library's stream in one extra untimed solve of the value path;
+the dy error of the mixed path against the plain FP64 path on solves taken
+from its own trajectory; and MKL PARDISO on all host cores (HELP's
+UsePardiso.cpp semantics) timed on a bounded sample of the same workload.
 """
 import argparse
 import ctypes as C
 import json
 import os
+import platform
 import sys
 import time
 
@@ -24,6 +36,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"hbm": 8000.0, "mfma_fp64": 78.6, "mfma_fp32": 157.3}   # GB/s, TFLOP/s (MI355X spec)
+
+# name -> (mode, ordering)
+PATHS = {
+    "plain_fp64_amd": ("fp64", 0),
+    "plain_fp64_nd": ("fp64", 1),
+    "mixed_fp32_amd": ("fp32", 0),
+    "mixed_fp32_nd": ("fp32", 1),
+}
+DEFAULT_PATHS = {0: "plain_fp64_amd,mixed_fp32_amd", 1: "plain_fp64_amd,mixed_fp32_amd",
+                 2: "plain_fp64_amd,plain_fp64_nd,mixed_fp32_nd"}
+NAMES = {0: "random LP m=2000 n=6000 (A=[B|I], 4 nnz/col)",
+         1: "multicommodity flow 150x150 grid, K=8 (m=269392, n=804600, nnz=2234968)",
+         2: "staircase T=200, 1000x3000 blocks (m=200000, n=800000, nnz=3797000)"}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def kernel_table(L, table):
@@ -52,53 +81,148 @@ def roofline(rows, mode, traffic_file):
     traffic = None
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
-            t = json.load(f)
-        traffic = t.get(dom["kernel"])
+            traffic = json.load(f).get(dom["kernel"])
     return dict(kernel=dom["kernel"], bound=bound, achieved=round(achieved, 4), peak=peak, unit=unit,
                 frac=round(achieved / peak, 5), traffic=traffic, avg_launch_ms=round(avg_ms, 5),
                 work_per_launch=work_per_launch, launches=dom["launches"])
 
 
-def cpu_baseline(inst, n_renew, n_solve, host_driver_s, sample_renews=1, sample_solves=1):
-    """CPU FP64 oracle (single thread) on a bounded sample of the same workload:
-    construct + `sample_renews` renews + `sample_solves` solves, scaled to the
-    GPU run's renew/solve counts; the HSD host driver time is shared."""
+def host_cores():
+    """Threads the CPU baseline uses: OMP_NUM_THREADS when set (the GPU box
+    sets it to the host-core share of its GPU), else the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(inst, gpu_iterations, sample_iters=1):
+    """HELP's PARDISO backend (UsePardiso.cpp semantics over MKL PARDISO from
+    libmkl_rt, oracle/libpardiso_be.so) under the same HSD driver on all host
+    cores, for `sample_iters` iterations (+ the initial point); the full run is
+    estimated as construct + per-iteration time x the GPU run's iterations."""
+    from help_amd import _lib as L
+    from help_amd.hsd import solve_lp
+    so = os.path.join(ROOT, "oracle", "libpardiso_be.so")
+    if not os.path.exists(so):
+        return None
+    pb = C.CDLL(so)
+    pb.pb_new.restype = C.c_void_p
+    pb.pb_new.argtypes = [C.c_int]
+    pb.pb_error.restype = C.c_char_p
+    pb.pb_stat.restype = C.c_double
+    pb.pb_stat.argtypes = [C.c_void_p, C.c_int]
+    pb.pb_free.argtypes = [C.c_void_p]
+    cores = host_cores()
+    ctx = pb.pb_new(cores)
+    if not ctx:
+        log(f"CPU baseline unavailable: {pb.pb_error().decode()}")
+        return None
+    table = L.Backend()
+    table.ctx = ctx
+    for nm in ("construct", "destruct", "renew", "solve"):
+        setattr(table, nm, C.cast(getattr(pb, "pb_" + nm), C.c_void_p))
+    t0 = time.time()
+    res, _ = solve_lp(inst, table, max_iter=sample_iters)
+    wall = time.time() - t0
+    t_con = pb.pb_stat(ctx, 0)
+    n_ren, n_sol = int(pb.pb_stat(ctx, 3)), int(pb.pb_stat(ctx, 4))
+    t_ren, t_sol = pb.pb_stat(ctx, 1), pb.pb_stat(ctx, 2)
+    pb.pb_free(ctx)
+    per_iter = (wall - t_con) / max(n_ren, 1)
+    est = t_con + per_iter * (gpu_iterations + 1)
+    sample = (f"MKL PARDISO (UsePardiso.cpp semantics: METIS ND ordering, mtype -2, phases 11/22/33) under "
+              f"HELP's HSD driver, {cores} threads on {cpu_model()} (nproc {os.cpu_count()}): construct "
+              f"{t_con:.2f}s, then {n_ren} renews ({t_ren / max(n_ren, 1):.2f}s each) and {n_sol} solves "
+              f"({t_sol / max(n_sol, 1):.3f}s each) in {res.iterations} iteration(s) + the initial point, "
+              f"{wall:.1f}s measured; the full run is estimated as construct + {per_iter:.2f}s per iteration "
+              f"x ({gpu_iterations} + 1) iterations of the GPU run")
+    return dict(value=round(est, 3), unit="s", cores=cores, kind="port", sample=sample,
+                measured_s=round(wall, 3))
+
+
+def run_path(inst, name, steps, warmup, driver, extra, L, GpuBackend, solve_lp):
+    mode_s, ordering = PATHS[name]
+    mode = L.MODE_DIRECT_FP32 if mode_s == "fp32" else L.MODE_DIRECT_FP64
+    for w in range(warmup):
+        gb = GpuBackend(mode=mode, ordering=ordering, **extra)
+        res, _ = solve_lp(inst, gb.table, device=driver == "gpu")
+        gb.close()
+        log(f"{name} warmup {w}: status {res.status} iters {res.iterations} {res.seconds:.2f}s")
+    L.lib().help_device_synchronize()
+    gb = GpuBackend(mode=mode, ordering=ordering, profile=0, **extra)
+    times, results = [], []
+    for k in range(steps):
+        ts = time.time()
+        res, _ = solve_lp(inst, gb.table, device=driver == "gpu")
+        L.lib().help_device_synchronize()
+        times.append(time.time() - ts)
+        results.append(res)
+        log(f"{name} step {k}: status {res.status} iters {res.iterations} gap {res.rel_gap:.2e} "
+            f"{times[-1]:.3f}s (construct {res.seconds_construct:.2f}s, renew {res.seconds_renew:.2f}s, "
+            f"solve {res.seconds_solve:.2f}s)")
+    n_sol, n_ref, max_rel = gb.solve_stats()
+    gb.close()
+    res = results[-1]
+    host_driver = max(0.0, res.seconds - res.seconds_renew - res.seconds_solve - res.seconds_construct)
+    return dict(
+        seconds=sum(times) / len(times), mode=mode_s, ordering="nd" if ordering else "amd",
+        status=L.HSD_STATUS.get(res.status, str(res.status)), iterations=res.iterations,
+        rel_gap=res.rel_gap, primal_obj=res.primal_obj, primal_infeas=res.primal_infeas,
+        dual_infeas=res.dual_infeas, construct_s=round(res.seconds_construct, 3),
+        renew_s=round(res.seconds_renew, 3), solve_s=round(res.seconds_solve, 3),
+        driver_s=round(host_driver, 3), n_renew=res.n_renew, n_solve=res.n_solve,
+        refine_iterations=int(n_ref // max(steps, 1)), max_rel_residual=max_rel,
+        ms_per_iteration=round(1e3 * (sum(times) / len(times)) / max(res.iterations, 1), 3))
+
+
+def dy_errors(inst, name, capture, extra):
+    """dy of the `name` path on the bottom-system solves of chosen iterations
+    of its own trajectory (HELP's driver on the host, HSDSolver.cpp:770),
+    against the plain FP64 path (AMD ordering, CHOLMOD semantics) on the same
+    (d, b1, b2); plus the FP64 backward error of each."""
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_bind as O
     from help_amd import LinearEquation
     from help_amd import _lib as L
-    be = LinearEquation(symbolic_only=True)
-    be.construct(inst)
-    perm, _, _ = be.symbolic_arrays()
-    be.destruct()
-    ob = O.OracleBackend(1e-5, perm)
-    rng = np.random.default_rng(0)
-    d = np.exp(rng.uniform(-3, 3, inst.n) * np.log(10.0))
-    b1 = rng.standard_normal(inst.n)
-    b2 = rng.standard_normal(inst.m)
-    x1 = np.empty(inst.n)
-    x2 = np.empty(inst.m)
-    lp = inst.as_struct()
-    o = O.lib()
-    t0 = time.time()
-    o.or_be_construct(C.c_void_p(ob.ptr), C.byref(lp))
-    t_con = time.time() - t0
-    t0 = time.time()
-    for _ in range(sample_renews):
-        o.or_be_renew(C.c_void_p(ob.ptr), L.dptr(d))
-    t_ren = (time.time() - t0) / sample_renews
-    t0 = time.time()
-    for _ in range(sample_solves):
-        o.or_be_solve(C.c_void_p(ob.ptr), L.dptr(d), L.dptr(b1), L.dptr(b2), L.dptr(x1), L.dptr(x2))
-    t_sol = (time.time() - t0) / sample_solves
-    ob.close()
-    est = t_con + t_ren * n_renew + t_sol * n_solve + host_driver_s
-    sample = (f"oracle construct + {sample_renews} renew (assembly + left-looking FP64 Cholesky, "
-              f"{t_ren:.2f}s) + {sample_solves} KKT solve ({t_sol:.3f}s) on the same instance, scaled to "
-              f"the GPU run's {n_renew} renews and {n_solve} solves plus its {host_driver_s:.2f}s of host "
-              f"HSD driver work")
-    return dict(value=round(est, 3), unit="s", cores=1, kind="port", sample=sample)
+    from help_amd.hsd import RecordingBackend, solve_lp
+    mode_s, ordering = PATHS[name]
+    mode = L.MODE_DIRECT_FP32 if mode_s == "fp32" else L.MODE_DIRECT_FP64
+    recs = {}
+
+    def keep(renews, k, d, b1, b2, x1, x2):
+        if k == 0 and renews in capture:
+            recs[renews] = (d.copy(), b1.copy(), b2.copy(), x2.copy())
+
+    rb = RecordingBackend(inst, keep, mode=mode, ordering=ordering, **extra)
+    res, _ = solve_lp(inst, rb.table)
+    A = inst.to_scipy().tocsr()
+    At = A.T.tocsr()
+    ref = LinearEquation(mode=L.MODE_DIRECT_FP64, ordering=0)
+    ref.construct(inst)
+    out = []
+    for k in sorted(recs):
+        d, b1, b2, x2 = recs[k]
+        ref.renew(d)
+        _, o2 = ref.solve(d, b1, b2)
+        rhs = A @ (b1 / d) - b2
+        bwd = lambda x: float(np.linalg.norm(rhs - (A @ ((At @ x) / d) + 1e-5 * x)) / np.linalg.norm(rhs))
+        out.append(dict(iteration=k, d_decades=round(float(np.log10(d.max() / d.min())), 2),
+                        dy_rel_err_vs_plain_fp64=float(np.linalg.norm(x2 - o2) / np.linalg.norm(o2)),
+                        backward_err=bwd(x2), backward_err_plain_fp64=bwd(o2)))
+        log(f"dy error {name} iteration {k}: {out[-1]}")
+    ref.destruct()
+    return out, L.HSD_STATUS.get(res.status, str(res.status)), res.iterations
 
 
 def main():
@@ -106,18 +230,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2])
+    ap.add_argument("--paths", default=None,
+                    help="comma list of " + ",".join(PATHS) + "; the last one is `value` "
+                         "(default per config: " + "; ".join(f"{k}: {v}" for k, v in DEFAULT_PATHS.items()) + ")")
     ap.add_argument("--driver", default="gpu", choices=["gpu", "host"],
                     help="gpu: device-resident HSD vectors; host: HELP-order host driver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-dy-error", action="store_true")
     ap.add_argument("--inverse-solve", type=int, default=None, choices=[-1, 0, 1],
                     help="explicit L11 inverse for supernodes wider than 64 columns: "
                          "0 or 1 on (default), -1 off (block substitution)")
-    ap.add_argument("--refine-max", type=int, default=None,
-                    help="FP64 refinement iterations per solve (default: the library's, "
-                         "0 in the FP64 mode as in UseCholmod.cpp, 20 in the FP32 mode)")
     args = ap.parse_args()
     if args.gpus != 1:
         msg = ("bench.py measures one MI355X: this backend is single-GPU "
@@ -132,88 +256,74 @@ def main():
 
     t0 = time.time()
     inst = instances.by_config(args.config)
-    gen_s = time.time() - t0
-    mode = L.MODE_DIRECT_FP32 if args.mode == "fp32" else L.MODE_DIRECT_FP64
-    print(f"[bench] config {args.config} m={inst.m} n={inst.n} nnz={inst.nnz} mode={args.mode} "
-          f"(generated in {gen_s:.1f}s)", file=sys.stderr, flush=True)
-
+    log(f"config {args.config} m={inst.m} n={inst.n} nnz={inst.nnz} (generated in {time.time() - t0:.1f}s)")
+    paths = (args.paths or DEFAULT_PATHS[args.config]).split(",")
+    for p in paths:
+        if p not in PATHS:
+            raise SystemExit(f"unknown path {p}")
     extra = {}
-    if args.refine_max is not None:
-        extra = {"refine_fp64" if args.mode == "fp64" else "refine_max": args.refine_max}
     if args.inverse_solve is not None:
         extra["inverse_solve"] = args.inverse_solve
-    for w in range(args.warmup):
-        gb = GpuBackend(mode=mode, **extra)
-        res, _ = solve_lp(inst, gb.table, device=args.driver == "gpu")
-        gb.close()
-        print(f"[bench] warmup {w}: status {res.status} iters {res.iterations} {res.seconds:.2f}s",
-              file=sys.stderr, flush=True)
 
-    L.lib().help_device_synchronize()
-    gb = GpuBackend(mode=mode, profile=0, **extra)
-    times, results = [], []
-    for k in range(args.steps):
-        ts = time.time()
-        res, _ = solve_lp(inst, gb.table, device=args.driver == "gpu")
-        L.lib().help_device_synchronize()
-        times.append(time.time() - ts)
-        results.append(res)
-        print(f"[bench] step {k}: status {res.status} iters {res.iterations} gap {res.rel_gap:.2e} "
-              f"{times[-1]:.3f}s (renew {res.seconds_renew:.2f}s, solve {res.seconds_solve:.2f}s)",
-              file=sys.stderr, flush=True)
-    n_sol, n_ref, max_rel = gb.solve_stats()
-    gb.close()
-    res = results[-1]
-    value = sum(times) / len(times)
-    # kernel-class timings come from one more, untimed solve with HIP events
-    # around every launch on the library's stream (the events cost ~20% of
-    # the wall time, so the timed steps run without them)
+    results = {}
+    for p in paths:
+        results[p] = run_path(inst, p, args.steps, args.warmup, args.driver, extra, L, GpuBackend, solve_lp)
+    vp = paths[-1]
+    vres = results[vp]
+    mode_s, ordering = PATHS[vp]
+
+    # kernel-class timings of the value path: one more, untimed solve with HIP
+    # events around every launch on the library's stream (the events cost
+    # time, so the timed steps run without them)
     rows, prof_s = [], None
     if not args.no_profile:
-        gp = GpuBackend(mode=mode, profile=1, **extra)
+        gp = GpuBackend(mode=L.MODE_DIRECT_FP32 if mode_s == "fp32" else L.MODE_DIRECT_FP64,
+                        ordering=ordering, profile=1, **extra)
         ts = time.time()
         rp, _ = solve_lp(inst, gp.table, device=args.driver == "gpu")
         L.lib().help_device_synchronize()
         prof_s = time.time() - ts
         rows = kernel_table(L, gp.table)
         gp.close()
-        print(f"[bench] profiled solve: status {rp.status} iters {rp.iterations} {prof_s:.3f}s",
-              file=sys.stderr, flush=True)
-    roof = roofline(rows, args.mode, os.path.join(ROOT, "profiles", "traffic.json")) if rows else None
+        log(f"profiled solve ({vp}): status {rp.status} iters {rp.iterations} {prof_s:.3f}s")
+    roof = roofline(rows, mode_s, os.path.join(ROOT, "profiles", "traffic.json")) if rows else None
     kernels = {r["kernel"]: dict(ms_total=round(r["ms"], 3), launches=r["launches"],
                                  rate=round(r["work"] / (r["ms"] * 1e-3) / (1e9 if r["bound"] == "hbm" else 1e12), 3),
                                  unit="GB/s" if r["bound"] == "hbm" else "TFLOP/s") for r in rows}
-    # driver work outside the backend (construct, vector updates, reductions)
-    host_driver = max(0.0, res.seconds - res.seconds_renew - res.seconds_solve)
+
+    dy = None
+    if not args.no_dy_error:
+        it = max(vres["iterations"], 1)
+        cap = sorted({1, max(1, it // 2), it})
+        dy_list, st, its = dy_errors(inst, vp, cap, extra)
+        dy = dict(path=vp, driver="host (HELP order)", status=st, iterations=its, solves=dy_list)
+
     cpu = None
     if not args.no_cpu_baseline:
-        print("[bench] timing the CPU oracle sample...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(inst, res.n_renew, res.n_solve, host_driver)
-    names = {0: "random LP m=2000 n=6000 (A=[B|I], 4 nnz/col)",
-             1: "multicommodity flow 150x150 grid, K=8 (m=269392, n=804600, nnz=2234968)",
-             2: "staircase T=200, 1000x3000 blocks (m=200000, n=800000)"}
-    path = {"fp64": "plain FP64 direct path", "fp32": "FP32 factor + FP64 refinement direct path"}[args.mode]
+        log("timing the CPU baseline sample (MKL PARDISO, all host cores)...")
+        cpu = cpu_baseline(inst, vres["iterations"])
+
+    value = vres["seconds"]
+    speed = {}
+    for p in paths[:-1]:
+        speed[f"speedup_vs_{p}"] = round(results[p]["seconds"] / value, 3)
+    for p in paths:
+        results[p]["seconds"] = round(results[p]["seconds"], 4)
+    metric = ("time to 1e-8 rel gap (HSD_Main span): "
+              + ("mixed precision (FP32 factor + FP64 refinement)" if mode_s == "fp32" else "plain FP64")
+              + " vs plain FP64, on 1 MI355X")
     line = {
-        "metric": f"time to 1e-8 rel gap (HSD_Main span), {path} on 1 MI355X",
+        "metric": metric,
         "value": round(value, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * value, 2), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if args.mode == "fp64" else "f32 factor + f64 refinement",
+        "vs_baseline": None, "dtype": "f64" if mode_s == "fp64" else "f32 factor + f64 refinement",
         "data": "synthetic (deterministic generator, seed 0)",
-        "config": {"workload": names[args.config], "mode": args.mode, "solver": "HELP HSD driver",
-                   "status": L.HSD_STATUS.get(res.status, str(res.status)), "iterations": res.iterations,
-                   "rel_gap": res.rel_gap, "primal_obj": res.primal_obj,
-                   "primal_infeas": res.primal_infeas, "dual_infeas": res.dual_infeas,
-                   "ms_per_iteration": round(1e3 * value / max(res.iterations, 1), 3),
-                   "renew_s": round(res.seconds_renew, 3), "solve_s": round(res.seconds_solve, 3),
-                   "driver": args.driver, "driver_s": round(host_driver, 3),
-                   "construct_s": round(res.seconds_construct, 3),
-                   "driver_newton_s": round(res.seconds_newton, 3),
-                   "driver_direction_s": round(res.seconds_direction, 3),
-                   "driver_step_s": round(res.seconds_step, 3),
-                   "n_renew": res.n_renew, "n_solve": res.n_solve,
-                   "refine_iterations": n_ref, "max_rel_residual": max_rel,
+        "config": {"workload": NAMES[args.config], "value_path": vp, "driver": args.driver,
+                   "solver": "HELP HSD driver", **speed,
+                   "paths": results,
                    "inverse_solve": "off" if args.inverse_solve == -1 else "on",
                    "profiled_solve_s": None if prof_s is None else round(prof_s, 3)},
+        "dy_error": dy,
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
     }
     print(json.dumps(line), flush=True)

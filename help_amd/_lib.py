@@ -30,6 +30,7 @@ FAULT_REFINE_NOCONV = 2
 FAULT_NET_NONFINITE = 4
 
 GEN_RANDOM, GEN_MCF, GEN_STAIRCASE, GEN_ZIPF = 0, 1, 2, 3
+ORDER_AMD, ORDER_ND = 0, 1
 
 HSD_STATUS = {0: "OK", 1: "STALLED", 2: "ITER_LIMIT", 3: "UNKNOWN_ERROR"}
 
@@ -40,7 +41,7 @@ class Opts(C.Structure):
                 ("pcg_tol", C.c_double), ("pcg_fp32_precond", C.c_int32),
                 ("fault_flags", C.c_int32), ("verbose", C.c_int32), ("symbolic_only", C.c_int32),
                 ("front_path", C.c_int32), ("profile", C.c_int32), ("refine_fp64", C.c_int32),
-                ("inverse_solve", C.c_int32)]
+                ("inverse_solve", C.c_int32), ("ordering", C.c_int32)]
 
 
 class SolveStats(C.Structure):

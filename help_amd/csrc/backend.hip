@@ -978,7 +978,10 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
             adat_adjacency(m, h->P, adjptr, adj);
             phase("adjacency");
             RelaxParams rp;
-            h->S = analyze(m, adjptr, adj, {}, rp);
+            std::vector<int32_t> perm;
+            if (h->opts.ordering == HELP_ORDER_ND) perm = help::nd_order(m, adjptr, adj);
+            else if (h->opts.ordering != HELP_ORDER_AMD) throw std::runtime_error("unknown ordering");
+            h->S = analyze(m, adjptr, adj, perm, rp);
             phase("analyze");
             build_assembly_maps(h);
             phase("assembly maps");

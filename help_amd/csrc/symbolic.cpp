@@ -3,7 +3,9 @@
 #include "symbolic.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <numeric>
 #include <stdexcept>
@@ -364,6 +366,204 @@ std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
     for (int32_t i : dense_nodes) order.push_back(i);
     if ((int32_t)order.size() != n) throw std::runtime_error("amd_order: incomplete ordering");
     return order;
+}
+
+// ---------------------------------------------------------------------------
+// Nested dissection by level-structure bisection (George & Liu, "Computer
+// Solution of Large Sparse Positive Definite Systems", 1981, ch. 8): a BFS
+// level structure rooted at a pseudo-peripheral node, a separator level near
+// the middle chosen for small size, trimmed to the vertices that touch the
+// far side, the two sides dissected recursively and the separator numbered
+// after them.  Subgraphs at or below `leaf` vertices, and subgraphs whose
+// best separator exceeds half of them, are ordered by approximate minimum
+// degree.  Disconnected subgraphs are dissected per component.  Entirely
+// deterministic.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct NdWork {
+    const std::vector<int64_t>& ptr;
+    const std::vector<int32_t>& adj;
+    std::vector<int32_t> stamp, level, loc;
+    int32_t cur = 0;
+    int32_t leaf;
+    std::vector<int32_t> order;
+    NdWork(int32_t n, const std::vector<int64_t>& p, const std::vector<int32_t>& a, int32_t lf)
+        : ptr(p), adj(a), stamp(n, -1), level(n, -1), loc(n, -1), leaf(lf) {
+        order.reserve(n);
+    }
+
+    // AMD on the subgraph induced by `set`, appended to the order
+    void amd_append(const std::vector<int32_t>& set) {
+        const int32_t k = (int32_t)set.size();
+        if (k == 0) return;
+        if (k == 1) { order.push_back(set[0]); return; }
+        for (int32_t i = 0; i < k; ++i) loc[set[i]] = i;
+        std::vector<int64_t> lp(k + 1, 0);
+        std::vector<int32_t> la;
+        for (int32_t i = 0; i < k; ++i) {
+            const int32_t v = set[i];
+            for (int64_t e = ptr[v]; e < ptr[v + 1]; ++e) {
+                const int32_t w = adj[e];
+                if (loc[w] >= 0 && w != v) la.push_back(loc[w]);
+            }
+            lp[i + 1] = (int64_t)la.size();
+        }
+        std::vector<int32_t> o = amd_order(k, lp, la);
+        for (int32_t i = 0; i < k; ++i) order.push_back(set[o[i]]);
+        for (int32_t i = 0; i < k; ++i) loc[set[i]] = -1;
+    }
+
+    // BFS inside the vertices stamped `tag` from `root`; returns the levels
+    // (level[] set for the reached vertices) and the visit order
+    int32_t bfs(int32_t root, int32_t tag, std::vector<int32_t>& visit) {
+        visit.clear();
+        visit.push_back(root);
+        level[root] = 0;
+        stamp[root] = tag + 1;   // reached mark for this BFS
+        int32_t depth = 0;
+        for (size_t h = 0; h < visit.size(); ++h) {
+            const int32_t v = visit[h];
+            for (int64_t e = ptr[v]; e < ptr[v + 1]; ++e) {
+                const int32_t w = adj[e];
+                if (stamp[w] == tag) {
+                    stamp[w] = tag + 1;
+                    level[w] = level[v] + 1;
+                    depth = std::max(depth, level[w]);
+                    visit.push_back(w);
+                }
+            }
+        }
+        for (int32_t v : visit) stamp[v] = tag;   // restore membership
+        return depth;
+    }
+
+    int32_t new_tag() { cur += 2; return cur; }
+
+    void dissect(std::vector<int32_t> set) {
+        if ((int32_t)set.size() <= leaf) { amd_append(set); return; }
+        const int32_t tag = new_tag();
+        for (int32_t v : set) stamp[v] = tag;
+        // connected components, each dissected on its own
+        std::vector<int32_t> visit;
+        {
+            std::vector<std::vector<int32_t>> comps;
+            size_t covered = 0;
+            const int32_t seen = new_tag();
+            for (int32_t v : set) {
+                if (stamp[v] != tag) continue;
+                visit.clear();
+                visit.push_back(v);
+                stamp[v] = seen;
+                for (size_t h = 0; h < visit.size(); ++h) {
+                    const int32_t u = visit[h];
+                    for (int64_t e = ptr[u]; e < ptr[u + 1]; ++e)
+                        if (stamp[adj[e]] == tag) { stamp[adj[e]] = seen; visit.push_back(adj[e]); }
+                }
+                covered += visit.size();
+                comps.push_back(visit);
+                if (covered == set.size()) break;
+            }
+            if (comps.size() > 1) {
+                for (auto& c : comps) dissect(std::move(c));
+                return;
+            }
+            for (int32_t v : set) stamp[v] = tag;
+        }
+        // pseudo-peripheral root: repeat BFS from a minimum-degree vertex of
+        // the last level while the depth grows (at most 4 sweeps)
+        int32_t root = set[0];
+        int32_t depth = bfs(root, tag, visit);
+        for (int it = 0; it < 4; ++it) {
+            int32_t best = -1;
+            int64_t bdeg = INT64_MAX;
+            for (int32_t v : visit)
+                if (level[v] == depth) {
+                    const int64_t dg = ptr[v + 1] - ptr[v];
+                    if (dg < bdeg) { bdeg = dg; best = v; }
+                }
+            std::vector<int32_t> vis2;
+            const int32_t d2 = bfs(best, tag, vis2);
+            if (d2 <= depth) { if (d2 == depth) { root = best; visit.swap(vis2); } break; }
+            root = best;
+            depth = d2;
+            visit.swap(vis2);
+        }
+        (void)root;
+        if (depth < 2) { amd_append(set); return; }
+        std::vector<int64_t> cnt(depth + 1, 0);
+        for (int32_t v : visit) cnt[level[v]]++;
+        // separator level: the smallest level whose both sides hold at least
+        // 30% of the vertices (ties to the one nearest the middle)
+        const int64_t tot = (int64_t)visit.size();
+        int64_t before = 0;
+        int32_t sl = -1;
+        int64_t sbest = INT64_MAX, mid_dist = INT64_MAX;
+        for (int32_t l = 0; l <= depth; ++l) {
+            const int64_t after = tot - before - cnt[l];
+            if (l > 0 && l < depth && 10 * before >= 3 * tot && 10 * after >= 3 * tot) {
+                const int64_t md = std::llabs(2 * before + cnt[l] - tot);
+                if (cnt[l] < sbest || (cnt[l] == sbest && md < mid_dist)) { sbest = cnt[l]; sl = l; mid_dist = md; }
+            }
+            before += cnt[l];
+        }
+        if (sl < 0) {   // no balanced level: take the median level
+            before = 0;
+            for (int32_t l = 0; l <= depth; ++l) {
+                if (2 * (before + cnt[l]) >= tot) { sl = std::min(std::max(l, 1), depth - 1); break; }
+                before += cnt[l];
+            }
+        }
+        // sides: 0 = near (level < sl), 1 = separator, 2 = far (level > sl);
+        // separator vertices without a far neighbour move to the near side,
+        // then those without a near neighbour move to the far side
+        std::vector<int32_t> sep, near_side, far_side;
+        for (int32_t v : visit) {
+            if (level[v] == sl) sep.push_back(v);
+        }
+        auto side = [&](int32_t v) { return level[v] < sl ? 0 : (level[v] == sl ? 1 : 2); };
+        std::vector<int32_t> keep;
+        for (int32_t v : sep) {
+            bool far = false;
+            for (int64_t e = ptr[v]; e < ptr[v + 1] && !far; ++e)
+                far = stamp[adj[e]] == tag && side(adj[e]) == 2;
+            if (!far) level[v] = sl - 1;   // near side
+            else keep.push_back(v);
+        }
+        sep.clear();
+        for (int32_t v : keep) {
+            bool nearn = false;
+            for (int64_t e = ptr[v]; e < ptr[v + 1] && !nearn; ++e)
+                nearn = stamp[adj[e]] == tag && side(adj[e]) == 0;
+            if (!nearn) level[v] = sl + 1;   // far side
+            else sep.push_back(v);
+        }
+        if (2 * sep.size() > set.size()) { amd_append(set); return; }
+        for (int32_t v : visit) {
+            const int s = side(v);
+            if (s == 0) near_side.push_back(v);
+            else if (s == 2) far_side.push_back(v);
+        }
+        std::sort(near_side.begin(), near_side.end());
+        std::sort(far_side.begin(), far_side.end());
+        std::sort(sep.begin(), sep.end());
+        set.clear();
+        set.shrink_to_fit();
+        dissect(std::move(near_side));
+        dissect(std::move(far_side));
+        amd_append(sep);
+    }
+};
+
+}  // namespace
+
+std::vector<int32_t> nd_order(int32_t n, const std::vector<int64_t>& adjptr,
+                              const std::vector<int32_t>& adj, int32_t leaf) {
+    NdWork w(n, adjptr, adj, std::max(leaf, 16));
+    std::vector<int32_t> all(n);
+    for (int32_t i = 0; i < n; ++i) all[i] = i;
+    w.dissect(std::move(all));
+    return w.order;
 }
 
 std::vector<int32_t> etree(int32_t m, const std::vector<int64_t>& adjptr,

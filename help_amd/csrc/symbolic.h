@@ -47,6 +47,11 @@ AdatPattern adat_pattern(const CscMatrix& A, const CsrMatrix& R, bool with_pairs
 std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
                                const std::vector<int32_t>& adj, double dense_alpha = 10.0);
 
+// Nested dissection by level-structure bisection, subgraphs of at most
+// `leaf` vertices ordered by amd_order.  Returns perm (new -> old).
+std::vector<int32_t> nd_order(int32_t n, const std::vector<int64_t>& adjptr,
+                              const std::vector<int32_t>& adj, int32_t leaf = 2048);
+
 struct Supernodal {
     int32_t m = 0;
     std::vector<int32_t> perm, iperm;        // new -> old, old -> new

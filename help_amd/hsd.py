@@ -6,6 +6,8 @@ the same four callbacks (construct/destruct/renew/solve).
 """
 import ctypes as C
 
+import numpy as np
+
 from . import _lib as L
 from .backend import HelpError
 
@@ -73,3 +75,52 @@ def solve_lp(inst, table, max_iter=0, log=None, exact=False, device=False):
     if st != L.HELP_OK:
         raise HelpError(st, f"help_hsd_solve failed (backend error {res.backend_error})")
     return res, rows
+
+
+class RecordingBackend:
+    """A help_backend table over a ``LinearEquation`` that hands every solve
+    to ``on_solve(renew_count, k, d, b1, b2, x1, x2)`` (k = solves since the
+    last renew; the arrays are views valid during the call).  HELP's driver
+    calls SolveLinearEquation first for the bottom system of an iteration
+    (HSDSolver.cpp:770), so k == 0 picks that system.  Used to harvest the
+    (d, b1, b2) of a real interior-point trajectory."""
+
+    def __init__(self, inst, on_solve, **opts):
+        from .backend import LinearEquation
+        self.inst = inst
+        self.le = LinearEquation(**opts)
+        self.renews = 0
+        self._k = 0
+        n, m = inst.n, inst.m
+
+        def construct(ctx, lp):
+            self.le.construct(self.inst)
+            return 0
+
+        def destruct(ctx):
+            self.le.destruct()
+            return 0
+
+        def renew(ctx, dp):
+            self.le.renew(np.ctypeslib.as_array(dp, shape=(n,)).copy())
+            self.renews += 1
+            self._k = 0
+            return 0
+
+        def solve(ctx, dp, b1p, b2p, x1p, x2p):
+            d = np.ctypeslib.as_array(dp, shape=(n,))
+            b1 = np.ctypeslib.as_array(b1p, shape=(n,))
+            b2 = np.ctypeslib.as_array(b2p, shape=(m,))
+            x1, x2 = self.le.solve(d, b1, b2)
+            np.ctypeslib.as_array(x1p, shape=(n,))[:] = x1
+            np.ctypeslib.as_array(x2p, shape=(m,))[:] = x2
+            on_solve(self.renews, self._k, d, b1, b2, x1, x2)
+            self._k += 1
+            return 0
+
+        self._fns = (L.CONSTRUCT_FN(construct), L.DESTRUCT_FN(destruct), L.RENEW_FN(renew),
+                     L.SOLVE_FN(solve))
+        self.table = L.Backend()
+        self.table.ctx = None
+        for name, f in zip(("construct", "destruct", "renew", "solve"), self._fns):
+            setattr(self.table, name, C.cast(f, C.c_void_p))

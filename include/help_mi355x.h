@@ -75,7 +75,15 @@ typedef struct help_opts {
                                 (formed during the factorization on a second
                                 stream): 0 or 1 on, -1 off (block substitution
                                 with the 64x64 diagonal-block inverses)        */
+    int32_t ordering;        /* fill-reducing ordering of A D^-1 A^T:
+                                HELP_ORDER_AMD (CHOLMOD's setting in HELP,
+                                Setting_CPU.cpp:21-26) or HELP_ORDER_ND (nested
+                                dissection, the family of the METIS ordering
+                                HELP's PARDISO backend asks for,
+                                UsePardiso.cpp:101)                            */
 } help_opts;
+#define HELP_ORDER_AMD 0
+#define HELP_ORDER_ND  1
 
 /* Defaults: FP64 direct, beta=1e-5, refine 20 / 1e-12 (FP32 mode; FP64 mode
  * refine_fp64 0), pcg_k 500, pcg_max_iter 5000, pcg_tol 1e-10, FP32

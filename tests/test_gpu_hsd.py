@@ -1,0 +1,262 @@
+"""HSD-level GPU parity: bounded and free columns, real interior-point
+trajectories, and the staircase family in both precisions.
+
+Bars (the same as test_gpu_parity.py):
+* HSD end to end: the oracle's status, iterations within +-1, objective
+  within 1e-8 relative.
+* Solves taken from a real IPM trajectory: dy (= x2) forward error against
+  the CPU FP64 oracle's solve of the same (d, b1, b2), and the backward error
+  ||r - (A D^-1 A^T + beta I) x2|| / ||r|| (FP64, on the host), with the
+  bound each test states.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_bind as O
+from help_amd import LinearEquation, instances
+from help_amd import _lib as L
+from help_amd.hsd import GpuBackend, RecordingBackend, solve_lp
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = json.load(open(os.path.join(HERE, "golden", "hsd_objectives.json")))
+
+
+def _flush(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_run(inst):
+    be = LinearEquation(symbolic_only=True)
+    be.construct(inst)
+    perm, _, _ = be.symbolic_arrays()
+    be.destruct()
+    ob = O.OracleBackend(1e-5, perm)
+    try:
+        ref, _ = solve_lp(inst, ob.table)
+    finally:
+        ob.close()
+    return ref, perm
+
+
+@pytest.mark.parametrize("name", ["random_300x600_ub150_s1010", "random_300x600_fr20_s1010",
+                                  "random_300x600_ub100_fr2_s1013"])
+@pytest.mark.parametrize("device", [True, False])
+def test_hsd_bounded_free(name, device):
+    """Upper-bounded columns (D += D_u, the n_UB residual terms and the
+    (p_u, q_u) block: HSDSolver.cpp:70-82,99-100,160-165,203-207) and free
+    columns (D = rho xf: :84-90) through the GPU-resident driver
+    (hsd_gpu.hip) and the host driver over the GPU backend, against the
+    oracle's run and the HiGHS objective."""
+    spec = FIX[name]
+    inst = instances.random_lp(**spec["params"])
+    _flush(f"hsd bounded/free {name} n_ub={inst.n_ub} n_fr={inst.n_fr} device_driver={device}")
+    gb = GpuBackend(mode=L.MODE_DIRECT_FP64)
+    try:
+        res, _ = solve_lp(inst, gb.table, device=device)
+    finally:
+        gb.close()
+    ref, _ = _oracle_run(inst)
+    _flush(f"  gpu {res.status} {res.iterations} {res.primal_obj!r} | oracle {ref.status} {ref.iterations} "
+           f"{ref.primal_obj!r} | highs {spec['objective']!r}")
+    assert res.backend_error == 0
+    assert res.status == ref.status == 0
+    assert abs(res.iterations - ref.iterations) <= 1
+    assert abs(res.primal_obj - ref.primal_obj) / (1 + abs(ref.primal_obj)) <= 1e-8
+    assert abs(res.primal_obj - spec["objective"]) / (1 + abs(spec["objective"])) <= 1e-7
+
+
+def test_hsd_bounded_free_stall_matches_oracle():
+    """An instance on which HELP's own driver (and the oracle) stop on the
+    inaccurate-direction test: the GPU-resident driver stops the same way."""
+    inst = instances.random_lp(300, 600, 4, seed=1011, ub_cols=150, free_cols=5)
+    gb = GpuBackend(mode=L.MODE_DIRECT_FP64)
+    try:
+        res, _ = solve_lp(inst, gb.table, device=True)
+    finally:
+        gb.close()
+    ref, _ = _oracle_run(inst)
+    _flush(f"  gpu {res.status}/{res.stall_reason} {res.iterations} | oracle {ref.status}/{ref.stall_reason} "
+           f"{ref.iterations}")
+    assert res.status == ref.status == 1 and res.stall_reason == ref.stall_reason == 2
+    assert abs(res.iterations - ref.iterations) <= 1
+
+
+def _normal_residual(inst, d, b1, b2, x2, beta=1e-5):
+    A = inst.to_scipy()
+    rhs = A @ (b1 / d) - b2
+    r = rhs - (A @ ((A.T @ x2) / d) + beta * x2)
+    return np.linalg.norm(r) / np.linalg.norm(rhs)
+
+
+def _trajectory_errors(inst, capture, inverse_solve):
+    records = {}
+
+    def keep(renews, k, d, b1, b2, x1, x2):
+        if k == 0 and renews in capture:
+            records[renews] = (d.copy(), b1.copy(), b2.copy(), x2.copy())
+
+    rec = RecordingBackend(inst, keep, inverse_solve=inverse_solve)
+    res, _ = solve_lp(inst, rec.table)
+    _flush(f"  host driver over the GPU backend: status {res.status} iterations {res.iterations}")
+    be = LinearEquation(symbolic_only=True)
+    be.construct(inst)
+    perm, _, _ = be.symbolic_arrays()
+    be.destruct()
+    arp, aci = O.adat_pattern(inst)
+    _, _, Lp, Li = O.symbolic(inst.m, arp, aci, perm)
+    out = []
+    for k in sorted(records):
+        d, b1, b2, x2 = records[k]
+        aval = O.adat_values(inst, d, arp, aci)
+        st, Lx = O.cholesky(inst.m, arp, aci, aval, 1e-5, perm, Lp, Li)
+        assert st == 0
+        _, o2 = O.solve_kkt(inst, perm, Lp, Li, Lx, d, b1, b2)
+        fwd = np.linalg.norm(x2 - o2) / np.linalg.norm(o2)
+        bwd_g = _normal_residual(inst, d, b1, b2, x2)
+        bwd_o = _normal_residual(inst, d, b1, b2, o2)
+        dec = np.log10(d.max() / d.min())
+        _flush(f"  renew {k}: d spans {dec:.1f} decades; dy forward err vs oracle {fwd:.3e}; "
+               f"backward err gpu {bwd_g:.3e} oracle {bwd_o:.3e}")
+        out.append((k, dec, fwd, bwd_g, bwd_o))
+    return res, out
+
+
+@pytest.mark.parametrize("inverse_solve", [1, -1])
+def test_ipm_trajectory_solves_mcf(inverse_solve):
+    """Solves harvested from a real HSD run on a mid-size multicommodity-flow
+    LP (the benchmark's family), early, middle and late, compared with the
+    oracle's FP64 solve of the same system (UseCholmod.cpp:157-238
+    semantics).  Early and middle solves must agree to 1e-10; late ones,
+    where d spans many decades and cond(A D^-1 A^T) ~ 1/eps, must have a
+    backward error within 10x of the oracle's own."""
+    inst = instances.mcf(side=40, commodities=6, pairs=20, seed=90)
+    _flush(f"ipm trajectory mcf 40x40 K=6 m={inst.m} n={inst.n} inverse_solve={inverse_solve}")
+    res, errs = _trajectory_errors(inst, (1, 10, 20, 30, 40), inverse_solve)
+    assert res.status == 0
+    assert len(errs) >= 4
+    for k, dec, fwd, bwd_g, bwd_o in errs:
+        if k <= 20:
+            assert fwd <= 1e-10, (k, fwd)
+        assert bwd_g <= max(10 * bwd_o, 1e-14), (k, bwd_g, bwd_o)
+
+
+def test_ipm_trajectory_solves_mcf150():
+    """The same on the full 150x150-grid, K=8 instance (m=269,392,
+    n=804,600): the bottom-system solves of iterations 10, 100 and 180 of
+    the host driver's run, each compared with the oracle's FP64 solve of
+    that system."""
+    inst = instances.mcf(150, 8, 20, 0)
+    _flush(f"ipm trajectory mcf 150x150 K=8 m={inst.m} n={inst.n}")
+    res, errs = _trajectory_errors(inst, (10, 100, 180), 1)
+    assert len(errs) == 3
+    for k, dec, fwd, bwd_g, bwd_o in errs:
+        if k <= 10:
+            assert fwd <= 1e-10, (k, fwd)
+        assert bwd_g <= max(10 * bwd_o, 1e-14), (k, bwd_g, bwd_o)
+
+
+@pytest.mark.parametrize("mode", [L.MODE_DIRECT_FP64, L.MODE_DIRECT_FP32])
+def test_nd_ordering_solve_matches_oracle(mode):
+    """The nested-dissection ordering feeds the same kernels as AMD: the
+    solve of a staircase LP (m=3,600) under it equals the oracle's solve
+    with that permutation to 1e-10."""
+    inst = instances.staircase(12, 300, 900, 5, seed=21)
+    _flush(f"nd ordering staircase 12x300 m={inst.m} mode={mode}")
+    be = LinearEquation(mode=mode, ordering=L.ORDER_ND)
+    be.construct(inst)
+    perm, _, _ = be.symbolic_arrays()
+    rng = np.random.default_rng(5)
+    d = np.exp(rng.uniform(-2, 2, inst.n) * np.log(10.0))
+    b1, b2 = rng.standard_normal(inst.n), rng.standard_normal(inst.m)
+    be.renew(d)
+    x1, x2 = be.solve(d, b1, b2)
+    be.destruct()
+    arp, aci = O.adat_pattern(inst)
+    aval = O.adat_values(inst, d, arp, aci)
+    _, _, Lp, Li = O.symbolic(inst.m, arp, aci, perm)
+    st, Lx = O.cholesky(inst.m, arp, aci, aval, 1e-5, perm, Lp, Li)
+    assert st == 0
+    o1, o2 = O.solve_kkt(inst, perm, Lp, Li, Lx, d, b1, b2)
+    e2 = np.linalg.norm(x2 - o2) / np.linalg.norm(o2)
+    e1 = np.linalg.norm(x1 - o1) / np.linalg.norm(o1)
+    _flush(f"  dy err {e2:.3e} dx err {e1:.3e}")
+    assert e2 <= 1e-10 and e1 <= 1e-10
+
+
+@pytest.mark.parametrize("mode,ordering", [(L.MODE_DIRECT_FP64, L.ORDER_AMD), (L.MODE_DIRECT_FP64, L.ORDER_ND),
+                                           (L.MODE_DIRECT_FP32, L.ORDER_AMD), (L.MODE_DIRECT_FP32, L.ORDER_ND)])
+def test_hsd_staircase_reduced(mode, ordering):
+    """The benchmark's staircase family at reduced size (T=12 periods of
+    300 x 900; m=3,600, n=14,400) through the GPU-resident driver, in both
+    precisions and both orderings, against the oracle's run (FP64, same
+    ordering): status, iterations +-1, objective 1e-8."""
+    inst = instances.staircase(12, 300, 900, 5, seed=22)
+    _flush(f"hsd staircase 12x300 m={inst.m} n={inst.n} mode={mode} ordering={ordering}")
+    gb = GpuBackend(mode=mode, ordering=ordering)
+    try:
+        res, _ = solve_lp(inst, gb.table, device=True)
+        n_sol, n_ref, max_rel = gb.solve_stats()
+    finally:
+        gb.close()
+    be = LinearEquation(symbolic_only=True, ordering=ordering)
+    be.construct(inst)
+    perm, _, _ = be.symbolic_arrays()
+    be.destruct()
+    ob = O.OracleBackend(1e-5, perm)
+    try:
+        ref, _ = solve_lp(inst, ob.table)
+    finally:
+        ob.close()
+    _flush(f"  gpu {res.status} {res.iterations} {res.primal_obj!r} (refinement steps {n_ref}, max rel "
+           f"residual {max_rel:.2e}) | oracle {ref.status} {ref.iterations} {ref.primal_obj!r}")
+    assert res.backend_error == 0
+    assert res.status == ref.status == 0
+    assert abs(res.iterations - ref.iterations) <= 1
+    assert abs(res.primal_obj - ref.primal_obj) / (1 + abs(ref.primal_obj)) <= 1e-8
+
+
+@pytest.mark.parametrize("mode", [L.MODE_DIRECT_FP64, L.MODE_DIRECT_FP32])
+def test_ipm_trajectory_staircase_t10(mode):
+    """Solves from a real HSD run on the staircase with the benchmark's
+    1000 x 3000 period blocks (T=10: m=10,000, n=40,000), nested dissection,
+    in both precisions: the bottom-system solves of the first, middle and
+    last iterations against the oracle's FP64 solve of the same system."""
+    inst = instances.staircase(10, 1000, 3000, 5, seed=0)
+    _flush(f"ipm trajectory staircase T=10 m={inst.m} n={inst.n} mode={mode}")
+    records = {}
+
+    def keep(renews, k, d, b1, b2, x1, x2):
+        if k == 0 and renews in (1, 12, 20):
+            records[renews] = (d.copy(), b1.copy(), b2.copy(), x2.copy())
+
+    rec = RecordingBackend(inst, keep, mode=mode, ordering=L.ORDER_ND)
+    res, _ = solve_lp(inst, rec.table)
+    _flush(f"  status {res.status} iterations {res.iterations}")
+    be = LinearEquation(symbolic_only=True, ordering=L.ORDER_ND)
+    be.construct(inst)
+    perm, _, _ = be.symbolic_arrays()
+    be.destruct()
+    arp, aci = O.adat_pattern(inst)
+    _, _, Lp, Li = O.symbolic(inst.m, arp, aci, perm)
+    assert res.status == 0 and len(records) >= 2
+    for k in sorted(records):
+        d, b1, b2, x2 = records[k]
+        aval = O.adat_values(inst, d, arp, aci)
+        st, Lx = O.cholesky(inst.m, arp, aci, aval, 1e-5, perm, Lp, Li)
+        assert st == 0
+        _, o2 = O.solve_kkt(inst, perm, Lp, Li, Lx, d, b1, b2)
+        fwd = np.linalg.norm(x2 - o2) / np.linalg.norm(o2)
+        bwd_g = _normal_residual(inst, d, b1, b2, x2)
+        bwd_o = _normal_residual(inst, d, b1, b2, o2)
+        _flush(f"  renew {k}: d spans {np.log10(d.max() / d.min()):.1f} decades; dy err vs oracle {fwd:.3e}; "
+               f"backward err gpu {bwd_g:.3e} oracle {bwd_o:.3e}")
+        assert bwd_g <= max(10 * bwd_o, 1e-12), (k, bwd_g, bwd_o)
+        if k == 1:
+            assert fwd <= 1e-10, (k, fwd)

@@ -111,3 +111,38 @@ def test_fill_is_close_to_mmd():
     lu = spla.splu(M, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
                    options=dict(SymmetricMode=True))
     assert nnz_l <= 1.3 * lu.L.nnz
+
+
+def _analyze_nd(inst):
+    be = LinearEquation(symbolic_only=True, ordering=1)
+    be.construct(inst)
+    return be
+
+
+def test_nd_ordering_is_valid_and_deterministic():
+    inst = instances.staircase(8, 400, 1200, 5, seed=11)
+    p1, _, _ = _analyze_nd(inst).symbolic_arrays()
+    p2, _, _ = _analyze_nd(inst).symbolic_arrays()
+    assert np.array_equal(np.sort(p1), np.arange(inst.m))
+    assert np.array_equal(p1, p2)
+
+
+def test_nd_etree_and_colcounts_match_structure_merging():
+    inst = instances.staircase(8, 400, 1200, 5, seed=12)
+    be = _analyze_nd(inst)
+    perm, parent, cc = be.symbolic_arrays()
+    arp, aci = O.adat_pattern(inst)
+    par_o, cc_o, Lp, Li = O.symbolic(inst.m, arp, aci, perm)
+    assert np.array_equal(parent, par_o)
+    assert np.array_equal(cc, cc_o)
+
+
+def test_nd_shortens_the_staircase_elimination_tree():
+    """On a chain of period blocks, dissection gives an elimination tree of
+    logarithmic depth where minimum degree gives one about as deep as the
+    chain, here (1000-row periods) with less fill."""
+    inst = instances.staircase(40, 1000, 3000, 5, seed=13)
+    amd = _analyze(inst).symbolic_info()
+    nd = _analyze_nd(inst).symbolic_info()
+    assert nd.n_levels * 3 <= amd.n_levels
+    assert nd.nnz_l_exact <= amd.nnz_l_exact
