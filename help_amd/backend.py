@@ -165,6 +165,13 @@ class LinearEquation:
                                                 L.dptr(Lf)), "pcg_factor")
         return perm, D[:k], S, Lf[:self.m * k].reshape(k, self.m).T.copy()
 
+    def pcg_apply(self, r):
+        """PCG mode: M^-1 r with the preconditioner of the last renew."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty(self.m)
+        self._check(L.lib().help_gpu_pcg_apply(self.h, L.dptr(r), L.dptr(z)), "pcg_apply")
+        return z
+
     def adat_pattern(self):
         info = self.symbolic_info()
         rp = np.empty(self.m + 1, np.int64)

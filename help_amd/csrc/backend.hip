@@ -1074,6 +1074,7 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
         hipStream_t st = h->st;
         const int32_t m = h->m, n = h->n;
         help_solve_stats loc{};
+        bool noconv = false;
         HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * n, hipMemcpyDefault, st));
         HIPCHK(hipMemcpyAsync(h->d_b1.p, b1, sizeof(double) * n, hipMemcpyDefault, st));
         HIPCHK(hipMemcpyAsync(h->d_b2.p, b2, sizeof(double) * m, hipMemcpyDefault, st));
@@ -1119,6 +1120,7 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
             }
             loc.pcg_iters = it;
             loc.rel_residual = rel;
+            noconv = !(rel <= h->opts.pcg_tol);
         } else {
             // the factor's solution, refined by FP64 conjugate gradients on
             // (A D^-1 A^T + beta I) preconditioned with the same factor; the
@@ -1164,6 +1166,11 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
         loc.ms_solve = elapsed_ms(h->ev[1], h->ev[2]);
         loc.ms_back = elapsed_ms(h->ev[2], h->ev[3]);
         if (ss) *ss = loc;
+        if (noconv) {
+            h->err = "solve: PCG stopped at pcg_max_iter = " + std::to_string(h->opts.pcg_max_iter) +
+                     " with relative residual " + std::to_string(loc.rel_residual) + " > pcg_tol";
+            return HELP_ERR_NOCONV;
+        }
         return HELP_OK;
     } catch (const std::exception& e) {
         return fail(h, e);
@@ -1178,8 +1185,12 @@ int help_gpu_solve2(help_gpu* h, const double* d, const double* b1a, const doubl
     const int max_it = h->fp32_off ? h->opts.refine_max : h->opts.refine_fp64;
     if (h->use_pcg || h->factor_fp32 || max_it > 0) {
         // iterative paths decide per right-hand side: two single solves
-        int st = help_gpu_solve(h, d, b1a, b2a, x1a, x2a, ssa);
-        return st != HELP_OK ? st : help_gpu_solve(h, d, b1b, b2b, x1b, x2b, ssb);
+        // (a PCG solve stopped at its cap still leaves an iterate: the
+        // second system is solved too and the first failure is returned)
+        const int sa = help_gpu_solve(h, d, b1a, b2a, x1a, x2a, ssa);
+        if (sa != HELP_OK && sa != HELP_ERR_NOCONV) return sa;
+        const int sb = help_gpu_solve(h, d, b1b, b2b, x1b, x2b, ssb);
+        return sa != HELP_OK ? sa : sb;
     }
     try {
         hipStream_t st = h->st;
@@ -1320,6 +1331,22 @@ int help_gpu_pcg_factor(help_gpu* h, int32_t* K, int32_t* perm, double* D, doubl
     if (!h->use_pcg || !h->device || !h->factored) { h->err = "pcg_factor: no PCG preconditioner"; return HELP_ERR_STATE; }
     try {
         help::pcg_download(h, K, perm, D, S, L);
+        return HELP_OK;
+    } catch (const std::exception& e) {
+        return fail(h, e);
+    }
+}
+
+int help_gpu_pcg_apply(help_gpu* h, const double* r, double* z) {
+    if (!h || !r || !z) return HELP_ERR_ARG;
+    if (!h->device || !h->use_pcg || !h->factored) { h->err = "pcg_apply: PCG mode after renew only"; return HELP_ERR_STATE; }
+    try {
+        hipStream_t st = h->st;
+        const int32_t m = h->m;
+        HIPCHK(hipMemcpyAsync(h->d_c.p, r, sizeof(double) * m, hipMemcpyDefault, st));
+        help::pcg_precond(h, h->d_c.p, h->d_z.p);
+        HIPCHK(hipMemcpyAsync(z, h->d_z.p, sizeof(double) * m, hipMemcpyDefault, st));
+        HIPCHK(hipStreamSynchronize(st));
         return HELP_OK;
     } catch (const std::exception& e) {
         return fail(h, e);
@@ -1483,7 +1510,7 @@ static int gb_solve(void* ctx, const double* d, const double* b1, const double* 
     auto* c = (GpuBackendCtx*)ctx;
     help_solve_stats ss{};
     int st = help_gpu_solve(c->h, d, b1, b2, x1, x2, &ss);
-    if (st == HELP_OK) {
+    if (st == HELP_OK || st == HELP_ERR_NOCONV) {
         c->n_solve += 1;
         c->refine_steps += ss.refine_steps;
         c->pcg_iters += ss.pcg_iters;
@@ -1499,7 +1526,7 @@ int help_gpu_backend_solve2(help_backend* be, const double* d, const double* b1a
     auto* c = (GpuBackendCtx*)be->ctx;
     help_solve_stats ss[2]{};
     int st = help_gpu_solve2(c->h, d, b1a, b2a, x1a, x2a, b1b, b2b, x1b, x2b, &ss[0], &ss[1]);
-    if (st == HELP_OK) {
+    if (st == HELP_OK || st == HELP_ERR_NOCONV) {
         for (const help_solve_stats& s : ss) {
             c->n_solve += 1;
             c->refine_steps += s.refine_steps;

@@ -537,6 +537,10 @@ struct Hsd {
         mu = mu0;
         int it;
         for (it = 0; it < max_iter; ++it) {
+            // a failed backend call (e.g. PCG stopped at its cap short of
+            // its tolerance) ends the run: the reference would exit in
+            // CheckError (Helper.cpp:24-35)
+            if (backend_error) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_BACKEND; break; }
             rho = std::max(mu / mu0, 1e-12);
             for (int i = 0; i < n_fr; ++i) xf[i] = 1.0 / (1.0 + 10.0 * std::fabs(x[n_lb + i]));
             newton_parameters();

@@ -724,6 +724,10 @@ struct HsdGpu {
         double* xfp = xf.p;
         const int nlb = n_lb;
         for (it = 0; it < max_iter; ++it) {
+            // a failed backend call (e.g. PCG stopped at its cap short of
+            // its tolerance) ends the run: the reference would exit in
+            // CheckError (Helper.cpp:24-35)
+            if (backend_error) { status = HELP_HSD_STALLED; stall_reason = HELP_STALL_BACKEND; break; }
             rho = std::max(mu / mu0, 1e-12);
             map(n_fr, [=] __device__(int64_t i) { xfp[i] = 1.0 / (1.0 + 10.0 * fabs(xp[nlb + i])); });
             newton_parameters();

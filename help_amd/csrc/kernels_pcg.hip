@@ -335,9 +335,11 @@ __global__ __launch_bounds__(256) void k_pca_y1(int32_t K, const int32_t* __rest
     if (lane == 0) y1[i] = s;
 }
 
-// z_j = (r_j - sum_c L(j,c) y1_c) / S_j off the pivots, 0 on them
+// z_j = (r_j - sum_c L(j,c) y1_c) / S_j off the pivots, 0 on them; the
+// factor in FP32 (the default) or FP64, the sums in FP64
+template <typename T>
 __global__ __launch_bounds__(256) void k_pca_t2(int32_t m, int32_t K, const int32_t* __restrict__ pos,
-                                                const float* __restrict__ L32, const double* __restrict__ y1g,
+                                                const T* __restrict__ L32, const double* __restrict__ y1g,
                                                 const double* __restrict__ S, const double* __restrict__ r,
                                                 double* __restrict__ z) {
     __shared__ double y1[kPcgMaxK];
@@ -360,7 +362,8 @@ __global__ __launch_bounds__(256) void k_pca_t2(int32_t m, int32_t K, const int3
 
 // ltpart[c * nb + b] = sum over the kLtRows rows of block b of L(j,c) z_j
 // (z is 0 on the pivots); a wave per column, 32 rows per lane in flight
-__global__ __launch_bounds__(256) void k_pca_lt(int32_t m, int32_t K, const float* __restrict__ L32,
+template <typename T>
+__global__ __launch_bounds__(256) void k_pca_lt(int32_t m, int32_t K, const T* __restrict__ L32,
                                                 const double* __restrict__ z, double* __restrict__ ltpart) {
     __shared__ double zs[kLtRows];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -369,7 +372,7 @@ __global__ __launch_bounds__(256) void k_pca_lt(int32_t m, int32_t K, const floa
     __syncthreads();
     const int64_t last = (int64_t)m - 1;
     for (int c = wave; c < K; c += 4) {
-        const float* Lc = L32 + (int64_t)c * m;
+        const T* Lc = L32 + (int64_t)c * m;
         double a[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int u = 0; u < kLtRows / 64; ++u) {
@@ -494,19 +497,27 @@ void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st) {
     if (n > 0) help::launch_kernel(k_to_f32, grid_for(n), kB, 0, st, n, x, y);
 }
 
-void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* pos, const float* L32,
+template <typename T>
+void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* pos, const T* L32,
                      const double* X, const double* S, const double* r, double* y1, double* ltpart, double* z,
                      hipStream_t st) {
     // K = 0: z = r / S
     const int nb = (int)(((int64_t)m + kLtRows - 1) / kLtRows);
     if (K > 0) help::launch_kernel(k_pca_y1, (K + 3) / 4, kB, 0, st, K, perm, X, r, y1);
-    help::launch_kernel(k_pca_t2, grid_for(m), kB, 0, st, m, K, pos, L32, y1, S, r, z);
+    help::launch_kernel(k_pca_t2<T>, grid_for(m), kB, 0, st, m, K, pos, L32, y1, S, r, z);
     if (K > 0) {
-        help::launch_kernel(k_pca_lt, nb, kB, 0, st, m, K, L32, z, ltpart);
+        help::launch_kernel(k_pca_lt<T>, nb, kB, 0, st, m, K, L32, z, ltpart);
         double* w = y1 + kPcgMaxK;
         help::launch_kernel(k_pca_w, (K + 3) / 4, kB, 0, st, K, nb, perm, y1, S, ltpart, w);
         help::launch_kernel(k_pca_final, grid_for(K), kB, 0, st, K, perm, X, w, z);
     }
 }
+
+template void launch_pc_apply<float>(int32_t, int32_t, const int32_t*, const int32_t*, const float*,
+                                      const double*, const double*, const double*, double*, double*, double*,
+                                      hipStream_t);
+template void launch_pc_apply<double>(int32_t, int32_t, const int32_t*, const int32_t*, const double*,
+                                       const double*, const double*, const double*, double*, double*, double*,
+                                       hipStream_t);
 
 }  // namespace help

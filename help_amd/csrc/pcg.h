@@ -66,8 +66,10 @@ void launch_pc_l11(int32_t m, int32_t K, const int32_t* perm, const double* L, d
 void launch_to_f32(int64_t n, const double* x, float* y, hipStream_t st);
 
 // ---- preconditioner apply z = M^-1 r (PCG.cpp:183-196) ----
-// y1: scratch of 2 * kPcgMaxK doubles; ltpart: K * ceil(m / kLtRows) doubles
-void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* pos, const float* L32,
+// y1: scratch of 2 * kPcgMaxK doubles; ltpart: K * ceil(m / kLtRows) doubles;
+// the m x K factor L32 in FP32 or FP64 (T = float / double)
+template <typename T>
+void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* pos, const T* L32,
                      const double* X, const double* S, const double* r, double* y1, double* ltpart,
                      double* z, hipStream_t st);
 

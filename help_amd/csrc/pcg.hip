@@ -31,7 +31,7 @@ void pcg_setup(help_gpu* h) {
     P->K = std::max(0, std::min<int32_t>(std::min(h->opts.pcg_k, kPcgMaxK), m));
     const int64_t K = std::max<int32_t>(P->K, 1);
     P->L.alloc((size_t)m * K);
-    P->L32.alloc((size_t)m * K);
+    if (h->opts.pcg_fp32_precond) P->L32.alloc((size_t)m * K);
     P->D.alloc(K);
     P->w.alloc(K);
     P->S.alloc(m);
@@ -79,7 +79,7 @@ void pcg_renew(help_gpu* h) {
         launch_pc_column(m, k, P->col.p, P->pos.p, P->g.p, P->w.p, P->D.p, P->L.p, P->S.p, st);
     }
     launch_pc_l11(m, K, P->perm.p, P->L.p, P->L11.p, P->X.p, st);
-    launch_to_f32((int64_t)m * K, P->L.p, P->L32.p, st);
+    if (h->opts.pcg_fp32_precond) launch_to_f32((int64_t)m * K, P->L.p, P->L32.p, st);
     HIPCHK(hipGetLastError());
 }
 
@@ -92,11 +92,12 @@ double pcg_build_bytes(const help_gpu* h) {
 }
 
 // per iteration: A^T p scaled (12 nnz + 8 m + 16 n), A t (12 nnz + 8 n + 8 m),
-// the preconditioner (two passes over the FP32 m x K factor, 8 m K) and
-// about ten m-vector streams
+// the preconditioner (two passes over the m x K factor: 8 m K in FP32,
+// 16 m K in FP64) and about ten m-vector streams
 double pcg_iteration_bytes(const help_gpu* h) {
     const double nnz = (double)h->nnz, m = h->m, n = h->n, K = h->pcg ? h->pcg->K : 0;
-    return 24.0 * nnz + 24.0 * n + 16.0 * m + 8.0 * m * K + 80.0 * m;
+    const double w = h->opts.pcg_fp32_precond ? 4.0 : 8.0;
+    return 24.0 * nnz + 24.0 * n + 16.0 * m + 2.0 * w * m * K + 80.0 * m;
 }
 
 static double dot(help_gpu* h, const double* a, const double* b, int64_t n) {
@@ -120,9 +121,7 @@ int pcg_solve(help_gpu* h, const double* rhs, const double* x0, double* x2, doub
         launch_csc_t_scaled(n, h->d_colptr.p, h->d_rowind.p, h->d_val.p, h->d_dinv.p, v, t, st);
         launch_csr_split(h->rows, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, t, out, false, st);
     };
-    auto precond = [&](const double* v, double* out) {
-        launch_pc_apply(m, P->K, P->perm.p, P->pos.p, P->L32.p, P->X.p, P->S.p, v, P->y1.p, P->ltpart.p, out, st);
-    };
+    auto precond = [&](const double* v, double* out) { pcg_precond(h, v, out); };
     const double bb = dot(h, rhs, rhs, m);
     if (x0) {
         HIPCHK(hipMemcpyAsync(x, x0, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
@@ -171,6 +170,18 @@ int pcg_solve(help_gpu* h, const double* rhs, const double* x0, double* x2, doub
     HIPCHK(hipMemcpyAsync(x2, x, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipGetLastError());
     return it;
+}
+
+// z = M^-1 v with the preconditioner of the last renew (device vectors,
+// original row order), as pcg_solve applies it
+void pcg_precond(help_gpu* h, const double* v, double* out) {
+    PcgState* P = h->pcg;
+    if (h->opts.pcg_fp32_precond)
+        launch_pc_apply<float>(h->m, P->K, P->perm.p, P->pos.p, P->L32.p, P->X.p, P->S.p, v, P->y1.p,
+                               P->ltpart.p, out, h->st);
+    else
+        launch_pc_apply<double>(h->m, P->K, P->perm.p, P->pos.p, P->L.p, P->X.p, P->S.p, v, P->y1.p,
+                                P->ltpart.p, out, h->st);
 }
 
 void pcg_download(help_gpu* h, int32_t* K, int32_t* perm, double* D, double* S, double* L) {

@@ -62,7 +62,8 @@ typedef struct help_opts {
     int32_t pcg_k;           /* partial-Cholesky columns (HELP PCG.h:20 uses 10)*/
     int32_t pcg_max_iter;    /* PCG iteration cap                               */
     double  pcg_tol;         /* PCG stop: ||r|| <= pcg_tol * ||r0||             */
-    int32_t pcg_fp32_precond;/* 1: apply the partial-Cholesky factor in FP32    */
+    int32_t pcg_fp32_precond;/* 1: apply the partial-Cholesky factor in FP32,
+                                0: in FP64 (vectors FP64 either way)          */
     int32_t fault_flags;     /* HELP_FAULT_*                                    */
     int32_t verbose;
     int32_t symbolic_only;   /* 1: analysis only, no device state (host tests)  */
@@ -126,7 +127,10 @@ int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* st);
 
 /* Replaces SolveLinearEquation (UseCholmod.cpp:157-238): d[n] (same values
  * as the last renew), b1[n], b2[m] in; x1[n], x2[m] out.  Host or device
- * pointers (unified addressing), here and in help_gpu_renew. */
+ * pointers (unified addressing), here and in help_gpu_renew.  In the PCG
+ * modes a solve that reaches pcg_max_iter short of pcg_tol writes its last
+ * iterate and returns HELP_ERR_NOCONV (the reference's PCG has no cap,
+ * PCG.h:19). */
 int help_gpu_solve(help_gpu* h, const double* d, const double* b1,
                    const double* b2, double* x1, double* x2,
                    help_solve_stats* st);
@@ -246,6 +250,10 @@ int  help_gpu_set_warmstart(help_gpu* h, help_mpnn* net);
  * factor columns, column-major by original row (L[c*m + row]).  Any output
  * pointer may be NULL. */
 int help_gpu_pcg_factor(help_gpu* h, int32_t* K, int32_t* perm, double* D, double* S, double* L);
+/* PCG modes: z[m] = M^-1 r[m] with the partial-Cholesky preconditioner of
+ * the last renew, applied as every CG iteration applies it (PCG.cpp:183-196:
+ * L^-T D_S^-1 L^-1 with L = [L11 0; L21 I]; FP32 factor, FP64 vectors). */
+int help_gpu_pcg_apply(help_gpu* h, const double* r, double* z);
 
 int help_gpu_profile(help_gpu* h, int32_t enable);
 int help_gpu_kernel_stats(const help_gpu* h, int32_t kind, double* ms, int64_t* launches,
@@ -343,6 +351,8 @@ typedef struct help_hsd_result {
 #define HELP_STALL_NONE        0
 #define HELP_STALL_NEGATIVE    1   /* x_j or z_j not positive (HSDSolver.cpp:63-66,77-80) */
 #define HELP_STALL_DIRECTION   2   /* inaccurate Newton direction (HSDSolver.cpp:814-818) */
+#define HELP_STALL_BACKEND     3   /* a backend call returned an error (backend_error),
+                                      e.g. HELP_ERR_NOCONV from PCG at pcg_max_iter */
 
 /* Optional per-iteration log: called after each Newton-parameter update
  * with (iter, pobj, dobj, gap, pinf, dinf). */

@@ -118,3 +118,72 @@ def test_hsd_pcg_mode_reaches_highs_objective(device):
     assert res.backend_error == 0
     assert L.HSD_STATUS[res.status] == "OK"
     assert abs(res.primal_obj - spec["objective"]) <= 1e-6 * (1 + abs(spec["objective"]))
+
+
+@pytest.mark.parametrize("fp32", [1, 0])
+@pytest.mark.parametrize("name,maker,K", [
+    ("random", lambda: instances.random_lp(300, 600, 4, seed=31), 20),
+    ("zipf-heavy-rows", lambda: instances.zipf(3000, 9000, 6, 1.1, seed=32), 40)])
+def test_preconditioner_apply_matches_oracle(name, maker, K, fp32):
+    """The apply z = M^-1 r of PCG.cpp:183-196 (L^-T D_S^-1 L^-1 with
+    L = [L11 0; L21 I]) against the oracle's or_pchol_apply, which the CPU
+    tests pin to the explicit dense inverse.  FP64 factor: within 1e-12
+    relative.  FP32 factor (every L21 entry rounded to FP32, relative 6e-8,
+    vectors and sums in FP64): within 1e-5 relative."""
+    inst = maker()
+    rng = np.random.default_rng(7)
+    d = np.exp(rng.uniform(-2, 2, inst.n) * np.log(10.0))
+    r = rng.standard_normal(inst.m)
+    _flush(f"pcg apply {name} m={inst.m} K={K} fp32={fp32}")
+    be = LinearEquation(mode=L.MODE_PCG, pcg_k=K, pcg_fp32_precond=fp32)
+    be.construct(inst)
+    be.renew(d)
+    z = be.pcg_apply(r)
+    z2 = be.pcg_apply(r)
+    be.destruct()
+    rp, ci, pos = O.csr(inst)
+    cv = np.ascontiguousarray(inst.val[pos])
+    pc = O.PChol()
+    O.lib().or_pchol_build(inst.m, inst.n, O.i64(rp), O.i32(ci), O.d(cv), O.d(d), 0.0, 0.0, K, pc)
+    perm = np.ctypeslib.as_array(pc.perm, shape=(inst.m,)).copy()
+    rperm = np.ascontiguousarray(r[perm])
+    zp = np.empty(inst.m)
+    work = np.empty(2 * inst.m)
+    O.lib().or_pchol_apply(pc, O.d(rperm), O.d(zp), O.d(work))
+    O.lib().or_pchol_free(pc)
+    zo = np.empty(inst.m)
+    zo[perm] = zp
+    err = np.linalg.norm(z - zo) / np.linalg.norm(zo)
+    _flush(f"  rel err vs oracle {err:.3e}")
+    assert np.array_equal(z, z2)
+    assert err <= (1e-5 if fp32 else 1e-12)
+
+
+def test_pcg_cap_reports_noconv():
+    """A PCG solve stopped by pcg_max_iter short of pcg_tol returns
+    HELP_ERR_NOCONV (with its last iterate and the residual in the stats),
+    and the HSD drivers stop with STALLED / HELP_STALL_BACKEND instead of
+    iterating on an unconverged direction."""
+    from help_amd.backend import HelpError
+    from help_amd.hsd import GpuBackend, solve_lp
+    inst = instances.random_lp(300, 600, 4, seed=41)
+    rng = np.random.default_rng(3)
+    d = np.exp(rng.uniform(-3, 3, inst.n) * np.log(10.0))
+    b1, b2 = rng.standard_normal(inst.n), rng.standard_normal(inst.m)
+    be = LinearEquation(mode=L.MODE_PCG, pcg_k=10, pcg_max_iter=2)
+    be.construct(inst)
+    be.renew(d)
+    with pytest.raises(HelpError) as ei:
+        be.solve(d, b1, b2)
+    assert ei.value.code == L.HELP_ERR_NOCONV
+    assert be.solve_stats.pcg_iters == 2 and be.solve_stats.rel_residual > 1e-10
+    be.destruct()
+    for device in (True, False):
+        gb = GpuBackend(mode=L.MODE_PCG, pcg_k=10, pcg_max_iter=2)
+        try:
+            res, _ = solve_lp(inst, gb.table, device=device)
+        finally:
+            gb.close()
+        _flush(f"  hsd device={device}: status {res.status} stall {res.stall_reason} "
+               f"backend_error {res.backend_error} iterations {res.iterations}")
+        assert res.status == 1 and res.stall_reason == 3 and res.backend_error == L.HELP_ERR_NOCONV
