@@ -157,6 +157,7 @@ SIGNATURES = {
     "help_gpu_backend_pcg_iterations": (C.c_int64, [C.POINTER(Backend)]),
     "help_gpu_pcg_factor": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D, _D, _D]),
     "help_gpu_pcg_apply": (C.c_int, [_P, _D, _D]),
+    "help_gpu_pcg_apply_noalloc": (C.c_int, [_P, C.c_void_p, C.c_void_p]),
     "help_gpu_profile": (C.c_int, [_P, C.c_int32]),
     "help_gpu_kernel_stats": (C.c_int, [_P, C.c_int32, _D, C.POINTER(C.c_int64), _D,
                                         C.POINTER(C.c_char_p)]),

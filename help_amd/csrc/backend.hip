@@ -1353,6 +1353,33 @@ int help_gpu_pcg_apply(help_gpu* h, const double* r, double* z) {
     }
 }
 
+static bool is_device_ptr(const void* p) {
+    hipPointerAttribute_t a{};
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    (void)hipGetLastError();   // a host pointer leaves an error behind: clear it
+    return e == hipSuccess && a.type == hipMemoryTypeDevice;
+}
+
+int help_gpu_pcg_apply_noalloc(help_gpu* h, const double* r, double* z) {
+    if (!h || !r || !z) return HELP_ERR_ARG;
+    if (!h->device || !h->use_pcg || !h->factored) { h->err = "pcg_apply_noalloc: PCG mode after renew only"; return HELP_ERR_STATE; }
+    if (!is_device_ptr(r) || !is_device_ptr(z)) {
+        h->err = "pcg_apply_noalloc: r and z must be device memory";
+        return HELP_ERR_ARG;
+    }
+    try {
+        help::pcg_precond_noalloc(h, r, z);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->st));
+        return HELP_OK;
+    } catch (const HipError& e) {
+        return fail(h, e);
+    } catch (const std::exception& e) {
+        h->err = e.what();
+        return HELP_ERR_ARG;
+    }
+}
+
 int help_gpu_spmv(help_gpu* h, int32_t trans, const double* x, double* y) {
     if (!h || !x || !y) return HELP_ERR_ARG;
     if (!h->device) { h->err = "spmv: no device state"; return HELP_ERR_STATE; }

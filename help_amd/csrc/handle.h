@@ -213,6 +213,7 @@ double pcg_build_bytes(const help_gpu* h);
 double pcg_iteration_bytes(const help_gpu* h);
 void pcg_download(help_gpu* h, int32_t* K, int32_t* perm, double* D, double* S, double* L);
 void pcg_precond(help_gpu* h, const double* v, double* out);
+void pcg_precond_noalloc(help_gpu* h, const double* v, double* out);
 // warm-start network forward on device vectors (mpnn.hip)
 void mpnn_predict_device(help_mpnn* net, help_gpu* h, const double* d, const double* r, double* x0);
 }  // namespace help

@@ -6,9 +6,39 @@
 // with the preconditioner applied from the FP32 copy of the factor.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "handle.h"
 #include "pcg.h"
+
+namespace {
+// Scratch of the allocation-free preconditioner apply
+// (help_gpu_pcg_apply_noalloc): the apply's y1/w vectors (2 * kPcgMaxK) and
+// its per-block partial sums (K * ceil(m / kLtRows)).  Every GPU holds one
+// copy until the process ends; calls that overlap on one GPU share it and
+// race.  The kernels write every entry they read, so nothing needs clearing.
+constexpr int64_t kNoAllocScratch = int64_t(1) << 20;   // doubles, 8 MiB
+__device__ __align__(256) double scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// device address of `scratch` on the current device, looked up once per
+// device (the host-side name of the array is not a device address)
+double* scratch_ptr() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw help::HipError("device index out of range", HELP_ERR_HIP);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (double*)p;
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
 
 namespace help {
 
@@ -182,6 +212,25 @@ void pcg_precond(help_gpu* h, const double* v, double* out) {
     else
         launch_pc_apply<double>(h->m, P->K, P->perm.p, P->pos.p, P->L.p, P->X.p, P->S.p, v, P->y1.p,
                                 P->ltpart.p, out, h->st);
+}
+
+// pcg_precond with its scratch in the file-scope `scratch` array: no
+// allocation, device vectors only
+void pcg_precond_noalloc(help_gpu* h, const double* v, double* out) {
+    PcgState* P = h->pcg;
+    const int64_t nb = ((int64_t)h->m + kLtRows - 1) / kLtRows;
+    const int64_t need = 2 * (int64_t)kPcgMaxK + (int64_t)std::max(P->K, 1) * nb;
+    if (need > kNoAllocScratch)
+        throw std::runtime_error("pcg_apply_noalloc: needs " + std::to_string(need) +
+                                 " scratch doubles (2 * 1024 + K * ceil(m / 2048)), above the maximum " +
+                                 std::to_string(kNoAllocScratch));
+    double* s = scratch_ptr();
+    if (h->opts.pcg_fp32_precond)
+        launch_pc_apply<float>(h->m, P->K, P->perm.p, P->pos.p, P->L32.p, P->X.p, P->S.p, v, s,
+                               s + 2 * kPcgMaxK, out, h->st);
+    else
+        launch_pc_apply<double>(h->m, P->K, P->perm.p, P->pos.p, P->L.p, P->X.p, P->S.p, v, s,
+                                s + 2 * kPcgMaxK, out, h->st);
 }
 
 void pcg_download(help_gpu* h, int32_t* K, int32_t* perm, double* D, double* S, double* L) {

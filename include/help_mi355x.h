@@ -254,6 +254,15 @@ int help_gpu_pcg_factor(help_gpu* h, int32_t* K, int32_t* perm, double* D, doubl
  * the last renew, applied as every CG iteration applies it (PCG.cpp:183-196:
  * L^-T D_S^-1 L^-1 with L = [L11 0; L21 I]; FP32 factor, FP64 vectors). */
 int help_gpu_pcg_apply(help_gpu* h, const double* r, double* z);
+/* help_gpu_pcg_apply without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  r and z must be
+ * device memory (HELP_ERR_ARG otherwise); the apply's scratch lives in a
+ * static 8 MiB device array per GPU, which limits it to
+ * 2 * 1024 + K * ceil(m / 2048) doubles (HELP_ERR_ARG beyond that: K = 500
+ * reaches m = 4.2 million).  Calls that overlap on one GPU share that array
+ * and race.  Blocking, on the handle's stream, like every entry point. */
+int help_gpu_pcg_apply_noalloc(help_gpu* h, const double* r, double* z);
 
 int help_gpu_profile(help_gpu* h, int32_t enable);
 int help_gpu_kernel_stats(const help_gpu* h, int32_t kind, double* ms, int64_t* launches,
