@@ -152,6 +152,12 @@ def cpu_baseline(inst, gpu_iterations, sample_iters=1):
                 measured_s=round(wall, 3))
 
 
+def path_opts(name, extra, mixed_opts):
+    """backend options of a path: the common ones, plus the refinement
+    settings for the mixed-precision paths"""
+    return {**extra, **(mixed_opts if PATHS[name][0] == "fp32" else {})}
+
+
 def run_path(inst, name, steps, warmup, driver, extra, L, GpuBackend, solve_lp):
     mode_s, ordering = PATHS[name]
     mode = L.MODE_DIRECT_FP32 if mode_s == "fp32" else L.MODE_DIRECT_FP64
@@ -239,6 +245,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-dy-error", action="store_true")
+    ap.add_argument("--refine-tol", type=float, default=None,
+                    help="mixed paths: FP64 refinement stops at this relative residual (library default 1e-12)")
+    ap.add_argument("--refine-max", type=int, default=None,
+                    help="mixed paths: refinement iterations per solve (library default 20)")
     ap.add_argument("--inverse-solve", type=int, default=None, choices=[-1, 0, 1],
                     help="explicit L11 inverse for supernodes wider than 64 columns: "
                          "0 or 1 on (default), -1 off (block substitution)")
@@ -265,9 +275,15 @@ def main():
     if args.inverse_solve is not None:
         extra["inverse_solve"] = args.inverse_solve
 
+    mixed = {}
+    if args.refine_tol is not None:
+        mixed["refine_tol"] = args.refine_tol
+    if args.refine_max is not None:
+        mixed["refine_max"] = args.refine_max
     results = {}
     for p in paths:
-        results[p] = run_path(inst, p, args.steps, args.warmup, args.driver, extra, L, GpuBackend, solve_lp)
+        results[p] = run_path(inst, p, args.steps, args.warmup, args.driver, path_opts(p, extra, mixed), L,
+                              GpuBackend, solve_lp)
     vp = paths[-1]
     vres = results[vp]
     mode_s, ordering = PATHS[vp]
@@ -278,7 +294,7 @@ def main():
     rows, prof_s = [], None
     if not args.no_profile:
         gp = GpuBackend(mode=L.MODE_DIRECT_FP32 if mode_s == "fp32" else L.MODE_DIRECT_FP64,
-                        ordering=ordering, profile=1, **extra)
+                        ordering=ordering, profile=1, **path_opts(vp, extra, mixed))
         ts = time.time()
         rp, _ = solve_lp(inst, gp.table, device=args.driver == "gpu")
         L.lib().help_device_synchronize()
@@ -295,7 +311,7 @@ def main():
     if not args.no_dy_error:
         it = max(vres["iterations"], 1)
         cap = sorted({1, max(1, it // 2), it})
-        dy_list, st, its = dy_errors(inst, vp, cap, extra)
+        dy_list, st, its = dy_errors(inst, vp, cap, path_opts(vp, extra, mixed))
         dy = dict(path=vp, driver="host (HELP order)", status=st, iterations=its, solves=dy_list)
 
     cpu = None
@@ -322,6 +338,7 @@ def main():
                    "solver": "HELP HSD driver", **speed,
                    "paths": results,
                    "inverse_solve": "off" if args.inverse_solve == -1 else "on",
+                   "refine_tol": args.refine_tol if args.refine_tol is not None else 1e-12,
                    "profiled_solve_s": None if prof_s is None else round(prof_s, 3)},
         "dy_error": dy,
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,

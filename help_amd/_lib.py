@@ -132,6 +132,8 @@ SIGNATURES = {
     "help_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(LP)]),
     "help_lp_free": (None, [C.POINTER(LP)]),
     "help_write_mps": (C.c_int, [C.POINTER(LP), C.c_char_p, C.c_char_p]),
+    "help_write_csc_bin": (C.c_int, [C.POINTER(LP), C.c_char_p]),
+    "help_read_csc_bin": (C.c_int, [C.c_char_p, C.POINTER(LP)]),
     "help_hsd_solve": (C.c_int, [C.POINTER(LP), C.POINTER(Backend), C.c_int32, C.c_int32, LOG_FN,
                                  _P, C.POINTER(HsdResult)]),
     "help_hsd_solve_gpu": (C.c_int, [C.POINTER(LP), C.POINTER(Backend), C.c_int32, LOG_FN, _P,

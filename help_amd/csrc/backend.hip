@@ -35,36 +35,8 @@ namespace {
 
 // Position of slot (i,k) (original indices) inside the supernodal L storage.
 void build_assembly_maps(help_gpu* h) {
-    const Supernodal& S = h->S;
-    int64_t nup = (int64_t)h->P.colind.size();
-    h->slot_dest.resize(nup);
-    h->diagpos.resize(h->m);
-    for (int32_t i = 0; i < h->m; ++i)
-        for (int64_t s = h->P.rowptr[i]; s < h->P.rowptr[i + 1]; ++s) {
-            int32_t k = h->P.colind[s];
-            int32_t a = S.iperm[i], b = S.iperm[k];
-            int32_t col = std::min(a, b), row = std::max(a, b);
-            int32_t sn = S.col2sn[col];
-            const int32_t* rb = S.rowidx.data() + S.rowoff[sn];
-            const int32_t* re = S.rowidx.data() + S.rowoff[sn + 1];
-            const int32_t* it = std::lower_bound(rb, re, row);
-            if (it == re || *it != row) throw std::runtime_error("assembly map: row not in structure");
-            int64_t pos = S.loff[sn] + (int64_t)(col - S.sfirst[sn]) * S.snrow[sn] + (it - rb);
-            h->slot_dest[s] = pos;
-        }
-    // the diagonal of every column (also of an empty row of A, whose ADA^T row
-    // has no slot but still receives beta): own columns lead the structure
-    for (int32_t i = 0; i < h->m; ++i) {
-        int32_t col = S.iperm[i], sn = S.col2sn[col];
-        int64_t r = col - S.sfirst[sn];
-        h->diagpos[i] = S.loff[sn] + r * S.snrow[sn] + r;
-    }
-    h->colof.resize(h->nnz);
-    for (int32_t j = 0; j < h->n; ++j)
-        for (int64_t p = h->A.colptr[j]; p < h->A.colptr[j + 1]; ++p) h->colof[p] = j;
+    help::assembly_maps(h->S, h->P, h->A, h->R, h->slot_dest, h->diagpos, h->colof, h->csrval);
     h->csrcol = h->R.colind;
-    h->csrval.resize(h->nnz);
-    for (int64_t t = 0; t < h->nnz; ++t) h->csrval[t] = h->A.val[h->R.cscpos[t]];
 }
 
 // ---- profiling: events around launches on the handle's stream ----

@@ -13,6 +13,7 @@
 #include "help_mi355x.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -360,4 +361,105 @@ extern "C" int help_write_mps(const help_lp* lp, const char* path, const char* n
     std::fprintf(f, "ENDATA\n");
     std::fclose(f);
     return HELP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Binary CSC instance file, for LPs beyond the fixed-MPS limits (HELP's
+// MAX_ROWS / MAX_COLS / MAX_ELEMENTS of LP.h:61-63, e.g. m = 1e6, n = 4e6).
+// Little-endian layout:
+//   char[8]  "HELPCSC1"
+//   int32    m, n, n_ub, n_lb, n_fr, reserved (0)
+//   int64    nnz
+//   double   c_intercept
+//   int64    colptr[n + 1]
+//   int32    rowind[nnz]
+//   double   val[nnz], b[m], c[n], ub[n_ub]
+//   uint64   FNV-1a hash of every byte above
+// ---------------------------------------------------------------------------
+namespace {
+constexpr char kCscMagic[8] = {'H', 'E', 'L', 'P', 'C', 'S', 'C', '1'};
+
+struct Fnv {
+    uint64_t h = 0xcbf29ce484222325ull;
+    void add(const void* p, size_t n) {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 0x100000001b3ull; }
+    }
+};
+
+bool put(FILE* f, Fnv& fnv, const void* p, size_t n) {
+    if (n == 0) return true;
+    fnv.add(p, n);
+    return std::fwrite(p, 1, n, f) == n;
+}
+
+bool get(FILE* f, Fnv& fnv, void* p, size_t n) {
+    if (n == 0) return true;
+    if (std::fread(p, 1, n, f) != n) return false;
+    fnv.add(p, n);
+    return true;
+}
+}  // namespace
+
+extern "C" int help_write_csc_bin(const help_lp* lp, const char* path) {
+    if (!lp || !path || lp->m < 0 || lp->n < 0 || lp->nnz < 0 || lp->n_ub < 0 || lp->n_fr < 0 ||
+        (lp->n_ub > 0 && !lp->ub))
+        return HELP_ERR_ARG;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return HELP_ERR_ARG;
+    Fnv fnv;
+    const int32_t hdr[6] = {lp->m, lp->n, lp->n_ub, lp->n_lb, lp->n_fr, 0};
+    bool ok = put(f, fnv, kCscMagic, 8) && put(f, fnv, hdr, sizeof hdr) && put(f, fnv, &lp->nnz, 8) &&
+              put(f, fnv, &lp->c_intercept, 8) && put(f, fnv, lp->colptr, sizeof(int64_t) * (lp->n + 1)) &&
+              put(f, fnv, lp->rowind, sizeof(int32_t) * lp->nnz) && put(f, fnv, lp->val, sizeof(double) * lp->nnz) &&
+              put(f, fnv, lp->b, sizeof(double) * lp->m) && put(f, fnv, lp->c, sizeof(double) * lp->n) &&
+              put(f, fnv, lp->ub, sizeof(double) * lp->n_ub);
+    const uint64_t h = fnv.h;
+    ok = ok && std::fwrite(&h, 1, 8, f) == 8;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? HELP_OK : HELP_ERR_ARG;
+}
+
+extern "C" int help_read_csc_bin(const char* path, help_lp* lp) {
+    if (!path || !lp) return HELP_ERR_ARG;
+    std::memset(lp, 0, sizeof(*lp));
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return HELP_ERR_ARG;
+    Fnv fnv;
+    char magic[8];
+    int32_t hdr[6];
+    int64_t nnz = 0;
+    double icpt = 0;
+    int st = HELP_ERR_ARG;
+    if (get(f, fnv, magic, 8) && std::memcmp(magic, kCscMagic, 8) == 0 && get(f, fnv, hdr, sizeof hdr) &&
+        get(f, fnv, &nnz, 8) && get(f, fnv, &icpt, 8)) {
+        const int32_t m = hdr[0], n = hdr[1], nub = hdr[2], nlb = hdr[3], nfr = hdr[4];
+        if (m >= 0 && n >= 0 && nnz >= 0 && nnz <= INT32_MAX && nub >= 0 && nfr >= 0 && nub + nfr <= n &&
+            nlb == n - nfr) {
+            lp->m = m; lp->n = n; lp->nnz = nnz; lp->n_ub = nub; lp->n_lb = nlb; lp->n_fr = nfr;
+            lp->c_intercept = icpt;
+            lp->colptr = (int64_t*)std::malloc(sizeof(int64_t) * (n + 1));
+            lp->rowind = (int32_t*)std::malloc(sizeof(int32_t) * std::max<int64_t>(1, nnz));
+            lp->val = (double*)std::malloc(sizeof(double) * std::max<int64_t>(1, nnz));
+            lp->b = (double*)std::malloc(sizeof(double) * std::max(1, m));
+            lp->c = (double*)std::malloc(sizeof(double) * std::max(1, n));
+            lp->ub = nub > 0 ? (double*)std::malloc(sizeof(double) * nub) : nullptr;
+            uint64_t h = 0;
+            if (!lp->colptr || !lp->rowind || !lp->val || !lp->b || !lp->c || (nub > 0 && !lp->ub)) {
+                st = HELP_ERR_NOMEM;
+            } else if (get(f, fnv, lp->colptr, sizeof(int64_t) * (n + 1)) &&
+                       get(f, fnv, lp->rowind, sizeof(int32_t) * nnz) && get(f, fnv, lp->val, sizeof(double) * nnz) &&
+                       get(f, fnv, lp->b, sizeof(double) * m) && get(f, fnv, lp->c, sizeof(double) * n) &&
+                       get(f, fnv, lp->ub, sizeof(double) * nub) && std::fread(&h, 1, 8, f) == 8 && h == fnv.h) {
+                // structural checks: monotone column pointers, rows in range
+                bool good = lp->colptr[0] == 0 && lp->colptr[n] == nnz;
+                for (int32_t j = 0; good && j < n; ++j) good = lp->colptr[j] <= lp->colptr[j + 1];
+                for (int64_t p = 0; good && p < nnz; ++p) good = lp->rowind[p] >= 0 && lp->rowind[p] < m;
+                if (good) st = HELP_OK;
+            }
+        }
+    }
+    std::fclose(f);
+    if (st != HELP_OK) help_lp_free(lp);
+    return st;
 }

@@ -3,7 +3,9 @@
 #include "symbolic.h"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
+#include <memory>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -381,23 +383,31 @@ std::vector<int32_t> amd_order(int32_t n, const std::vector<int64_t>& adjptr,
 // ---------------------------------------------------------------------------
 namespace {
 
+using NdJobs = std::vector<std::vector<int32_t>>;
+
 struct NdWork {
     const std::vector<int64_t>& ptr;
     const std::vector<int32_t>& adj;
-    std::vector<int32_t> stamp, level, loc;
-    int32_t cur = 0;
+    // membership / visit marks: a subgraph's vertices carry its tag; the two
+    // halves of a bisection are dissected as parallel tasks, so a BFS may
+    // read the mark of a vertex another task is writing (never its own tag)
+    std::unique_ptr<std::atomic<int32_t>[]> stamp;
+    std::vector<int32_t> level;   // written and read only by the task owning the vertex
+    std::atomic<int32_t> cur{0};
     int32_t leaf;
-    std::vector<int32_t> order;
     NdWork(int32_t n, const std::vector<int64_t>& p, const std::vector<int32_t>& a, int32_t lf)
-        : ptr(p), adj(a), stamp(n, -1), level(n, -1), loc(n, -1), leaf(lf) {
-        order.reserve(n);
+        : ptr(p), adj(a), stamp(new std::atomic<int32_t>[n]), level(n, -1), leaf(lf) {
+        for (int32_t i = 0; i < n; ++i) stamp[i].store(-1, std::memory_order_relaxed);
     }
+    int32_t ld(int32_t v) const { return stamp[v].load(std::memory_order_relaxed); }
+    void st(int32_t v, int32_t t) { stamp[v].store(t, std::memory_order_relaxed); }
+    int32_t new_tag() { return cur.fetch_add(2) + 2; }
 
-    // AMD on the subgraph induced by `set`, appended to the order
-    void amd_append(const std::vector<int32_t>& set) {
+    // AMD on the subgraph induced by `set`; loc is a scratch map (all -1 on
+    // entry and exit) private to the calling thread
+    void amd_job(std::vector<int32_t>& set, std::vector<int32_t>& loc) const {
         const int32_t k = (int32_t)set.size();
-        if (k == 0) return;
-        if (k == 1) { order.push_back(set[0]); return; }
+        if (k <= 1) return;
         for (int32_t i = 0; i < k; ++i) loc[set[i]] = i;
         std::vector<int64_t> lp(k + 1, 0);
         std::vector<int32_t> la;
@@ -410,70 +420,71 @@ struct NdWork {
             lp[i + 1] = (int64_t)la.size();
         }
         std::vector<int32_t> o = amd_order(k, lp, la);
-        for (int32_t i = 0; i < k; ++i) order.push_back(set[o[i]]);
         for (int32_t i = 0; i < k; ++i) loc[set[i]] = -1;
+        std::vector<int32_t> out(k);
+        for (int32_t i = 0; i < k; ++i) out[i] = set[o[i]];
+        set.swap(out);
     }
 
-    // BFS inside the vertices stamped `tag` from `root`; returns the levels
+    // BFS inside the vertices stamped `tag` from `root`; returns the depth
     // (level[] set for the reached vertices) and the visit order
     int32_t bfs(int32_t root, int32_t tag, std::vector<int32_t>& visit) {
         visit.clear();
         visit.push_back(root);
         level[root] = 0;
-        stamp[root] = tag + 1;   // reached mark for this BFS
+        st(root, tag + 1);   // reached mark for this BFS
         int32_t depth = 0;
         for (size_t h = 0; h < visit.size(); ++h) {
             const int32_t v = visit[h];
             for (int64_t e = ptr[v]; e < ptr[v + 1]; ++e) {
                 const int32_t w = adj[e];
-                if (stamp[w] == tag) {
-                    stamp[w] = tag + 1;
+                if (ld(w) == tag) {
+                    st(w, tag + 1);
                     level[w] = level[v] + 1;
                     depth = std::max(depth, level[w]);
                     visit.push_back(w);
                 }
             }
         }
-        for (int32_t v : visit) stamp[v] = tag;   // restore membership
+        for (int32_t v : visit) st(v, tag);   // restore membership
         return depth;
     }
 
-    int32_t new_tag() { cur += 2; return cur; }
-
-    void dissect(std::vector<int32_t> set) {
-        if ((int32_t)set.size() <= leaf) { amd_append(set); return; }
+    // the sets to number by minimum degree for the subgraph `set`, in
+    // elimination order, appended to `out`
+    void dissect(std::vector<int32_t> set, NdJobs& out) {
+        if ((int32_t)set.size() <= leaf) { if (!set.empty()) out.push_back(std::move(set)); return; }
         const int32_t tag = new_tag();
-        for (int32_t v : set) stamp[v] = tag;
+        for (int32_t v : set) st(v, tag);
         // connected components, each dissected on its own
         std::vector<int32_t> visit;
         {
-            std::vector<std::vector<int32_t>> comps;
+            NdJobs comps;
             size_t covered = 0;
             const int32_t seen = new_tag();
             for (int32_t v : set) {
-                if (stamp[v] != tag) continue;
+                if (ld(v) != tag) continue;
                 visit.clear();
                 visit.push_back(v);
-                stamp[v] = seen;
+                st(v, seen);
                 for (size_t h = 0; h < visit.size(); ++h) {
                     const int32_t u = visit[h];
                     for (int64_t e = ptr[u]; e < ptr[u + 1]; ++e)
-                        if (stamp[adj[e]] == tag) { stamp[adj[e]] = seen; visit.push_back(adj[e]); }
+                        if (ld(adj[e]) == tag) { st(adj[e], seen); visit.push_back(adj[e]); }
                 }
                 covered += visit.size();
                 comps.push_back(visit);
                 if (covered == set.size()) break;
             }
             if (comps.size() > 1) {
-                for (auto& c : comps) dissect(std::move(c));
+                for (auto& c : comps) dissect(std::move(c), out);
                 return;
             }
-            for (int32_t v : set) stamp[v] = tag;
+            for (int32_t v : set) st(v, tag);
         }
         // pseudo-peripheral root: repeat BFS from a minimum-degree vertex of
         // the last level while the depth grows (at most 4 sweeps)
-        int32_t root = set[0];
-        int32_t depth = bfs(root, tag, visit);
+        int32_t depth = bfs(set[0], tag, visit);
         for (int it = 0; it < 4; ++it) {
             int32_t best = -1;
             int64_t bdeg = INT64_MAX;
@@ -484,13 +495,16 @@ struct NdWork {
                 }
             std::vector<int32_t> vis2;
             const int32_t d2 = bfs(best, tag, vis2);
-            if (d2 <= depth) { if (d2 == depth) { root = best; visit.swap(vis2); } break; }
-            root = best;
+            if (d2 < depth) {   // the previous level structure stays
+                bfs(visit[0], tag, visit);
+                break;
+            }
+            const bool grew = d2 > depth;
             depth = d2;
             visit.swap(vis2);
+            if (!grew) break;
         }
-        (void)root;
-        if (depth < 2) { amd_append(set); return; }
+        if (depth < 2) { out.push_back(std::move(set)); return; }
         std::vector<int64_t> cnt(depth + 1, 0);
         for (int32_t v : visit) cnt[level[v]]++;
         // separator level: the smallest level whose both sides hold at least
@@ -518,15 +532,14 @@ struct NdWork {
         // separator vertices without a far neighbour move to the near side,
         // then those without a near neighbour move to the far side
         std::vector<int32_t> sep, near_side, far_side;
-        for (int32_t v : visit) {
+        for (int32_t v : visit)
             if (level[v] == sl) sep.push_back(v);
-        }
         auto side = [&](int32_t v) { return level[v] < sl ? 0 : (level[v] == sl ? 1 : 2); };
         std::vector<int32_t> keep;
         for (int32_t v : sep) {
             bool far = false;
             for (int64_t e = ptr[v]; e < ptr[v + 1] && !far; ++e)
-                far = stamp[adj[e]] == tag && side(adj[e]) == 2;
+                far = ld(adj[e]) == tag && side(adj[e]) == 2;
             if (!far) level[v] = sl - 1;   // near side
             else keep.push_back(v);
         }
@@ -534,11 +547,11 @@ struct NdWork {
         for (int32_t v : keep) {
             bool nearn = false;
             for (int64_t e = ptr[v]; e < ptr[v + 1] && !nearn; ++e)
-                nearn = stamp[adj[e]] == tag && side(adj[e]) == 0;
+                nearn = ld(adj[e]) == tag && side(adj[e]) == 0;
             if (!nearn) level[v] = sl + 1;   // far side
             else sep.push_back(v);
         }
-        if (2 * sep.size() > set.size()) { amd_append(set); return; }
+        if (2 * sep.size() > set.size()) { out.push_back(std::move(set)); return; }
         for (int32_t v : visit) {
             const int s = side(v);
             if (s == 0) near_side.push_back(v);
@@ -547,11 +560,17 @@ struct NdWork {
         std::sort(near_side.begin(), near_side.end());
         std::sort(far_side.begin(), far_side.end());
         std::sort(sep.begin(), sep.end());
+        const bool par = set.size() > 20000;
         set.clear();
         set.shrink_to_fit();
-        dissect(std::move(near_side));
-        dissect(std::move(far_side));
-        amd_append(sep);
+        NdJobs jn, jf;
+#pragma omp task shared(jn) if (par)
+        dissect(std::move(near_side), jn);
+        dissect(std::move(far_side), jf);
+#pragma omp taskwait
+        for (auto& j : jn) out.push_back(std::move(j));
+        for (auto& j : jf) out.push_back(std::move(j));
+        out.push_back(std::move(sep));
     }
 };
 
@@ -562,8 +581,21 @@ std::vector<int32_t> nd_order(int32_t n, const std::vector<int64_t>& adjptr,
     NdWork w(n, adjptr, adj, std::max(leaf, 16));
     std::vector<int32_t> all(n);
     for (int32_t i = 0; i < n; ++i) all[i] = i;
-    w.dissect(std::move(all));
-    return w.order;
+    NdJobs jobs;
+#pragma omp parallel
+#pragma omp single
+    w.dissect(std::move(all), jobs);
+    const int64_t nj = (int64_t)jobs.size();
+#pragma omp parallel
+    {
+        std::vector<int32_t> loc(n, -1);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t j = 0; j < nj; ++j) w.amd_job(jobs[j], loc);
+    }
+    std::vector<int32_t> order;
+    order.reserve(n);
+    for (const auto& j : jobs) order.insert(order.end(), j.begin(), j.end());
+    return order;
 }
 
 std::vector<int32_t> etree(int32_t m, const std::vector<int64_t>& adjptr,
@@ -774,30 +806,56 @@ Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vec
     // row structures: own columns U lower entries of M U children's update rows
     S.rowoff.assign(ns + 1, 0);
     S.snrow.assign(ns, 0);
-    std::vector<int32_t> mk(m, -1), rows;
     std::vector<std::vector<int32_t>> tmp(ns);
-    for (int32_t s = 0; s < ns; ++s) {
-        int32_t f = S.sfirst[s], l = S.sfirst[s + 1];
-        rows.clear();
-        for (int32_t j = f; j < l; ++j) { mk[j] = s; rows.push_back(j); }
-        for (int32_t j = f; j < l; ++j) {
-            int32_t old = perm[j];
-            for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
-                int32_t i = iperm[adj[t]];
-                if (i >= l && mk[i] != s) { mk[i] = s; rows.push_back(i); }
+    {
+        // a supernode's structure needs its children's: one parallel sweep
+        // per elimination-tree level (children sit on lower levels)
+        std::vector<int32_t> lev(ns, 0);
+        int32_t top = 0;
+        for (int32_t s = 0; s < ns; ++s) {
+            for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) lev[s] = std::max(lev[s], lev[S.child[t]] + 1);
+            top = std::max(top, lev[s]);
+        }
+        std::vector<int32_t> lptr(top + 2, 0), lsn(ns);
+        for (int32_t s = 0; s < ns; ++s) lptr[lev[s] + 1]++;
+        for (int32_t l = 0; l <= top; ++l) lptr[l + 1] += lptr[l];
+        {
+            std::vector<int32_t> f(lptr.begin(), lptr.end() - 1);
+            for (int32_t s = 0; s < ns; ++s) lsn[f[lev[s]]++] = s;
+        }
+#pragma omp parallel
+        {
+            std::vector<int32_t> mk(m, -1), rows;
+            for (int32_t l = 0; l <= top; ++l) {
+#pragma omp for schedule(dynamic, 16)
+                for (int32_t q = lptr[l]; q < lptr[l + 1]; ++q) {
+                    const int32_t s = lsn[q];
+                    const int32_t f = S.sfirst[s], lc = S.sfirst[s + 1];
+                    rows.clear();
+                    for (int32_t j = f; j < lc; ++j) { mk[j] = s; rows.push_back(j); }
+                    for (int32_t j = f; j < lc; ++j) {
+                        const int32_t old = perm[j];
+                        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
+                            const int32_t i = iperm[adj[t]];
+                            if (i >= lc && mk[i] != s) { mk[i] = s; rows.push_back(i); }
+                        }
+                    }
+                    for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
+                        const int32_t c = S.child[t];
+                        const int32_t cn = S.sfirst[c + 1] - S.sfirst[c];
+                        const std::vector<int32_t>& cr = tmp[c];
+                        for (size_t u = cn; u < cr.size(); ++u)
+                            if (mk[cr[u]] != s) { mk[cr[u]] = s; rows.push_back(cr[u]); }
+                    }
+                    std::sort(rows.begin(), rows.end());
+                    tmp[s] = rows;
+                }
             }
         }
-        for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
-            int32_t c = S.child[t];
-            int32_t cn = S.sfirst[c + 1] - S.sfirst[c];
-            const std::vector<int32_t>& cr = tmp[c];
-            for (size_t u = cn; u < cr.size(); ++u)
-                if (mk[cr[u]] != s) { mk[cr[u]] = s; rows.push_back(cr[u]); }
-        }
-        std::sort(rows.begin(), rows.end());
-        tmp[s] = rows;
-        S.snrow[s] = (int32_t)rows.size();
-        S.rowoff[s + 1] = S.rowoff[s] + (int64_t)rows.size();
+    }
+    for (int32_t s = 0; s < ns; ++s) {
+        S.snrow[s] = (int32_t)tmp[s].size();
+        S.rowoff[s + 1] = S.rowoff[s] + (int64_t)tmp[s].size();
     }
     S.rowidx.resize(S.rowoff[ns]);
     for (int32_t s = 0; s < ns; ++s) {
@@ -845,6 +903,48 @@ Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vec
         for (int32_t s = 0; s < ns; ++s) S.levelsn[f[S.level[s]]++] = s;
     }
     return S;
+}
+
+// Where each ADA^T slot lands in the supernodal factor (slot_dest), the
+// diagonal of every row (diagpos; also for an empty row of A, whose ADA^T
+// row has no slot but still receives beta), the column of every nonzero of
+// A (colof) and the values of A in CSR order (csrval).
+void assembly_maps(const Supernodal& S, const AdatPattern& P, const CscMatrix& A, const CsrMatrix& R,
+                   std::vector<int64_t>& slot_dest, std::vector<int64_t>& diagpos,
+                   std::vector<int32_t>& colof, std::vector<double>& csrval) {
+    const int32_t m = A.m, n = A.n;
+    const int64_t nnz = (int64_t)A.rowind.size();
+    slot_dest.resize(P.colind.size());
+    diagpos.resize(m);
+    int64_t missing = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : missing)
+    for (int32_t i = 0; i < m; ++i)
+        for (int64_t s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
+            const int32_t k = P.colind[s];
+            const int32_t a = S.iperm[i], b = S.iperm[k];
+            const int32_t col = std::min(a, b), row = std::max(a, b);
+            const int32_t sn = S.col2sn[col];
+            const int32_t* rb = S.rowidx.data() + S.rowoff[sn];
+            const int32_t* re = S.rowidx.data() + S.rowoff[sn + 1];
+            const int32_t* it = std::lower_bound(rb, re, row);
+            if (it == re || *it != row) { ++missing; slot_dest[s] = 0; continue; }
+            slot_dest[s] = S.loff[sn] + (int64_t)(col - S.sfirst[sn]) * S.snrow[sn] + (it - rb);
+        }
+    if (missing) throw std::runtime_error("assembly map: row not in structure");
+    // own columns lead a supernode's row structure
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < m; ++i) {
+        const int32_t col = S.iperm[i], sn = S.col2sn[col];
+        const int64_t r = col - S.sfirst[sn];
+        diagpos[i] = S.loff[sn] + r * S.snrow[sn] + r;
+    }
+    colof.resize(nnz);
+#pragma omp parallel for schedule(static)
+    for (int32_t j = 0; j < n; ++j)
+        for (int64_t p = A.colptr[j]; p < A.colptr[j + 1]; ++p) colof[p] = j;
+    csrval.resize(nnz);
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < nnz; ++t) csrval[t] = A.val[R.cscpos[t]];
 }
 
 }  // namespace help

@@ -106,4 +106,11 @@ struct RelaxParams {          // merge child into parent when the merged superno
 Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vector<int32_t>& adj,
                    const std::vector<int32_t>& perm_in, const RelaxParams& rp);
 
+// Assembly maps of the numeric phase: factor position of every ADA^T slot,
+// of every diagonal, the column of every nonzero of A, A's values in CSR
+// order.  OpenMP-parallel.
+void assembly_maps(const Supernodal& S, const AdatPattern& P, const CscMatrix& A, const CsrMatrix& R,
+                   std::vector<int64_t>& slot_dest, std::vector<int64_t>& diagpos,
+                   std::vector<int32_t>& colof, std::vector<double>& csrval);
+
 }  // namespace help

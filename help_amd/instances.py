@@ -24,6 +24,11 @@ def _generate(params):
     st = L.lib().help_generate(C.byref(params), C.byref(lp))
     if st != L.HELP_OK:
         raise HelpError(st, "help_generate failed")
+    return _from_lp(lp)
+
+
+def _from_lp(lp):
+    """numpy copy of a library-allocated help_lp, which is then freed"""
     try:
         n, m, nnz = lp.n, lp.m, lp.nnz
         colptr = np.ctypeslib.as_array(lp.colptr, shape=(n + 1,)).copy()
@@ -32,10 +37,10 @@ def _generate(params):
         b = np.ctypeslib.as_array(lp.b, shape=(m,)).copy()
         c = np.ctypeslib.as_array(lp.c, shape=(n,)).copy()
         ub = np.ctypeslib.as_array(lp.ub, shape=(lp.n_ub,)).copy() if lp.n_ub > 0 else None
-        n_ub, n_fr = lp.n_ub, lp.n_fr
+        n_ub, n_fr, icpt = lp.n_ub, lp.n_fr, lp.c_intercept
     finally:
         L.lib().help_lp_free(C.byref(lp))
-    return Instance(m, n, colptr, rowind, val, b, c, n_ub=n_ub, n_fr=n_fr, ub=ub)
+    return Instance(m, n, colptr, rowind, val, b, c, n_ub=n_ub, n_fr=n_fr, ub=ub, c_intercept=icpt)
 
 
 def random_lp(m=2000, nb=4000, k=4, seed=0, ub_cols=0, free_cols=0):
@@ -66,6 +71,22 @@ def write_mps(inst, path, name="HELPGEN"):
     st = L.lib().help_write_mps(C.byref(s), path.encode(), name.encode())
     if st != L.HELP_OK:
         raise HelpError(st, "help_write_mps failed")
+
+
+def write_csc(inst, path):
+    """Binary CSC instance file (help_write_csc_bin), for LPs beyond MPS."""
+    s = inst.as_struct()
+    st = L.lib().help_write_csc_bin(C.byref(s), path.encode())
+    if st != L.HELP_OK:
+        raise HelpError(st, "help_write_csc_bin failed")
+
+
+def read_csc(path):
+    lp = L.LP()
+    st = L.lib().help_read_csc_bin(path.encode(), C.byref(lp))
+    if st != L.HELP_OK:
+        raise HelpError(st, f"help_read_csc_bin failed on {path}")
+    return _from_lp(lp)
 
 
 def by_config(cfg, seed=0):

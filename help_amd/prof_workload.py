@@ -3,6 +3,8 @@
     python -m help_amd.prof_workload solve --config 1 --renews 1 --solves 3
         construct the backend on a benchmark instance, then renew and solve
         with a fixed log-uniform d (the factor and solve kernels);
+    python -m help_amd.prof_workload solve --config 2 --mode fp32 --ordering 1
+        the staircase LP's FP32 factor under nested dissection;
     python -m help_amd.prof_workload solve --config 3 --mode pcg --pcg-max-iter 60
         the same in the matrix-free PCG mode, each solve capped at a fixed
         number of CG iterations;
@@ -23,7 +25,7 @@ def _solve(args):
     from .backend import LinearEquation
     inst = instances.by_config(args.config)
     mode = {"fp64": L.MODE_DIRECT_FP64, "fp32": L.MODE_DIRECT_FP32, "pcg": L.MODE_PCG}[args.mode]
-    extra = {"pcg_max_iter": args.pcg_max_iter} if args.mode == "pcg" else {}
+    extra = {"pcg_max_iter": args.pcg_max_iter} if args.mode == "pcg" else {"ordering": args.ordering}
     be = LinearEquation(mode=mode, **extra)
     be.construct(inst)
     rng = np.random.default_rng(args.seed)
@@ -62,6 +64,8 @@ def main(argv=None):
     s.add_argument("--mode", default="fp64", choices=["fp64", "fp32", "pcg"])
     s.add_argument("--pcg-max-iter", type=int, default=60,
                    help="PCG mode: CG iterations per solve (a fixed amount of work)")
+    s.add_argument("--ordering", type=int, default=0, choices=[0, 1],
+                   help="direct modes: 0 AMD (default), 1 nested dissection")
     s.add_argument("--renews", type=int, default=1)
     s.add_argument("--solves", type=int, default=3)
     s.add_argument("--decades", type=float, default=2.0)

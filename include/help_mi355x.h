@@ -322,6 +322,14 @@ void help_lp_free(help_lp* lp);
  * ones) that Crushing.cpp and Presolve_FinalizeModel turn back into the
  * same column layout. */
 int  help_write_mps(const help_lp* lp, const char* path, const char* name);
+/* Binary CSC instance file for LPs beyond the fixed-MPS limits (HELP's
+ * MAX_ROWS / MAX_COLS / MAX_ELEMENTS, LP.h:61-63): magic "HELPCSC1", the
+ * sizes and layout counts, colptr, rowind, val, b, c, ub, and an FNV-1a
+ * checksum.  help_read_csc_bin allocates *lp (free with help_lp_free) and
+ * returns HELP_ERR_ARG on a missing, truncated, corrupted or inconsistent
+ * file. */
+int  help_write_csc_bin(const help_lp* lp, const char* path);
+int  help_read_csc_bin(const char* path, help_lp* lp);
 
 /* ------------------------------------------------------------------ */
 /* HSD driver (restatement of HELP HSDSolver.cpp) over any backend.    */
