@@ -66,7 +66,7 @@ def kernel_table(L, table):
     return rows
 
 
-def roofline(rows, mode, traffic_file):
+def roofline(rows, mode, traffic_file, workload_key):
     dom = max(rows, key=lambda r: r["ms"])
     avg_ms = dom["ms"] / dom["launches"]
     work_per_launch = dom["work"] / dom["launches"]
@@ -81,7 +81,7 @@ def roofline(rows, mode, traffic_file):
     traffic = None
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
-            traffic = json.load(f).get(dom["kernel"])
+            traffic = json.load(f).get(workload_key, {}).get(dom["kernel"])
     return dict(kernel=dom["kernel"], bound=bound, achieved=round(achieved, 4), peak=peak, unit=unit,
                 frac=round(achieved / peak, 5), traffic=traffic, avg_launch_ms=round(avg_ms, 5),
                 work_per_launch=work_per_launch, launches=dom["launches"])
@@ -302,7 +302,11 @@ def main():
         rows = kernel_table(L, gp.table)
         gp.close()
         log(f"profiled solve ({vp}): status {rp.status} iters {rp.iterations} {prof_s:.3f}s")
-    roof = roofline(rows, mode_s, os.path.join(ROOT, "profiles", "traffic.json")) if rows else None
+    # HBM bytes per dispatch from the rocprofv3 counter runs of the same
+    # workload (profiles/traffic.json; null when that workload was not counted)
+    wkey = {0: "random2000", 1: "mcf150x8", 2: "staircase200"}[args.config] + f"_{mode_s}" + \
+        ("_nd" if ordering else "")
+    roof = roofline(rows, mode_s, os.path.join(ROOT, "profiles", "traffic.json"), wkey) if rows else None
     kernels = {r["kernel"]: dict(ms_total=round(r["ms"], 3), launches=r["launches"],
                                  rate=round(r["work"] / (r["ms"] * 1e-3) / (1e9 if r["bound"] == "hbm" else 1e12), 3),
                                  unit="GB/s" if r["bound"] == "hbm" else "TFLOP/s") for r in rows}
