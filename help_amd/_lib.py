@@ -41,7 +41,7 @@ class Opts(C.Structure):
                 ("pcg_tol", C.c_double), ("pcg_fp32_precond", C.c_int32),
                 ("fault_flags", C.c_int32), ("verbose", C.c_int32), ("symbolic_only", C.c_int32),
                 ("front_path", C.c_int32), ("profile", C.c_int32), ("refine_fp64", C.c_int32),
-                ("inverse_solve", C.c_int32), ("ordering", C.c_int32)]
+                ("inverse_solve", C.c_int32), ("ordering", C.c_int32), ("refine_switch", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -52,7 +52,7 @@ class SolveStats(C.Structure):
 
 class RenewStats(C.Structure):
     _fields_ = [("ms_assemble", C.c_double), ("ms_factor", C.c_double),
-                ("ms_precond", C.c_double), ("n_tiny_pivots", C.c_int32)]
+                ("ms_precond", C.c_double), ("n_tiny_pivots", C.c_int32), ("factor_bits", C.c_int32)]
 
 
 class SymbolicInfo(C.Structure):

@@ -876,6 +876,7 @@ void help_opts_default(help_opts* o) {
     o->beta = 1e-5;
     o->refine_max = 20;
     o->refine_tol = 1e-12;
+    o->refine_switch = 2.5;
     o->pcg_k = 500;
     o->pcg_max_iter = 5000;
     o->pcg_tol = 1e-10;
@@ -1014,6 +1015,12 @@ int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* rs) {
             }
             return HELP_OK;
         }
+        // refinement has grown more expensive than the FP32 factor saves:
+        // factor in FP64 from now on
+        if (h->use_fp32 && !h->fp32_off && h->opts.refine_switch > 0 && h->cyc_solves > 0 &&
+            (double)h->cyc_steps > h->opts.refine_switch * (double)h->cyc_solves)
+            h->fp32_off = true;
+        h->cyc_solves = h->cyc_steps = 0;
         const bool f32 = h->use_fp32 && !h->fp32_off;
         if (!f32) ensure_fp64_factor(h);
         assemble(h, f32);
@@ -1031,6 +1038,7 @@ int help_gpu_renew(help_gpu* h, const double* d, help_renew_stats* rs) {
             rs->ms_assemble = elapsed_ms(h->ev[0], h->ev[1]);
             rs->ms_factor = elapsed_ms(h->ev[1], h->ev[2]);
             rs->n_tiny_pivots = nerr;
+            rs->factor_bits = f32 ? 32 : 64;
         }
         return HELP_OK;
     } catch (const std::exception& e) {
@@ -1101,6 +1109,7 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
             int it = h->factor_fp32 ? pcg_factor<float>(h, rel, h->opts.refine_max)
                                     : pcg_factor<double>(h, rel, h->fp32_off ? h->opts.refine_max
                                                                              : h->opts.refine_fp64);
+            if (h->factor_fp32) { h->cyc_solves += 1; h->cyc_steps += it; }
             if (h->factor_fp32 &&
                 (!(rel <= h->opts.refine_tol) ||
                  (h->opts.fault_flags & (HELP_FAULT_FP32_BREAKDOWN | HELP_FAULT_REFINE_NOCONV)))) {

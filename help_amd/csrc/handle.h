@@ -129,7 +129,8 @@ struct help_gpu {
     bool factored = false;
     bool use_fp32 = false;     // FP32 mode requested
     bool factor_fp32 = false;  // precision of the current factor
-    bool fp32_off = false;     // FP32 abandoned after a refinement failure
+    bool fp32_off = false;     // FP32 abandoned (refinement failure or its cost)
+    int64_t cyc_solves = 0, cyc_steps = 0;   // FP32 solves / refinement steps since the last renew
     int32_t last_tiny = 0;
     // device state
     hipStream_t st = nullptr;

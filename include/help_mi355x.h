@@ -82,13 +82,19 @@ typedef struct help_opts {
                                 dissection, the family of the METIS ordering
                                 HELP's PARDISO backend asks for,
                                 UsePardiso.cpp:101)                            */
+    double  refine_switch;   /* FP32 mode: when the solves since the last renew
+                                took more than this many refinement steps on
+                                average, the next renews factor in FP64 (d's
+                                spread only grows along an IPM run, and each
+                                step costs a triangular-solve pair); 0: only a
+                                refinement failure switches                    */
 } help_opts;
 #define HELP_ORDER_AMD 0
 #define HELP_ORDER_ND  1
 
 /* Defaults: FP64 direct, beta=1e-5, refine 20 / 1e-12 (FP32 mode; FP64 mode
- * refine_fp64 0), pcg_k 500, pcg_max_iter 5000, pcg_tol 1e-10, FP32
- * preconditioner. */
+ * refine_fp64 0), refine_switch 2.5, pcg_k 500, pcg_max_iter 5000, pcg_tol
+ * 1e-10, FP32 preconditioner, AMD ordering. */
 void help_opts_default(help_opts* o);
 
 typedef struct help_solve_stats {
@@ -105,6 +111,7 @@ typedef struct help_renew_stats {
     double  ms_factor;       /* numeric factorization                           */
     double  ms_precond;      /* partial-Cholesky build (PCG modes)              */
     int32_t n_tiny_pivots;   /* pivots replaced in the FP32 factor              */
+    int32_t factor_bits;     /* 32 or 64: precision of this renew's factor      */
 } help_renew_stats;
 
 typedef struct help_gpu help_gpu;   /* opaque handle */

@@ -273,3 +273,64 @@ def test_ipm_trajectory_staircase_t10(mode):
         assert bwd_g <= max(10 * bwd_o, 1e-12), (k, bwd_g, bwd_o)
         if k == 1:
             assert fwd <= 1e-10, (k, fwd)
+
+
+def test_fp32_mode_switches_to_fp64_on_refinement_cost():
+    """FP32 mode: when the solves since the last renew averaged more than
+    refine_switch refinement steps, the next renew factors in FP64 (and so
+    do all later ones); refine_switch = 0 keeps FP32 until refinement fails.
+    The solutions stay within the refinement tolerance either way."""
+    inst = instances.staircase(8, 300, 900, 5, seed=24)
+    rng = np.random.default_rng(8)
+    d = np.exp(rng.uniform(-4, 4, inst.n) * np.log(10.0))
+    b1, b2 = rng.standard_normal(inst.n), rng.standard_normal(inst.m)
+    bits = {}
+    for sw in (0.5, 0.0):
+        be = LinearEquation(mode=L.MODE_DIRECT_FP32, ordering=L.ORDER_ND, refine_switch=sw)
+        be.construct(inst)
+        seen = []
+        for _ in range(3):
+            be.renew(d)
+            seen.append(be.renew_stats.factor_bits)
+            x1, x2 = be.solve(d, b1, b2)
+            steps = be.solve_stats.refine_steps
+            assert _normal_residual(inst, d, b1, b2, x2) <= 1e-11
+        be.destruct()
+        bits[sw] = seen
+        _flush(f"refine_switch={sw}: factor bits per renew {seen} (last solve {steps} refinement steps)")
+    assert bits[0.5] == [32, 64, 64]
+    assert bits[0.0] == [32, 32, 32]
+
+
+def test_hsd_mcf150_full_size_against_oracle_fixture():
+    """The full 150x150-grid, K=8 multicommodity-flow LP (m=269,392,
+    n=804,600) through the GPU-resident driver, plain FP64, against the
+    oracle's runs in tests/golden/mcf150_k8_oracle.json (written by
+    make_mcf150_fixture.py: the CPU FP64 oracle, CHOLMOD semantics, same
+    ordering).  The oracle's two runs differ only in the order in which the
+    HSD driver sums its reductions, yet one ends OPTIMAL at iteration 188 and
+    the other (HELP's own summation order) stops on the inaccurate-direction
+    test at iteration 193 with gap 1.5e-8: on this instance the last
+    iterations are decided by rounding, so status and iteration count are
+    not reproducible across implementations.  Pinned instead: the stop test
+    (gap, primal and dual infeasibility <= 1e-8) and the objective within
+    1e-8 relative of the oracle's optimal run; the iteration count is
+    printed."""
+    fix = json.load(open(os.path.join(HERE, "golden", "mcf150_k8_oracle.json")))
+    inst = instances.mcf(150, 8, 20, 0)
+    assert (inst.m, inst.n, inst.nnz) == (fix["instance"]["m"], fix["instance"]["n"], fix["instance"]["nnz"])
+    gb = GpuBackend(mode=L.MODE_DIRECT_FP64)
+    try:
+        res, _ = solve_lp(inst, gb.table, device=True)
+    finally:
+        gb.close()
+    runs = fix["runs"]
+    _flush(f"mcf150 K=8: gpu status {L.HSD_STATUS[res.status]} iterations {res.iterations} gap {res.rel_gap:.2e} "
+           f"pobj {res.primal_obj!r} | oracle " +
+           ", ".join(f"{k}: {r['status']} {r['iterations']} {r['primal_obj']!r}" for k, r in runs.items()))
+    assert res.backend_error == 0 and res.status == 0
+    assert res.rel_gap <= 1e-8 and res.primal_infeas <= 1e-8 and res.dual_infeas <= 1e-8
+    optimal = [r for r in runs.values() if r["status"] == "OK"]
+    assert optimal
+    for r in optimal:
+        assert abs(res.primal_obj - r["primal_obj"]) / (1 + abs(r["primal_obj"])) <= 1e-8
