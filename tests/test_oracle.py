@@ -255,3 +255,64 @@ def test_pcg_finite_termination_and_exact_preconditioner():
     assert it <= inst.m + 2 and rel < 1e-12
     it, x, rel = _pcg(inst, d, inst.m, b, tol=1e-12, max_iter=200)
     assert it <= 2 and rel < 1e-12
+
+
+def _ref_helper():
+    import ctypes as C
+    import os
+    path = os.path.join(O.REF_DIR, "libref_helper.so")
+    if not os.path.exists(path):
+        pytest.skip("reference Helper harness not built (no reference sources)")
+    lib = C.CDLL(path)
+    D, I32, I64 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    lib.ref_load.restype = C.c_int
+    lib.ref_load.argtypes = [C.c_int32, C.c_int32, I64, I32, D]
+    lib.ref_adat_pattern.restype = C.c_int64
+    lib.ref_adat_pattern.argtypes = [I64, I32]
+    lib.ref_adat_values.restype = None
+    lib.ref_adat_values.argtypes = [D, D]
+    lib.ref_a_times.restype = None
+    lib.ref_a_times.argtypes = [C.c_int32, C.c_int32, D, D]
+    return lib
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: instances.random_lp(300, 600, 4, seed=71),
+    lambda: instances.mcf(side=8, commodities=2, pairs=4, seed=72),
+    lambda: instances.staircase(5, 40, 120, 5, seed=73),
+    lambda: instances.zipf(2000, 6000, 6, 1.1, seed=74),
+    lambda: instances.random_lp(300, 600, 4, seed=75, ub_cols=100, free_cols=5)])
+def test_oracle_matches_reference_helper_bitwise(maker):
+    """The oracle's ADA^T pattern (or_adat_count/fill) and values
+    (or_adat_values) and its products with A and A^T (or_a_times,
+    or_at_times) against HELP's own Helper.cpp compiled from the reference
+    sources (ADAt_Allocate / ADAt_Calc in the linker form UsePardiso.cpp
+    calls, Helper.cpp:273-377; SetATimesVector, Helper.cpp:110-156), bit for
+    bit, on d spread over 16 decades and both signs."""
+    import ctypes as C
+    lib = _ref_helper()
+    inst = maker()
+    assert lib.ref_load(inst.m, inst.n, O.i64(inst.colptr), O.i32(inst.rowind), O.d(inst.val)) == 0
+    rp = np.empty(inst.m + 1, np.int64)
+    nnz = lib.ref_adat_pattern(O.i64(rp), C.cast(None, C.POINTER(C.c_int32)))
+    ci = np.empty(nnz, np.int32)
+    lib.ref_adat_pattern(O.i64(rp), O.i32(ci))
+    arp, aci = O.adat_pattern(inst)
+    assert np.array_equal(arp, rp) and np.array_equal(aci, ci)
+    rng = np.random.default_rng(76)
+    d = np.exp(rng.uniform(-8, 8, inst.n) * np.log(10.0))
+    ref_vals = np.empty(nnz)
+    lib.ref_adat_values(O.d(d), O.d(ref_vals))
+    vals = O.adat_values(inst, d, arp, aci)
+    assert np.array_equal(vals.view(np.uint64), ref_vals.view(np.uint64))
+    for trans in (0, 1):
+        nin, nout = (inst.n, inst.m) if trans == 0 else (inst.m, inst.n)
+        v = rng.standard_normal(nin)
+        y0 = rng.standard_normal(nout)
+        for sign in (1, -1):
+            yr = y0.copy()
+            lib.ref_a_times(trans, sign, O.d(v), O.d(yr))
+            yo = y0.copy()
+            f = O.lib().or_a_times if trans == 0 else O.lib().or_at_times
+            f(inst.m, inst.n, O.i64(inst.colptr), O.i32(inst.rowind), O.d(inst.val), sign, O.d(v), O.d(yo))
+            assert np.array_equal(yo.view(np.uint64), yr.view(np.uint64)), (trans, sign)
