@@ -516,10 +516,13 @@ void setup_device_common(help_gpu* h) {
 }
 
 // symbolic structures, schedules and factor storage of the direct modes
+void ensure_fp64_factor(help_gpu* h);
+
 void setup_device_factor(help_gpu* h) {
     hipStream_t st = h->st;
     const Supernodal& S = h->S;
     h->d_pairptr.upload(h->P.pairptr, st);
+    h->d_wnz.alloc(std::max<int64_t>(1, h->nnz));
     h->d_pair_p.upload(h->P.pair_p, st);
     h->d_pair_q.upload(h->P.pair_q, st);
     h->d_colof.upload(h->colof, st);
@@ -572,6 +575,9 @@ void setup_device_factor(help_gpu* h) {
             h->d_X32.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
             h->d_S32.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
         }
+        // the FP64 factor the run may switch to (refinement cost, or a
+        // refinement failure) is allocated here, not in the middle of a run
+        ensure_fp64_factor(h);
     } else {
         h->d_L64.alloc(std::max<int64_t>(1, nl));
         h->d_U64.alloc(std::max<int64_t>(1, nu));
@@ -831,7 +837,7 @@ void assemble(help_gpu* h, bool f32) {
         {
             ProfScope ps(h, HELP_K_ADAT, h->work_adat);
             launch_adat_scaled_f32(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
-                                   h->d_dinv.p, nslot, h->d_slot_row.p, h->d_slot_col.p, h->d_scale.p,
+                                   h->d_dinv.p, h->nnz, h->d_wnz.p, nslot, h->d_slot_row.p, h->d_slot_col.p, h->d_scale.p,
                                    h->d_dest.p, h->d_L32.p, st);
         }
         launch_add_beta_scaled_f32(h->d_diagpos.p, m, h->opts.beta, h->d_scale.p, h->d_L32.p, st);
@@ -841,7 +847,7 @@ void assemble(help_gpu* h, bool f32) {
         {
             ProfScope ps(h, HELP_K_ADAT, h->work_adat);
             launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
-                                 h->d_dinv.p, nslot, h->d_dest.p, h->d_L64.p, nullptr, st);
+                                 h->d_dinv.p, h->nnz, h->d_wnz.p, nslot, h->d_dest.p, h->d_L64.p, nullptr, st);
         }
         launch_add_beta(h->d_diagpos.p, m, h->opts.beta, h->d_L64.p, nullptr, st);
     }
@@ -1068,8 +1074,8 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1, const double*
                                  h->d_rp.p, false, st);
                 launch_sub(m, h->d_rp.p, h->d_b2.p, h->d_rp.p, st);
             } else {
-                launch_rhs(m, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p, h->d_d.p, h->d_b2.p,
-                           h->d_iperm.p, h->d_rp.p, st);
+                launch_rhs(m, n, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p, h->d_d.p, h->d_b2.p,
+                           h->d_iperm.p, h->d_tn.p, h->d_rp.p, st);
             }
         }
         HIPCHK(hipEventRecord(h->ev[1], st));
@@ -1189,8 +1195,8 @@ int help_gpu_solve2(help_gpu* h, const double* d, const double* b1a, const doubl
         {
             ProfScope ps(h, HELP_K_RHS, 2 * h->work_rhs);
             for (int r = 0; r < 2; ++r)
-                launch_rhs(m, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p + (size_t)r * n, h->d_d.p,
-                           h->d_b2.p + (size_t)r * m, h->d_iperm.p, h->d_rp.p + (size_t)r * m, st);
+                launch_rhs(m, n, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p + (size_t)r * n, h->d_d.p,
+                           h->d_b2.p + (size_t)r * m, h->d_iperm.p, h->d_tn.p, h->d_rp.p + (size_t)r * m, st);
         }
         HIPCHK(hipEventRecord(h->ev[1], st));
         // the FP64 factor solve of both right-hand sides, then each one's
@@ -1289,7 +1295,7 @@ int help_gpu_adat_values(help_gpu* h, const double* d, double* vals) {
         HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * h->n, hipMemcpyHostToDevice, st));
         launch_dinv(h->d_d.p, h->d_dinv.p, h->n, st);
         launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
-                             h->d_dinv.p, nslot, nullptr, h->d_slotvals.p, nullptr, st);
+                             h->d_dinv.p, h->nnz, h->d_wnz.p, nslot, nullptr, h->d_slotvals.p, nullptr, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(vals, h->d_slotvals.p, sizeof(double) * nslot, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));

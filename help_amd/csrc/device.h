@@ -80,17 +80,18 @@ constexpr int kSmallElems = 4032;  // panel elements of a small front; with the 
 
 // ---- sparse kernels (kernels_sparse.hip) ----
 void launch_dinv(const double* d, double* dinv, int32_t n, hipStream_t st);
+// w (nnz doubles) receives A_ij * dinv_j per nonzero on the way
 void launch_adat_assemble(const int64_t* pairptr, const int32_t* pair_p, const int32_t* pair_q,
-                          const int32_t* colof, const double* val, const double* dinv,
-                          int64_t nslots, const int64_t* dest, double* out_d, float* out_f,
+                          const int32_t* colof, const double* val, const double* dinv, int64_t nnz,
+                          double* w, int64_t nslots, const int64_t* dest, double* out_d, float* out_f,
                           hipStream_t st);
 void launch_add_beta(const int64_t* diagpos, int32_t m, double beta, double* Ld, float* Lf,
                      hipStream_t st);
 // FP32 factor of S (A D^-1 A^T + beta I) S with S = diag(1/sqrt(diag(A D^-1 A^T) + beta))
 void launch_adat_scaled_f32(const int64_t* pairptr, const int32_t* pair_p, const int32_t* pair_q,
-                            const int32_t* colof, const double* val, const double* dinv, int64_t nslots,
-                            const int32_t* slot_row, const int32_t* slot_col, const double* scale,
-                            const int64_t* dest, float* out_f, hipStream_t st);
+                            const int32_t* colof, const double* val, const double* dinv, int64_t nnz,
+                            double* w, int64_t nslots, const int32_t* slot_row, const int32_t* slot_col,
+                            const double* scale, const int64_t* dest, float* out_f, hipStream_t st);
 void launch_add_beta_scaled_f32(const int64_t* diagpos, int32_t m, double beta, const double* scale,
                                 float* Lf, hipStream_t st);
 void launch_diag_scale(int32_t m, const double* diag, double beta, double* s, hipStream_t st);
@@ -99,8 +100,9 @@ void launch_spmv_csr(int32_t m, const int64_t* rowptr, const int32_t* colind, co
                      const double* x, double* y, hipStream_t st);
 void launch_spmv_csc_t(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
                        const double* x, double* y, hipStream_t st);
-void launch_rhs(int32_t m, const int64_t* rowptr, const int32_t* colind, const double* csrval,
-                const double* b1, const double* d, const double* b2, const int32_t* iperm,
+// t (n doubles) receives b1 ./ d on the way
+void launch_rhs(int32_t m, int32_t n, const int64_t* rowptr, const int32_t* colind, const double* csrval,
+                const double* b1, const double* d, const double* b2, const int32_t* iperm, double* t,
                 double* rp, hipStream_t st);
 void launch_backsub(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,
                     const double* b1, const double* d, const double* x2, double* x1,

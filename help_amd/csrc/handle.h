@@ -142,6 +142,7 @@ struct help_gpu {
         d_rowidx, d_relmap, d_childptr, d_child, d_list_small, d_list_fwd, d_list_bwd, d_err;
     DBuf<double> d_val, d_csrval, d_d, d_dinv, d_b1, d_b2, d_x1, d_x2, d_rp, d_W, d_t, d_tn, d_r,
         d_c, d_z, d_p, d_q, d_part, d_scal, d_slotvals;
+    DBuf<double> d_wnz;   // A_ij / d_j per nonzero (the assembly's first factor)
     DBuf<double> d_L64, d_U64, d_Dinv64, d_Uacc64;
     DBuf<double> d_Lstage64;   // L11 of the fused panels (Dinv layout) until the final copy
     // FP32 mode: symmetric diagonal scaling of the factored matrix

@@ -16,6 +16,7 @@
 #include <climits>
 #include <cmath>
 
+#include "device.h"
 #include "launch.h"
 #include "pcg.h"
 
@@ -418,6 +419,92 @@ __global__ void k_pca_final(int32_t K, const int32_t* __restrict__ perm, const d
     z[perm[i]] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
+// ---- conjugate gradients with the scalars on the device ----
+// s[kCgRR] = r.r, s[kCgPQ] = p.q, s[kCgRZ + 0/1] = r.z of alternate
+// iterations, s[kCgState] = 0 running / 1 converged / 2 breakdown,
+// s[kCgIters] = iterations started.  Every kernel of an iteration returns at
+// once when the state is not 0, so iterations the host enqueued after the
+// stop change nothing.  The block partition of the reductions is that of
+// k_dot_partial (kDotPartials blocks, grid-stride), so every sum is the one the
+// separate dot kernels form.
+__device__ inline bool cg_stopped(const double* s) { return s[kCgState] != 0.0; }
+
+__device__ inline void block_sum_to(double acc, double* __restrict__ dst) {
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *dst = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_cg_dot_partial(const double* __restrict__ a, const double* __restrict__ b,
+                                                        int64_t n, double* __restrict__ part,
+                                                        const double* __restrict__ s) {
+    if (cg_stopped(s)) return;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        acc = acc + a[i] * b[i];
+    block_sum_to(acc, part + blockIdx.x);
+}
+
+// s[slot] = sum of the partials; kind 1 (p.q): the iteration has started,
+// and it breaks down unless p.q > 0 and the current r.z > 0; kind 2 (r.r):
+// converged when sqrt(r.r) <= target
+__global__ __launch_bounds__(256) void k_cg_final(const double* __restrict__ part, int np, double* __restrict__ s,
+                                                  int slot, int kind, int rz_cur, double target) {
+    if (cg_stopped(s)) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) acc = acc + part[i];
+    __shared__ double tot;
+    block_sum_to(acc, &tot);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double v = tot;
+    s[slot] = v;
+    if (kind == 1) {
+        s[kCgIters] += 1.0;
+        if (!(v > 0) || !(s[rz_cur] > 0)) s[kCgState] = 2.0;
+    } else if (kind == 2) {
+        if (sqrt(v) <= target) s[kCgState] = 1.0;
+    }
+}
+
+// alpha = r.z / p.q;  x += alpha p;  r -= alpha q;  partials of r.r
+__global__ __launch_bounds__(256) void k_cg_update(int64_t n, const double* __restrict__ s, int rz_cur,
+                                                   const double* __restrict__ p, const double* __restrict__ q,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ part) {
+    if (cg_stopped(s)) return;
+    const double alpha = s[rz_cur] / s[kCgPQ];
+    const double nalpha = -alpha;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        x[i] = x[i] + alpha * p[i];
+        const double ri = r[i] + nalpha * q[i];
+        r[i] = ri;
+        acc = acc + ri * ri;
+    }
+    block_sum_to(acc, part + blockIdx.x);
+}
+
+// beta = r.z(new) / r.z(old);  p = z + beta p
+__global__ void k_cg_xpby(int64_t n, const double* __restrict__ s, int rz_cur, int rz_next,
+                          const double* __restrict__ z, double* __restrict__ p) {
+    if (cg_stopped(s)) return;
+    const double beta = s[rz_next] / s[rz_cur];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+__global__ void k_cg_init(double* __restrict__ s, double rr, double rz) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        s[kCgRR] = rr; s[kCgPQ] = 0.0; s[kCgRZ] = rz; s[kCgRZ + 1] = 0.0; s[kCgState] = 0.0; s[kCgIters] = 0.0;
+    }
+}
+
 }  // namespace
 
 void build_row_split(int32_t m, const std::vector<int64_t>& rowptr, std::vector<RowChunk>& chunks,
@@ -519,5 +606,28 @@ template void launch_pc_apply<float>(int32_t, int32_t, const int32_t*, const int
 template void launch_pc_apply<double>(int32_t, int32_t, const int32_t*, const int32_t*, const double*,
                                        const double*, const double*, const double*, double*, double*, double*,
                                        hipStream_t);
+
+void launch_cg_init(double* s, double rr, double rz, hipStream_t st) {
+    help::launch_kernel(k_cg_init, 1, 64, 0, st, s, rr, rz);
+}
+
+// one CG iteration after the matrix product q = M p and before the
+// preconditioner: p.q (breakdown test), the x/r update with r.r
+// (convergence test); then, after z = P^-1 r, r.z and the new direction
+void launch_cg_step1(int64_t n, double* s, int rz_cur, double target, const double* p, const double* q, double* x,
+                     double* r, double* part, hipStream_t st) {
+    const int nb = (int)std::min<int64_t>(kDotPartials, std::max<int64_t>(1, (n + 255) / 256));
+    help::launch_kernel(k_cg_dot_partial, nb, 256, 0, st, p, q, n, part, (const double*)s);
+    help::launch_kernel(k_cg_final, 1, 256, 0, st, (const double*)part, nb, s, (int)kCgPQ, 1, rz_cur, target);
+    help::launch_kernel(k_cg_update, nb, 256, 0, st, n, (const double*)s, rz_cur, p, q, x, r, part);
+    help::launch_kernel(k_cg_final, 1, 256, 0, st, (const double*)part, nb, s, (int)kCgRR, 2, rz_cur, target);
+}
+void launch_cg_step2(int64_t n, double* s, int rz_cur, int rz_next, const double* r, const double* z, double* p,
+                     double* part, hipStream_t st) {
+    const int nb = (int)std::min<int64_t>(kDotPartials, std::max<int64_t>(1, (n + 255) / 256));
+    help::launch_kernel(k_cg_dot_partial, nb, 256, 0, st, r, z, n, part, (const double*)s);
+    help::launch_kernel(k_cg_final, 1, 256, 0, st, (const double*)part, nb, s, rz_next, 0, rz_cur, 0.0);
+    help::launch_kernel(k_cg_xpby, (unsigned)((n + 255) / 256), 256, 0, st, n, (const double*)s, rz_cur, rz_next, z, p);
+}
 
 }  // namespace help

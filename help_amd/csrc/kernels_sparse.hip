@@ -42,22 +42,29 @@ __global__ void k_dinv(const double* __restrict__ d, double* __restrict__ dinv, 
     if (j < n) dinv[j] = 1.0 / d[j];   // UsePardiso.cpp:172-173
 }
 
+// w_p = A_ij * dinv_j for every nonzero p = (i,j) of A (CSC order: the
+// column index is nondecreasing in p, so the dinv reads stream): the factor
+// (A_ij * dinv_j) of Helper.cpp:360, computed once per nonzero instead of
+// once per product it enters.
+__global__ void k_scale_nz(int64_t nnz, const int32_t* __restrict__ colof, const double* __restrict__ val,
+                           const double* __restrict__ dinv, double* __restrict__ w) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) w[p] = val[p] * dinv[colof[p]];
+}
+
 // One slot (i,k) of the upper ADA^T per thread: sum over the columns j it
 // shares with, in ascending j, of (A_ij * dinv_j) * A_kj starting from zero
-// (Helper.cpp:358-373).  The result goes to dest[s] (a position in the
-// supernodal factor storage) or to slot s when dest is null.
+// (Helper.cpp:358-373), with w_p = A_ij * dinv_j from k_scale_nz.  The
+// result goes to dest[s] (a position in the supernodal factor storage) or
+// to slot s when dest is null.
 __global__ void k_adat(const int64_t* __restrict__ pairptr, const int32_t* __restrict__ pp,
-                       const int32_t* __restrict__ pq, const int32_t* __restrict__ colof,
-                       const double* __restrict__ val, const double* __restrict__ dinv,
-                       int64_t nslots, const int64_t* __restrict__ dest, double* __restrict__ outd,
-                       float* __restrict__ outf) {
+                       const int32_t* __restrict__ pq, const double* __restrict__ w,
+                       const double* __restrict__ val, int64_t nslots, const int64_t* __restrict__ dest,
+                       double* __restrict__ outd, float* __restrict__ outf) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
-    const double acc = seq_acc<4, false>(pairptr[s], pairptr[s + 1], 0.0, [&](int64_t t) {
-        const int32_t p = pp[t], q = pq[t];
-        const double vpdj = val[p] * dinv[colof[p]];
-        return vpdj * val[q];
-    });
+    const double acc = seq_acc<4, false>(pairptr[s], pairptr[s + 1], 0.0,
+                                         [&](int64_t t) { return w[pp[t]] * val[pq[t]]; });
     int64_t o = dest ? dest[s] : s;
     if (outd) outd[o] = acc;
     else outf[o] = (float)acc;
@@ -76,18 +83,15 @@ __global__ void k_add_beta(const int64_t* __restrict__ diagpos, int32_t m, doubl
 // S (A D^-1 A^T + beta I) S, S = diag(1/sqrt(diag)), so every entry is at
 // most 1 in magnitude and the FP32 range cannot overflow
 __global__ void k_adat_scaled_f32(const int64_t* __restrict__ pairptr, const int32_t* __restrict__ pp,
-                                  const int32_t* __restrict__ pq, const int32_t* __restrict__ colof,
-                                  const double* __restrict__ val, const double* __restrict__ dinv,
-                                  int64_t nslots, const int32_t* __restrict__ slot_row,
-                                  const int32_t* __restrict__ slot_col, const double* __restrict__ scale,
-                                  const int64_t* __restrict__ dest, float* __restrict__ outf) {
+                                  const int32_t* __restrict__ pq, const double* __restrict__ w,
+                                  const double* __restrict__ val, int64_t nslots,
+                                  const int32_t* __restrict__ slot_row, const int32_t* __restrict__ slot_col,
+                                  const double* __restrict__ scale, const int64_t* __restrict__ dest,
+                                  float* __restrict__ outf) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
-    const double acc = seq_acc<4, false>(pairptr[s], pairptr[s + 1], 0.0, [&](int64_t t) {
-        const int32_t p = pp[t], q = pq[t];
-        const double vpdj = val[p] * dinv[colof[p]];
-        return vpdj * val[q];
-    });
+    const double acc = seq_acc<4, false>(pairptr[s], pairptr[s + 1], 0.0,
+                                         [&](int64_t t) { return w[pp[t]] * val[pq[t]]; });
     outf[dest[s]] = (float)(acc * scale[slot_row[s]] * scale[slot_col[s]]);
 }
 
@@ -133,18 +137,23 @@ __global__ void k_spmv_csc_t(int32_t n, const int64_t* __restrict__ colptr,
 // r_i = -b2_i + sum_j A_ij (b1_j / d_j), columns ascending (cholmod_sdmult
 // with alpha = beta = 1 on Y = -b2, UseCholmod.cpp:173-179), written to the
 // permuted position iperm[i] (UseCholmod.cpp:180-182).
+// t_j = b1_j / d_j (UseCholmod.cpp:176-177), then per row the gather of t
+__global__ void k_div(int32_t n, const double* __restrict__ b1, const double* __restrict__ d,
+                      double* __restrict__ t) {
+    int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) t[j] = b1[j] / d[j];
+}
+
+// rp[iperm[i]] = -b2_i + sum_j A_ij t_j, over the row's columns in
+// ascending order (UseCholmod.cpp:173-182: A (b1 ./ d) - b2, permuted)
 __global__ void k_rhs(int32_t m, const int64_t* __restrict__ rowptr,
                       const int32_t* __restrict__ colind, const double* __restrict__ csrval,
-                      const double* __restrict__ b1, const double* __restrict__ d,
-                      const double* __restrict__ b2, const int32_t* __restrict__ iperm,
-                      double* __restrict__ rp) {
+                      const double* __restrict__ t, const double* __restrict__ b2,
+                      const int32_t* __restrict__ iperm, double* __restrict__ rp) {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    const double acc = seq_acc<8, false>(rowptr[i], rowptr[i + 1], -b2[i], [&](int64_t t) {
-        const int32_t j = colind[t];
-        const double tj = b1[j] / d[j];
-        return csrval[t] * tj;
-    });
+    const double acc = seq_acc<8, false>(rowptr[i], rowptr[i + 1], -b2[i],
+                                         [&](int64_t q) { return csrval[q] * t[colind[q]]; });
     rp[iperm[i]] = acc;
 }
 
@@ -268,22 +277,23 @@ void launch_dinv(const double* d, double* dinv, int32_t n, hipStream_t st) {
 }
 
 void launch_adat_assemble(const int64_t* pairptr, const int32_t* pair_p, const int32_t* pair_q,
-                          const int32_t* colof, const double* val, const double* dinv,
-                          int64_t nslots, const int64_t* dest, double* out_d, float* out_f,
+                          const int32_t* colof, const double* val, const double* dinv, int64_t nnz,
+                          double* w, int64_t nslots, const int64_t* dest, double* out_d, float* out_f,
                           hipStream_t st) {
-    if (nslots > 0)
-        help::launch_kernel(k_adat, grid_for(nslots), kBlock, 0, st, pairptr, pair_p, pair_q, colof, val, dinv,
-                                                    nslots, dest, out_d, out_f);
+    if (nslots <= 0) return;
+    if (nnz > 0) help::launch_kernel(k_scale_nz, grid_for(nnz), kBlock, 0, st, nnz, colof, val, dinv, w);
+    help::launch_kernel(k_adat, grid_for(nslots), kBlock, 0, st, pairptr, pair_p, pair_q, (const double*)w, val,
+                            nslots, dest, out_d, out_f);
 }
 
 void launch_adat_scaled_f32(const int64_t* pairptr, const int32_t* pair_p, const int32_t* pair_q,
-                            const int32_t* colof, const double* val, const double* dinv, int64_t nslots,
-                            const int32_t* slot_row, const int32_t* slot_col, const double* scale,
-                            const int64_t* dest, float* out_f, hipStream_t st) {
-    if (nslots > 0)
-        help::launch_kernel(k_adat_scaled_f32, grid_for(nslots), kBlock, 0, st, pairptr, pair_p, pair_q, colof,
-                                                               val, dinv, nslots, slot_row, slot_col, scale,
-                                                               dest, out_f);
+                            const int32_t* colof, const double* val, const double* dinv, int64_t nnz,
+                            double* w, int64_t nslots, const int32_t* slot_row, const int32_t* slot_col,
+                            const double* scale, const int64_t* dest, float* out_f, hipStream_t st) {
+    if (nslots <= 0) return;
+    if (nnz > 0) help::launch_kernel(k_scale_nz, grid_for(nnz), kBlock, 0, st, nnz, colof, val, dinv, w);
+    help::launch_kernel(k_adat_scaled_f32, grid_for(nslots), kBlock, 0, st, pairptr, pair_p, pair_q,
+                            (const double*)w, val, nslots, slot_row, slot_col, scale, dest, out_f);
 }
 
 void launch_add_beta_scaled_f32(const int64_t* diagpos, int32_t m, double beta, const double* scale,
@@ -314,10 +324,12 @@ void launch_spmv_csc_t(int32_t n, const int64_t* colptr, const int32_t* rowind, 
     if (n > 0) help::launch_kernel(k_spmv_csc_t, grid_for(n), kBlock, 0, st, n, colptr, rowind, val, x, y);
 }
 
-void launch_rhs(int32_t m, const int64_t* rowptr, const int32_t* colind, const double* csrval,
-                const double* b1, const double* d, const double* b2, const int32_t* iperm,
+void launch_rhs(int32_t m, int32_t n, const int64_t* rowptr, const int32_t* colind, const double* csrval,
+                const double* b1, const double* d, const double* b2, const int32_t* iperm, double* t,
                 double* rp, hipStream_t st) {
-    if (m > 0) help::launch_kernel(k_rhs, grid_for(m), kBlock, 0, st, m, rowptr, colind, csrval, b1, d, b2, iperm, rp);
+    if (n > 0) help::launch_kernel(k_div, grid_for(n), kBlock, 0, st, n, b1, d, t);
+    if (m > 0) help::launch_kernel(k_rhs, grid_for(m), kBlock, 0, st, m, rowptr, colind, csrval, (const double*)t, b2,
+                                   iperm, rp);
 }
 
 void launch_backsub(int32_t n, const int64_t* colptr, const int32_t* rowind, const double* val,

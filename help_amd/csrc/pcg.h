@@ -73,6 +73,19 @@ void launch_pc_apply(int32_t m, int32_t K, const int32_t* perm, const int32_t* p
                      const double* X, const double* S, const double* r, double* y1, double* ltpart,
                      double* z, hipStream_t st);
 
+// ---- conjugate gradients with device-resident scalars ----
+// slots of the scalar array: r.r, p.q, r.z of alternate iterations, the
+// state (0 running, 1 converged, 2 breakdown) and the iterations started
+constexpr int kCgRR = 0, kCgPQ = 1, kCgRZ = 2, kCgState = 4, kCgIters = 5, kCgSlots = 6;
+void launch_cg_init(double* s, double rr, double rz, hipStream_t st);
+// after q = M p: p.q with the breakdown test, x += alpha p and r -= alpha q
+// with r.r and the convergence test sqrt(r.r) <= target
+void launch_cg_step1(int64_t n, double* s, int rz_cur, double target, const double* p, const double* q, double* x,
+                     double* r, double* part, hipStream_t st);
+// after z = P^-1 r: r.z into slot rz_next, p = z + (r.z new / r.z old) p
+void launch_cg_step2(int64_t n, double* s, int rz_cur, int rz_next, const double* r, const double* z, double* p,
+                     double* part, hipStream_t st);
+
 struct PcgState;
 // host side (pcg.hip): allocation at create, the partial-Cholesky build at
 // renew, the conjugate-gradient solve (rhs in, x2 out, device vectors)

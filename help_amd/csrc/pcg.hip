@@ -44,6 +44,14 @@ namespace help {
 
 struct PcgState {
     int32_t K = 0;
+    DBuf<double> cg, part;           // CG scalars (kCgSlots) and reduction partials
+    double* hpin = nullptr;          // pinned: the scalars of the last two iterations
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~PcgState() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (hpin) (void)hipHostFree(hpin);
+    }
     DBuf<double> L, D, S, w, g, sc, L11, X, y1, ltpart, pval;
     DBuf<float> L32;
     DBuf<int32_t> perm, pos, col, pidx;
@@ -78,6 +86,10 @@ void pcg_setup(help_gpu* h) {
     P->col.alloc(1);
     P->x.alloc(m); P->r.alloc(m); P->z.alloc(m); P->p.alloc(m); P->q.alloc(m);
     P->t.alloc(n);
+    P->cg.alloc(kCgSlots);
+    P->part.alloc(kDotPartials);
+    HIPCHK(hipHostMalloc((void**)&P->hpin, 2 * kCgSlots * sizeof(double), hipHostMallocDefault));
+    for (auto& e : P->ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     P->iota.resize(m);
     for (int32_t i = 0; i < m; ++i) P->iota[i] = i;
     HIPCHK(hipMemsetAsync(P->sc.p, 0, sizeof(double) * n, st));
@@ -142,6 +154,13 @@ static double dot(help_gpu* h, const double* a, const double* b, int64_t n) {
 // zero), all vectors on the device in the original row order.  Stops at
 // ||r|| <= pcg_tol ||r0|| or pcg_max_iter iterations; x2 <- x.  Returns the
 // iteration count; rel is the final ||r|| / ||r0||.
+//
+// The scalars of the iteration (p.q, r.r, r.z, alpha, beta) stay on the
+// device (launch_cg_step1/2): no host round trip inside an iteration.  The
+// host copies the state of every iteration to pinned memory and reads it one
+// iteration later, while the next one is already queued, and stops queueing
+// once it sees convergence or breakdown; the one extra iteration in flight
+// then changes nothing (its kernels see the stopped state and return).
 int pcg_solve(help_gpu* h, const double* rhs, const double* x0, double* x2, double& rel) {
     PcgState* P = h->pcg;
     hipStream_t st = h->st;
@@ -175,27 +194,34 @@ int pcg_solve(help_gpu* h, const double* rhs, const double* x0, double* x2, doub
     int it = 0;
     if (r0 == 0.0) {
         rel = 0;
+    } else if (!(std::sqrt(rr) > target) || h->opts.pcg_max_iter <= 0) {
+        rel = std::sqrt(rr) / r0;
     } else {
         precond(r, z);
         HIPCHK(hipMemcpyAsync(p, z, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
-        double rz = dot(h, r, z, m);
-        while (std::sqrt(rr) > target && it < h->opts.pcg_max_iter) {
-            ++it;
+        const double rz = dot(h, r, z, m);
+        double* sc = P->cg.p;
+        launch_cg_init(sc, rr, rz, st);
+        const int maxit = h->opts.pcg_max_iter;
+        int queued = 0;
+        for (; queued < maxit; ++queued) {
+            const int cur = kCgRZ + (queued & 1), next = kCgRZ + ((queued + 1) & 1);
             matvec(p, q);
-            const double pq = dot(h, p, q, m);
-            if (!(pq > 0) || !(rz > 0)) break;
-            const double alpha = rz / pq;
-            launch_axpy(m, alpha, p, x, st);
-            launch_axpy(m, -alpha, q, r, st);
-            rr = dot(h, r, r, m);
-            if (std::sqrt(rr) <= target) break;
+            launch_cg_step1(m, sc, cur, target, p, q, x, r, P->part.p, st);
             precond(r, z);
-            const double rz_new = dot(h, r, z, m);
-            const double beta = rz_new / rz;
-            rz = rz_new;
-            launch_xpby(m, z, beta, p, st);
+            launch_cg_step2(m, sc, cur, next, r, z, p, P->part.p, st);
+            double* hp = P->hpin + kCgSlots * (queued & 1);
+            HIPCHK(hipMemcpyAsync(hp, sc, sizeof(double) * kCgSlots, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipEventRecord(P->ev[queued & 1], st));
+            if (queued >= 1) {   // the state after the previous iteration
+                HIPCHK(hipEventSynchronize(P->ev[(queued - 1) & 1]));
+                if (P->hpin[kCgSlots * ((queued - 1) & 1) + kCgState] != 0.0) { ++queued; break; }
+            }
         }
-        rel = std::sqrt(rr) / r0;
+        HIPCHK(hipStreamSynchronize(st));
+        const double* fin = P->hpin + kCgSlots * ((queued - 1) & 1);
+        it = (int)fin[kCgIters];
+        rel = std::sqrt(fin[kCgRR]) / r0;
     }
     HIPCHK(hipMemcpyAsync(x2, x, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipGetLastError());
