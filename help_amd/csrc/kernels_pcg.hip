@@ -425,8 +425,9 @@ __global__ void k_pca_final(int32_t K, const int32_t* __restrict__ perm, const d
 // s[kCgIters] = iterations started.  Every kernel of an iteration returns at
 // once when the state is not 0, so iterations the host enqueued after the
 // stop change nothing.  The block partition of the reductions is that of
-// k_dot_partial (kDotPartials blocks, grid-stride), so every sum is the one the
-// separate dot kernels form.
+// k_dot_partial (kDotPartials blocks, grid-stride), and products and sums are
+// not contracted into FMAs, as in those kernels, so every value is the one
+// the separate dot and axpy kernels form.
 __device__ inline bool cg_stopped(const double* s) { return s[kCgState] != 0.0; }
 
 __device__ inline void block_sum_to(double acc, double* __restrict__ dst) {
@@ -443,6 +444,7 @@ __device__ inline void block_sum_to(double acc, double* __restrict__ dst) {
 __global__ __launch_bounds__(256) void k_cg_dot_partial(const double* __restrict__ a, const double* __restrict__ b,
                                                         int64_t n, double* __restrict__ part,
                                                         const double* __restrict__ s) {
+#pragma clang fp contract(off)
     if (cg_stopped(s)) return;
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
@@ -455,6 +457,7 @@ __global__ __launch_bounds__(256) void k_cg_dot_partial(const double* __restrict
 // converged when sqrt(r.r) <= target
 __global__ __launch_bounds__(256) void k_cg_final(const double* __restrict__ part, int np, double* __restrict__ s,
                                                   int slot, int kind, int rz_cur, double target) {
+#pragma clang fp contract(off)
     if (cg_stopped(s)) return;
     double acc = 0.0;
     for (int i = threadIdx.x; i < np; i += 256) acc = acc + part[i];
@@ -477,6 +480,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const double* __re
                                                    const double* __restrict__ p, const double* __restrict__ q,
                                                    double* __restrict__ x, double* __restrict__ r,
                                                    double* __restrict__ part) {
+#pragma clang fp contract(off)
     if (cg_stopped(s)) return;
     const double alpha = s[rz_cur] / s[kCgPQ];
     const double nalpha = -alpha;
@@ -493,6 +497,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int64_t n, const double* __re
 // beta = r.z(new) / r.z(old);  p = z + beta p
 __global__ void k_cg_xpby(int64_t n, const double* __restrict__ s, int rz_cur, int rz_next,
                           const double* __restrict__ z, double* __restrict__ p) {
+#pragma clang fp contract(off)
     if (cg_stopped(s)) return;
     const double beta = s[rz_next] / s[rz_cur];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -83,7 +83,9 @@ class RecordingBackend:
     last renew; the arrays are views valid during the call).  HELP's driver
     calls SolveLinearEquation first for the bottom system of an iteration
     (HSDSolver.cpp:770), so k == 0 picks that system.  Used to harvest the
-    (d, b1, b2) of a real interior-point trajectory."""
+    (d, b1, b2) of a real interior-point trajectory.  The backend's statuses
+    go back to the driver; the handle's last solve statistics are in
+    ``le.solve_stats`` during the call."""
 
     def __init__(self, inst, on_solve, **opts):
         from .backend import LinearEquation
@@ -102,21 +104,23 @@ class RecordingBackend:
             return 0
 
         def renew(ctx, dp):
-            self.le.renew(np.ctypeslib.as_array(dp, shape=(n,)).copy())
+            # the library's status goes back to the driver (no exception
+            # crosses the C callback)
+            st = L.lib().help_gpu_renew(self.le.h, dp, C.byref(self.le.renew_stats))
             self.renews += 1
             self._k = 0
-            return 0
+            return st
 
         def solve(ctx, dp, b1p, b2p, x1p, x2p):
+            st = L.lib().help_gpu_solve(self.le.h, dp, b1p, b2p, x1p, x2p, C.byref(self.le.solve_stats))
             d = np.ctypeslib.as_array(dp, shape=(n,))
             b1 = np.ctypeslib.as_array(b1p, shape=(n,))
             b2 = np.ctypeslib.as_array(b2p, shape=(m,))
-            x1, x2 = self.le.solve(d, b1, b2)
-            np.ctypeslib.as_array(x1p, shape=(n,))[:] = x1
-            np.ctypeslib.as_array(x2p, shape=(m,))[:] = x2
+            x1 = np.ctypeslib.as_array(x1p, shape=(n,))
+            x2 = np.ctypeslib.as_array(x2p, shape=(m,))
             on_solve(self.renews, self._k, d, b1, b2, x1, x2)
             self._k += 1
-            return 0
+            return st
 
         self._fns = (L.CONSTRUCT_FN(construct), L.DESTRUCT_FN(destruct), L.RENEW_FN(renew),
                      L.SOLVE_FN(solve))

@@ -8,6 +8,9 @@
     python -m help_amd.prof_workload solve --config 3 --mode pcg --pcg-max-iter 60
         the same in the matrix-free PCG mode, each solve capped at a fixed
         number of CG iterations;
+    python -m help_amd.prof_workload trace --config 3 --mode pcg --pcg-max-iter 3000
+        HELP's HSD driver over the backend, one line per solve (refinement
+        steps or CG iterations, residual, time);
     python -m help_amd.prof_workload copy --mib 256
         one device-to-device copy of known size: the reference that scales
         the FETCH_SIZE / WRITE_SIZE counters to bytes.
@@ -45,6 +48,34 @@ def _solve(args):
     be.destruct()
 
 
+def _trace(args):
+    """HELP's HSD driver (host) over the backend, one line per solve: the
+    renew it follows, refinement steps or CG iterations, relative residual
+    and solve time; then the run's result."""
+    from . import _lib as L
+    from . import instances
+    from .hsd import RecordingBackend, solve_lp
+    inst = instances.by_config(args.config)
+    mode = {"fp64": L.MODE_DIRECT_FP64, "fp32": L.MODE_DIRECT_FP32, "pcg": L.MODE_PCG}[args.mode]
+    opts = dict(pcg_k=args.pcg_k, pcg_max_iter=args.pcg_max_iter, pcg_fp32_precond=args.pcg_fp32)
+    if args.mode != "pcg":
+        opts = dict(ordering=args.ordering, refine_switch=args.refine_switch)
+    print(f"{args.mode} config {args.config} m={inst.m} n={inst.n} nnz={inst.nnz} {opts}", flush=True)
+    holder = {}
+
+    def on_solve(renews, k, d, b1, b2, x1, x2):
+        s = holder["rb"].le.solve_stats
+        print(f"renew {renews} solve {k}: refine {s.refine_steps} cg {s.pcg_iters} rel {s.rel_residual:.2e} "
+              f"{s.ms_solve:.1f} ms", flush=True)
+
+    rb = RecordingBackend(inst, on_solve, mode=mode, **opts)
+    holder["rb"] = rb
+    t = time.time()
+    res, _ = solve_lp(inst, rb.table, max_iter=args.max_iter)
+    print(f"result: status {L.HSD_STATUS[res.status]} stall {res.stall_reason} backend_error {res.backend_error} "
+          f"iterations {res.iterations} gap {res.rel_gap:.3e} {time.time() - t:.1f}s", flush=True)
+
+
 def _copy(args):
     import torch
     n = args.mib * (1 << 20) // 8
@@ -70,11 +101,22 @@ def main(argv=None):
     s.add_argument("--solves", type=int, default=3)
     s.add_argument("--decades", type=float, default=2.0)
     s.add_argument("--seed", type=int, default=0)
+    t = sub.add_parser("trace")
+    t.add_argument("--config", type=int, default=2)
+    t.add_argument("--mode", default="fp32", choices=["fp64", "fp32", "pcg"])
+    t.add_argument("--ordering", type=int, default=1, choices=[0, 1])
+    t.add_argument("--refine-switch", type=float, default=2.5)
+    t.add_argument("--pcg-k", type=int, default=500)
+    t.add_argument("--pcg-max-iter", type=int, default=5000)
+    t.add_argument("--pcg-fp32", type=int, default=1, choices=[0, 1])
+    t.add_argument("--max-iter", type=int, default=0)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=256)
     args = ap.parse_args(argv)
     if args.cmd == "solve":
         _solve(args)
+    elif args.cmd == "trace":
+        _trace(args)
     else:
         _copy(args)
     return 0

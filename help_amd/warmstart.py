@@ -217,6 +217,8 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--pcg-k", type=int, default=50)
     ap.add_argument("--eval-per-instance", type=int, default=3)
+    ap.add_argument("--e2e", type=int, default=1,
+                    help="1: HSD end to end in PCG and learned-PCG mode on the smallest held-out instance")
     ap.add_argument("--out", default=".")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
@@ -232,7 +234,8 @@ def main(argv=None):
         data += [(be, t) for t in tuples]
         log(f"train {key}: m={inst.m} n={inst.n} hsd status {res.status} iters {res.iterations} "
             f"tuples {len(tuples)}")
-    log(f"{len(data)} training tuples ({time.time() - t0:.1f}s)")
+    t_harvest = time.time() - t0
+    log(f"{len(data)} training tuples ({t_harvest:.1f}s)")
     net = Mpnn(bf16=True, seed=0)
     t1 = time.time()
     hist = train(net, data, epochs=args.epochs, lr=args.lr, log=log)
@@ -262,10 +265,12 @@ def main(argv=None):
     it1 = np.array([r["it_net"] for r in rows], float)
     tz = np.array([r["t_zero"] for r in rows], float)
     tn = np.array([r["t_net"] for r in rows], float)
-    # end to end on the first held-out instance: PCG vs learned PCG
-    key, inst = test_inst[0]
-    e2e = {}
+    # end to end on the smallest held-out instance: PCG vs learned PCG
+    key, inst = min(test_inst, key=lambda ki: ki[1].m)
+    e2e = {"instance": key}
     for name, mode in (("pcg", L.MODE_PCG), ("pcg_learned", L.MODE_PCG_LEARNED)):
+        if not args.e2e:
+            break
         gb = GpuBackend(mode=mode, pcg_k=args.pcg_k)
         if mode == L.MODE_PCG_LEARNED:
             gb.set_warmstart(net)
@@ -280,7 +285,7 @@ def main(argv=None):
         family="multicommodity flow on a 4-connected grid (instances.mcf)", side=args.side, commodities=args.k,
         n_train=args.train, n_test=args.test, training_tuples=len(data), epochs=args.epochs,
         params=net.nparams, precision="bf16 activations and weights, fp32 accumulation",
-        loss_history=hist, train_seconds=t_train, pcg_k=args.pcg_k, pcg_tol=1e-10,
+        loss_history=hist, harvest_seconds=t_harvest, train_seconds=t_train, pcg_k=args.pcg_k, pcg_tol=1e-10,
         pcg_iterations_zero_mean=float(it0.mean()), pcg_iterations_net_mean=float(it1.mean()),
         iteration_ratio_mean=float((it1 / np.maximum(it0, 1)).mean()),
         time_zero_mean=float(tz.mean()), time_net_mean=float(tn.mean()),
