@@ -954,12 +954,13 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
             phase("adat pattern");
             std::vector<int64_t> adjptr;
             std::vector<int32_t> adj;
-            adat_adjacency(m, h->P, adjptr, adj);
+            adat_adjacency(h->A, h->R, adjptr, adj);
             phase("adjacency");
             RelaxParams rp;
             std::vector<int32_t> perm;
             if (h->opts.ordering == HELP_ORDER_ND) perm = help::nd_order(m, adjptr, adj);
             else if (h->opts.ordering != HELP_ORDER_AMD) throw std::runtime_error("unknown ordering");
+            phase("ordering (ND)");
             h->S = analyze(m, adjptr, adj, perm, rp);
             phase("analyze");
             build_assembly_maps(h);

@@ -112,39 +112,63 @@ AdatPattern adat_pattern(const CscMatrix& A, const CsrMatrix& R, bool with_pairs
     return P;
 }
 
-void adat_adjacency(int32_t m, const AdatPattern& P, std::vector<int64_t>& adjptr,
+void adat_adjacency(const CscMatrix& A, const CsrMatrix& R, std::vector<int64_t>& adjptr,
                     std::vector<int32_t>& adj) {
-    // symmetric adjacency without the diagonal; rows are sorted at the end,
-    // so the order in which threads fill them does not matter
+    // row i of the symmetric pattern (without the diagonal) = every row that
+    // shares a column of A with row i: each row is discovered on its own
+    // (thread-local markers, no atomics), counted, then filled and sorted
+    const int32_t m = A.m;
     adjptr.assign(m + 1, 0);
-#pragma omp parallel for schedule(dynamic, 512)
-    for (int32_t i = 0; i < m; ++i)
-        for (int64_t s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
-            int32_t k = P.colind[s];
-            if (k == i) continue;
-#pragma omp atomic
-            adjptr[i + 1]++;
-#pragma omp atomic
-            adjptr[k + 1]++;
+#pragma omp parallel
+    {
+        std::vector<int32_t> flag(m, -1);
+#pragma omp for schedule(dynamic, 512)
+        for (int32_t i = 0; i < m; ++i) {
+            int64_t cnt = 0;
+            flag[i] = i;
+            for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
+                const int32_t j = R.colind[t];
+                for (int64_t q = A.colptr[j]; q < A.colptr[j + 1]; ++q) {
+                    const int32_t k = A.rowind[q];
+                    if (flag[k] != i) { flag[k] = i; ++cnt; }
+                }
+            }
+            adjptr[i + 1] = cnt;
         }
+    }
     for (int32_t i = 0; i < m; ++i) adjptr[i + 1] += adjptr[i];
     adj.resize(adjptr[m]);
-    std::vector<int64_t> fill(adjptr.begin(), adjptr.end() - 1);
-#pragma omp parallel for schedule(dynamic, 512)
-    for (int32_t i = 0; i < m; ++i)
-        for (int64_t s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
-            int32_t k = P.colind[s];
-            if (k == i) continue;
-            int64_t a, b;
-#pragma omp atomic capture
-            a = fill[i]++;
-#pragma omp atomic capture
-            b = fill[k]++;
-            adj[a] = k;
-            adj[b] = i;
+#pragma omp parallel
+    {
+        std::vector<int32_t> flag(m, -1);
+#pragma omp for schedule(dynamic, 512)
+        for (int32_t i = 0; i < m; ++i) {
+            int64_t o = adjptr[i];
+            flag[i] = i;
+            for (int64_t t = R.rowptr[i]; t < R.rowptr[i + 1]; ++t) {
+                const int32_t j = R.colind[t];
+                for (int64_t q = A.colptr[j]; q < A.colptr[j + 1]; ++q) {
+                    const int32_t k = A.rowind[q];
+                    if (flag[k] != i) { flag[k] = i; adj[o++] = k; }
+                }
+            }
+            std::sort(adj.begin() + adjptr[i], adj.begin() + adjptr[i + 1]);
         }
+    }
+}
+
+void permute_graph(int32_t m, const std::vector<int64_t>& adjptr, const std::vector<int32_t>& adj,
+                   const std::vector<int32_t>& perm, const std::vector<int32_t>& iperm,
+                   std::vector<int64_t>& pp, std::vector<int32_t>& pj) {
+    pp.assign(m + 1, 0);
+    for (int32_t k = 0; k < m; ++k) pp[k + 1] = pp[k] + (adjptr[perm[k] + 1] - adjptr[perm[k]]);
+    pj.resize(pp[m]);
 #pragma omp parallel for schedule(dynamic, 512)
-    for (int32_t i = 0; i < m; ++i) std::sort(adj.begin() + adjptr[i], adj.begin() + adjptr[i + 1]);
+    for (int32_t k = 0; k < m; ++k) {
+        const int32_t old = perm[k];
+        int64_t o = pp[k];
+        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) pj[o++] = iperm[adj[t]];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -598,14 +622,11 @@ std::vector<int32_t> nd_order(int32_t n, const std::vector<int64_t>& adjptr,
     return order;
 }
 
-std::vector<int32_t> etree(int32_t m, const std::vector<int64_t>& adjptr,
-                           const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
-                           const std::vector<int32_t>& iperm) {
+std::vector<int32_t> etree(int32_t m, const std::vector<int64_t>& pp, const std::vector<int32_t>& pj) {
     std::vector<int32_t> parent(m, -1), anc(m, -1);
     for (int32_t k = 0; k < m; ++k) {
-        int32_t old = perm[k];
-        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
-            int32_t i = iperm[adj[t]];
+        for (int64_t t = pp[k]; t < pp[k + 1]; ++t) {
+            int32_t i = pj[t];
             while (i != -1 && i < k) {
                 int32_t inext = anc[i];
                 anc[i] = k;
@@ -650,9 +671,8 @@ std::vector<int32_t> postorder(const std::vector<int32_t>& parent) {
 // neighbour of row i.  The least common ancestor of the previous leaf and j
 // is found in a disjoint-set forest where every finished node is linked to
 // its parent.
-std::vector<int32_t> colcounts(int32_t m, const std::vector<int64_t>& adjptr,
-                               const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
-                               const std::vector<int32_t>& iperm, const std::vector<int32_t>& parent) {
+std::vector<int32_t> colcounts(int32_t m, const std::vector<int64_t>& pp, const std::vector<int32_t>& pj,
+                               const std::vector<int32_t>& parent) {
     for (int32_t j = 0; j < m; ++j)
         if (parent[j] != -1 && parent[j] <= j) throw std::runtime_error("colcounts: tree not postordered");
     std::vector<int32_t> first(m, -1);
@@ -667,9 +687,8 @@ std::vector<int32_t> colcounts(int32_t m, const std::vector<int64_t>& adjptr,
         return r;
     };
     for (int32_t j = 0; j < m; ++j) {
-        const int32_t old = perm[j];
-        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
-            const int32_t i = iperm[adj[t]];
+        for (int64_t t = pp[j]; t < pp[j + 1]; ++t) {
+            const int32_t i = pj[t];
             if (i <= j || prevnbr[i] >= first[j]) { if (i > j) prevnbr[i] = j; continue; }
             delta[j] += 1;
             if (prevleaf[i] != -1) delta[find(prevleaf[i])] -= 1;
@@ -714,7 +733,12 @@ Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vec
     // etree, then postorder it: a postorder is a topological order of the
     // tree, so the permuted matrix has the same fill and the same tree with
     // its nodes relabelled
-    std::vector<int32_t> par = etree(m, adjptr, adj, perm, iperm);
+    // (the graph relabelled once per numbering: the sequential sweeps below
+    // then read it in order)
+    std::vector<int64_t> pp;
+    std::vector<int32_t> pj;
+    permute_graph(m, adjptr, adj, perm, iperm, pp, pj);
+    std::vector<int32_t> par = etree(m, pp, pj);
     std::vector<int32_t> post = postorder(par);
     std::vector<int32_t> perm2(m), ipost(m);
     for (int32_t k = 0; k < m; ++k) { perm2[k] = perm[post[k]]; ipost[post[k]] = k; }
@@ -723,7 +747,8 @@ Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vec
     std::vector<int32_t> par2(m);
     for (int32_t k = 0; k < m; ++k) par2[k] = par[post[k]] >= 0 ? ipost[par[post[k]]] : -1;
     par.swap(par2);
-    std::vector<int32_t> cc = colcounts(m, adjptr, adj, perm, iperm, par);
+    permute_graph(m, adjptr, adj, perm, iperm, pp, pj);
+    std::vector<int32_t> cc = colcounts(m, pp, pj, par);
     S.perm = perm; S.iperm = iperm; S.parent = par; S.colcount = cc;
     for (int32_t j = 0; j < m; ++j) {
         S.nnz_l_exact += cc[j];
@@ -833,13 +858,11 @@ Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vec
                     const int32_t f = S.sfirst[s], lc = S.sfirst[s + 1];
                     rows.clear();
                     for (int32_t j = f; j < lc; ++j) { mk[j] = s; rows.push_back(j); }
-                    for (int32_t j = f; j < lc; ++j) {
-                        const int32_t old = perm[j];
-                        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
-                            const int32_t i = iperm[adj[t]];
+                    for (int32_t j = f; j < lc; ++j)
+                        for (int64_t t = pp[j]; t < pp[j + 1]; ++t) {
+                            const int32_t i = pj[t];
                             if (i >= lc && mk[i] != s) { mk[i] = s; rows.push_back(i); }
                         }
-                    }
                     for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
                         const int32_t c = S.child[t];
                         const int32_t cn = S.sfirst[c + 1] - S.sfirst[c];

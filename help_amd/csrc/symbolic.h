@@ -77,18 +77,24 @@ struct Supernodal {
     double flops = 0;                        // sum colcount^2
 };
 
-// Full symmetric adjacency (no diagonal) of the upper pattern.
-void adat_adjacency(int32_t m, const AdatPattern& P, std::vector<int64_t>& adjptr,
+// Full symmetric adjacency (no diagonal, rows sorted) of the pattern of
+// A A^T, i.e. of ADA^T: rows i and k are adjacent when they share a column.
+void adat_adjacency(const CscMatrix& A, const CsrMatrix& R, std::vector<int64_t>& adjptr,
                     std::vector<int32_t>& adj);
 
-// Elimination tree of P M P^T (Liu's algorithm with path compression).
-std::vector<int32_t> etree(int32_t m, const std::vector<int64_t>& adjptr,
-                           const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
-                           const std::vector<int32_t>& iperm);
+// The adjacency relabelled by perm (new -> old): row k holds iperm[] of the
+// neighbours of perm[k], in their order there.
+void permute_graph(int32_t m, const std::vector<int64_t>& adjptr, const std::vector<int32_t>& adj,
+                   const std::vector<int32_t>& perm, const std::vector<int32_t>& iperm,
+                   std::vector<int64_t>& pp, std::vector<int32_t>& pj);
+
+// Elimination tree of P M P^T (Liu's algorithm with path compression), from
+// the permuted adjacency (pp, pj).
+std::vector<int32_t> etree(int32_t m, const std::vector<int64_t>& pp, const std::vector<int32_t>& pj);
 std::vector<int32_t> postorder(const std::vector<int32_t>& parent);
-std::vector<int32_t> colcounts(int32_t m, const std::vector<int64_t>& adjptr,
-                               const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,
-                               const std::vector<int32_t>& iperm, const std::vector<int32_t>& parent);
+// column counts from the permuted adjacency of a postordered tree
+std::vector<int32_t> colcounts(int32_t m, const std::vector<int64_t>& pp, const std::vector<int32_t>& pj,
+                               const std::vector<int32_t>& parent);
 // the same counts by walking every row subtree (O(nnz(L))); test reference
 std::vector<int32_t> colcounts_walk(int32_t m, const std::vector<int64_t>& adjptr,
                                const std::vector<int32_t>& adj, const std::vector<int32_t>& perm,

@@ -90,52 +90,29 @@ class Mpnn:
             pass
 
 
-_CONSTRUCT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
-_DESTRUCT = C.CFUNCTYPE(C.c_int, C.c_void_p)
-_RENEW = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double))
-_SOLVE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double))
-
-
-def harvest(inst, every=3, max_tuples=64, max_iter=0):
+def harvest(inst, every=3, max_tuples=64, max_iter=0, log=None):
     """(d, r, dy) tuples from every ``every``-th normal-equation solve of an
-    HSD run with the plain FP64 direct backend (HELP's driver on the host)."""
-    from .hsd import solve_lp
+    HSD run with the plain FP64 direct backend (HELP's driver on the host,
+    the backend called through the C-ABI), r = A D^-1 b1 - b2, dy = x2.
+    ``log`` gets a progress line every 25 interior-point iterations."""
+    from .hsd import RecordingBackend, solve_lp
     A = inst.to_scipy().tocsr()
-    be = LinearEquation(mode=L.MODE_DIRECT_FP64)
     tuples = []
     count = [0]
-    m, n = inst.m, inst.n
+    t0 = time.time()
 
-    def construct(ctx, lp):
-        be.construct(inst)
-        return 0
-
-    def destruct(ctx):
-        be.destruct()
-        return 0
-
-    def renew(ctx, d):
-        be.renew(np.ctypeslib.as_array(d, shape=(n,)))
-        return 0
-
-    def solve(ctx, d, b1, b2, x1, x2):
-        D = np.ctypeslib.as_array(d, shape=(n,)).copy()
-        B1 = np.ctypeslib.as_array(b1, shape=(n,)).copy()
-        B2 = np.ctypeslib.as_array(b2, shape=(m,)).copy()
-        X1, X2 = be.solve(D, B1, B2)
-        np.ctypeslib.as_array(x1, shape=(n,))[:] = X1
-        np.ctypeslib.as_array(x2, shape=(m,))[:] = X2
+    def on_solve(renews, k, d, b1, b2, x1, x2):
         if count[0] % every == 0 and len(tuples) < max_tuples:
-            tuples.append((D, A @ (B1 / D) - B2, X2.copy()))
+            D = d.copy()
+            tuples.append((D, A @ (b1 / D) - b2, x2.copy()))
         count[0] += 1
-        return 0
 
-    fns = (_CONSTRUCT(construct), _DESTRUCT(destruct), _RENEW(renew), _SOLVE(solve))
-    table = L.Backend()
-    table.ctx = None
-    table.construct, table.destruct, table.renew, table.solve = [C.cast(f, C.c_void_p) for f in fns]
-    res, _ = solve_lp(inst, table, max_iter=max_iter)
+    def progress(it, *_):
+        if log is not None and it % 25 == 0:
+            log(f"  harvest m={inst.m}: iteration {it}, {count[0]} solves, {time.time() - t0:.1f}s")
+
+    rb = RecordingBackend(inst, on_solve, mode=L.MODE_DIRECT_FP64)
+    res, _ = solve_lp(inst, rb.table, max_iter=max_iter, log=progress)
     return tuples, res
 
 
@@ -217,8 +194,12 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--pcg-k", type=int, default=50)
     ap.add_argument("--eval-per-instance", type=int, default=3)
+    ap.add_argument("--harvest-iter", type=int, default=0,
+                    help="interior-point iteration cap of each harvesting run (0: the driver's own)")
     ap.add_argument("--e2e", type=int, default=1,
                     help="1: HSD end to end in PCG and learned-PCG mode on the smallest held-out instance")
+    ap.add_argument("--e2e-iter", type=int, default=0,
+                    help="interior-point iteration cap of the end-to-end runs (0: the driver's own)")
     ap.add_argument("--out", default=".")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
@@ -228,12 +209,14 @@ def main(argv=None):
     test_inst = _instances(args.test, 2000, args.side, args.k)
     data, handles = [], []
     for key, inst in train_inst:
-        tuples, res = harvest(inst, every=args.every, max_tuples=args.tuples)
+        t_i = time.time()
+        tuples, res = harvest(inst, every=args.every, max_tuples=args.tuples, max_iter=args.harvest_iter,
+                              log=log)
         be = graph_handle(inst)
         handles.append(be)
         data += [(be, t) for t in tuples]
         log(f"train {key}: m={inst.m} n={inst.n} hsd status {res.status} iters {res.iterations} "
-            f"tuples {len(tuples)}")
+            f"tuples {len(tuples)} ({time.time() - t_i:.1f}s)")
     t_harvest = time.time() - t0
     log(f"{len(data)} training tuples ({t_harvest:.1f}s)")
     net = Mpnn(bf16=True, seed=0)
@@ -245,7 +228,8 @@ def main(argv=None):
         be.destruct()
     rows = []
     for key, inst in test_inst:
-        tuples, res = harvest(inst, every=args.every, max_tuples=args.tuples)
+        tuples, res = harvest(inst, every=args.every, max_tuples=args.tuples, max_iter=args.harvest_iter,
+                              log=log)
         g = graph_handle(inst)
         losses = [net.loss_grad(g, *t) for t in tuples]
         g.destruct()
@@ -275,7 +259,8 @@ def main(argv=None):
         if mode == L.MODE_PCG_LEARNED:
             gb.set_warmstart(net)
         t = time.time()
-        res, _ = solve_lp(inst, gb.table, device=True)
+        res, _ = solve_lp(inst, gb.table, device=True, max_iter=args.e2e_iter,
+                          log=lambda it, *_: log(f"  {name} iteration {it} ({time.time() - t:.1f}s)"))
         e2e[name] = dict(status=L.HSD_STATUS.get(res.status, str(res.status)), iterations=res.iterations,
                          rel_gap=res.rel_gap, primal_obj=res.primal_obj, seconds=time.time() - t,
                          pcg_iterations=int(gb.pcg_iterations()))

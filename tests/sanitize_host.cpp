@@ -5,6 +5,7 @@
 // HSD driver restatement (hsd.cpp) over the CPU oracle backend, on small
 // instances of every generator, bounded and free columns included.  Exits
 // non-zero on a wrong result; the sanitizers abort on a memory or UB error.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -38,7 +39,20 @@ static void symbolic_checks(const help_lp& lp, int ordering) {
     AdatPattern P = adat_pattern(A, R, true);
     std::vector<int64_t> ap;
     std::vector<int32_t> aj;
-    adat_adjacency(A.m, P, ap, aj);
+    adat_adjacency(A, R, ap, aj);
+    // the adjacency is the off-diagonal upper pattern mirrored, rows sorted
+    int64_t offdiag = 0;
+    for (int32_t i = 0; i < A.m; ++i) {
+        for (int64_t t = ap[i] + 1; t < ap[i + 1]; ++t) CHECK(aj[t - 1] < aj[t]);
+        for (int64_t s = P.rowptr[i]; s < P.rowptr[i + 1]; ++s) {
+            const int32_t k = P.colind[s];
+            if (k == i) continue;
+            ++offdiag;
+            CHECK(std::binary_search(aj.begin() + ap[i], aj.begin() + ap[i + 1], k));
+            CHECK(std::binary_search(aj.begin() + ap[k], aj.begin() + ap[k + 1], i));
+        }
+    }
+    CHECK(ap[A.m] == 2 * offdiag);
     std::vector<int32_t> perm;
     if (ordering == 1) perm = nd_order(A.m, ap, aj, 64);
     RelaxParams rp;
