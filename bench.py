@@ -15,6 +15,13 @@ Three paths run on the same instance, each as W untimed and K timed steps:
 * mixed_fp32_nd   FP32 factor (MFMA, FP32 accumulation) + FP64 conjugate
                   gradients preconditioned by it, nested dissection.
 
+Other workloads (``--config``): 0 a random LP, 1 the 150x150 K=8
+multicommodity-flow grid (plain FP64 vs mixed, AMD), 3 the Zipf LP with
+m=10^6 in PCG mode (pcg_fp64: FP64 partial-Cholesky apply; pcg_fp32: FP32
+apply, the value), 4 a held-out instance of the warm-start family in PCG
+mode (pcg_zero: CG from zero; pcg_learned: CG from the committed network's
+guess, help_amd/data/warmstart_net_mcf.npz, the value).
+
 A "step" is one complete HSD solve (construct, iterate to the stop test,
 destruct): the span HELP times as HSD_Main (Main.cpp:95-97).  `value` is the
 mixed path's mean seconds per step; the plain times and the speedups sit
@@ -37,18 +44,33 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"hbm": 8000.0, "mfma_fp64": 78.6, "mfma_fp32": 157.3}   # GB/s, TFLOP/s (MI355X spec)
 
-# name -> (mode, ordering)
+# name -> backend mode and options
 PATHS = {
-    "plain_fp64_amd": ("fp64", 0),
-    "plain_fp64_nd": ("fp64", 1),
-    "mixed_fp32_amd": ("fp32", 0),
-    "mixed_fp32_nd": ("fp32", 1),
+    "plain_fp64_amd": dict(mode="fp64", ordering=0),
+    "plain_fp64_nd": dict(mode="fp64", ordering=1),
+    "mixed_fp32_amd": dict(mode="fp32", ordering=0),
+    "mixed_fp32_nd": dict(mode="fp32", ordering=1),
+    "pcg_fp64": dict(mode="pcg", pcg_fp32_precond=0),
+    "pcg_fp32": dict(mode="pcg", pcg_fp32_precond=1),
+    "pcg_zero": dict(mode="pcg", pcg_fp32_precond=1),
+    "pcg_learned": dict(mode="pcg_learned", pcg_fp32_precond=1),
 }
 DEFAULT_PATHS = {0: "plain_fp64_amd,mixed_fp32_amd", 1: "plain_fp64_amd,mixed_fp32_amd",
-                 2: "plain_fp64_amd,plain_fp64_nd,mixed_fp32_nd"}
+                 2: "plain_fp64_amd,plain_fp64_nd,mixed_fp32_nd", 3: "pcg_fp64,pcg_fp32",
+                 4: "pcg_zero,pcg_learned"}
 NAMES = {0: "random LP m=2000 n=6000 (A=[B|I], 4 nnz/col)",
          1: "multicommodity flow 150x150 grid, K=8 (m=269392, n=804600, nnz=2234968)",
-         2: "staircase T=200, 1000x3000 blocks (m=200000, n=800000, nnz=3797000)"}
+         2: "staircase T=200, 1000x3000 blocks (m=200000, n=800000, nnz=3797000)",
+         3: "Zipf LP m=1000000, n=4000000 (6 nnz/col, s=1.1), PCG with K=500",
+         4: "multicommodity flow 76x76 grid, K=2 (held-out instance of the warm-start family, seed 2002), "
+            "PCG with K=50"}
+WORKLOAD_KEYS = {0: "random2000", 1: "mcf150x8", 2: "staircase200", 3: "zipf1e6", 4: "mcf76x2"}
+NET = os.path.join(ROOT, "help_amd", "data", "warmstart_net_mcf.npz")
+
+
+def mode_const(L, mode_s):
+    return {"fp64": L.MODE_DIRECT_FP64, "fp32": L.MODE_DIRECT_FP32, "pcg": L.MODE_PCG,
+            "pcg_learned": L.MODE_PCG_LEARNED}[mode_s]
 
 
 def log(msg):
@@ -153,21 +175,40 @@ def cpu_baseline(inst, gpu_iterations, sample_iters=1):
 
 
 def path_opts(name, extra, mixed_opts):
-    """backend options of a path: the common ones, plus the refinement
-    settings for the mixed-precision paths"""
-    return {**extra, **(mixed_opts if PATHS[name][0] == "fp32" else {})}
+    """backend options of a path: its own, the common ones, plus the
+    refinement settings for the mixed-precision paths"""
+    own = {k: v for k, v in PATHS[name].items() if k != "mode"}
+    return {**own, **extra, **(mixed_opts if PATHS[name]["mode"] == "fp32" else {})}
 
 
-def run_path(inst, name, steps, warmup, driver, extra, L, GpuBackend, solve_lp):
-    mode_s, ordering = PATHS[name]
-    mode = L.MODE_DIRECT_FP32 if mode_s == "fp32" else L.MODE_DIRECT_FP64
+_net = {}
+
+
+def warmstart_net():
+    """The committed warm-start network (help_amd/data), loaded once."""
+    if "net" not in _net:
+        from help_amd.warmstart import Mpnn
+        _net["net"] = Mpnn.load(NET)
+    return _net["net"]
+
+
+def make_backend(name, opts, L, GpuBackend, **more):
+    mode_s = PATHS[name]["mode"]
+    gb = GpuBackend(mode=mode_const(L, mode_s), **opts, **more)
+    if mode_s == "pcg_learned":
+        gb.set_warmstart(warmstart_net())
+    return gb
+
+
+def run_path(inst, name, steps, warmup, driver, opts, L, GpuBackend, solve_lp):
+    mode_s = PATHS[name]["mode"]
     for w in range(warmup):
-        gb = GpuBackend(mode=mode, ordering=ordering, **extra)
+        gb = make_backend(name, opts, L, GpuBackend)
         res, _ = solve_lp(inst, gb.table, device=driver == "gpu")
         gb.close()
         log(f"{name} warmup {w}: status {res.status} iters {res.iterations} {res.seconds:.2f}s")
     L.lib().help_device_synchronize()
-    gb = GpuBackend(mode=mode, ordering=ordering, profile=0, **extra)
+    gb = make_backend(name, opts, L, GpuBackend, profile=0)
     times, results = [], []
     for k in range(steps):
         ts = time.time()
@@ -179,18 +220,26 @@ def run_path(inst, name, steps, warmup, driver, extra, L, GpuBackend, solve_lp):
             f"{times[-1]:.3f}s (construct {res.seconds_construct:.2f}s, renew {res.seconds_renew:.2f}s, "
             f"solve {res.seconds_solve:.2f}s)")
     n_sol, n_ref, max_rel = gb.solve_stats()
+    pcg_it = gb.pcg_iterations() if mode_s.startswith("pcg") else None
     gb.close()
     res = results[-1]
     host_driver = max(0.0, res.seconds - res.seconds_renew - res.seconds_solve - res.seconds_construct)
-    return dict(
-        seconds=sum(times) / len(times), mode=mode_s, ordering="nd" if ordering else "amd",
+    out = dict(
+        seconds=sum(times) / len(times), mode=mode_s,
         status=L.HSD_STATUS.get(res.status, str(res.status)), iterations=res.iterations,
         rel_gap=res.rel_gap, primal_obj=res.primal_obj, primal_infeas=res.primal_infeas,
         dual_infeas=res.dual_infeas, construct_s=round(res.seconds_construct, 3),
         renew_s=round(res.seconds_renew, 3), solve_s=round(res.seconds_solve, 3),
         driver_s=round(host_driver, 3), n_renew=res.n_renew, n_solve=res.n_solve,
-        refine_iterations=int(n_ref // max(steps, 1)), max_rel_residual=max_rel,
+        max_rel_residual=max_rel,
         ms_per_iteration=round(1e3 * (sum(times) / len(times)) / max(res.iterations, 1), 3))
+    if "ordering" in PATHS[name]:
+        out["ordering"] = "nd" if PATHS[name]["ordering"] else "amd"
+        out["refine_iterations"] = int(n_ref // max(steps, 1))
+    else:
+        out["stall_reason"] = res.stall_reason
+        out["pcg_iterations"] = int(pcg_it // max(steps, 1))
+    return out
 
 
 def dy_errors(inst, name, capture, extra):
@@ -202,15 +251,14 @@ def dy_errors(inst, name, capture, extra):
     from help_amd import LinearEquation
     from help_amd import _lib as L
     from help_amd.hsd import RecordingBackend, solve_lp
-    mode_s, ordering = PATHS[name]
-    mode = L.MODE_DIRECT_FP32 if mode_s == "fp32" else L.MODE_DIRECT_FP64
+    mode = mode_const(L, PATHS[name]["mode"])
     recs = {}
 
     def keep(renews, k, d, b1, b2, x1, x2):
         if k == 0 and renews in capture:
             recs[renews] = (d.copy(), b1.copy(), b2.copy(), x2.copy())
 
-    rb = RecordingBackend(inst, keep, mode=mode, ordering=ordering, **extra)
+    rb = RecordingBackend(inst, keep, mode=mode, **extra)
     res, _ = solve_lp(inst, rb.table)
     A = inst.to_scipy().tocsr()
     At = A.T.tocsr()
@@ -236,7 +284,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2])
+    ap.add_argument("--config", type=int, default=2, choices=[0, 1, 2, 3, 4])
     ap.add_argument("--paths", default=None,
                     help="comma list of " + ",".join(PATHS) + "; the last one is `value` "
                          "(default per config: " + "; ".join(f"{k}: {v}" for k, v in DEFAULT_PATHS.items()) + ")")
@@ -286,15 +334,15 @@ def main():
                               GpuBackend, solve_lp)
     vp = paths[-1]
     vres = results[vp]
-    mode_s, ordering = PATHS[vp]
+    mode_s = PATHS[vp]["mode"]
+    direct = mode_s in ("fp64", "fp32")
 
     # kernel-class timings of the value path: one more, untimed solve with HIP
     # events around every launch on the library's stream (the events cost
     # time, so the timed steps run without them)
     rows, prof_s = [], None
     if not args.no_profile:
-        gp = GpuBackend(mode=L.MODE_DIRECT_FP32 if mode_s == "fp32" else L.MODE_DIRECT_FP64,
-                        ordering=ordering, profile=1, **path_opts(vp, extra, mixed))
+        gp = make_backend(vp, path_opts(vp, extra, mixed), L, GpuBackend, profile=1)
         ts = time.time()
         rp, _ = solve_lp(inst, gp.table, device=args.driver == "gpu")
         L.lib().help_device_synchronize()
@@ -304,22 +352,22 @@ def main():
         log(f"profiled solve ({vp}): status {rp.status} iters {rp.iterations} {prof_s:.3f}s")
     # HBM bytes per dispatch from the rocprofv3 counter runs of the same
     # workload (profiles/traffic.json; null when that workload was not counted)
-    wkey = {0: "random2000", 1: "mcf150x8", 2: "staircase200"}[args.config] + f"_{mode_s}" + \
-        ("_nd" if ordering else "")
+    wkey = WORKLOAD_KEYS[args.config] + f"_{mode_s}" + ("_nd" if PATHS[vp].get("ordering") else "")
     roof = roofline(rows, mode_s, os.path.join(ROOT, "profiles", "traffic.json"), wkey) if rows else None
     kernels = {r["kernel"]: dict(ms_total=round(r["ms"], 3), launches=r["launches"],
                                  rate=round(r["work"] / (r["ms"] * 1e-3) / (1e9 if r["bound"] == "hbm" else 1e12), 3),
                                  unit="GB/s" if r["bound"] == "hbm" else "TFLOP/s") for r in rows}
 
     dy = None
-    if not args.no_dy_error:
+    if not args.no_dy_error and direct:
         it = max(vres["iterations"], 1)
         cap = sorted({1, max(1, it // 2), it})
         dy_list, st, its = dy_errors(inst, vp, cap, path_opts(vp, extra, mixed))
         dy = dict(path=vp, driver="host (HELP order)", status=st, iterations=its, solves=dy_list)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.config != 3:
+        # (the Zipf LP's heavy rows make A D^-1 A^T dense: no direct CPU factor)
         log("timing the CPU baseline sample (MKL PARDISO, all host cores)...")
         cpu = cpu_baseline(inst, vres["iterations"])
 
@@ -329,14 +377,19 @@ def main():
         speed[f"speedup_vs_{p}"] = round(results[p]["seconds"] / value, 3)
     for p in paths:
         results[p]["seconds"] = round(results[p]["seconds"], 4)
-    metric = ("time to 1e-8 rel gap (HSD_Main span): "
-              + ("mixed precision (FP32 factor + FP64 refinement)" if mode_s == "fp32" else "plain FP64")
-              + " vs plain FP64, on 1 MI355X")
+    metric = "time to 1e-8 rel gap (HSD_Main span): " + {
+        "fp32": "mixed precision (FP32 factor + FP64 refinement) vs plain FP64",
+        "fp64": "plain FP64",
+        "pcg": "PCG with the FP32 partial-Cholesky apply vs the FP64 apply" if args.config == 3
+               else "PCG from zero (the learned warm start's baseline)",
+        "pcg_learned": "PCG from the learned warm start vs from zero"}[mode_s] + ", on 1 MI355X"
+    dtype = {"fp64": "f64", "fp32": "f32 factor + f64 refinement", "pcg": "f32 preconditioner + f64 CG",
+             "pcg_learned": "bf16 network + f32 preconditioner + f64 CG"}[mode_s]
     line = {
         "metric": metric,
         "value": round(value, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * value, 2), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if mode_s == "fp64" else "f32 factor + f64 refinement",
+        "vs_baseline": None, "dtype": dtype,
         "data": "synthetic (deterministic generator, seed 0)",
         "config": {"workload": NAMES[args.config], "value_path": vp, "driver": args.driver,
                    "solver": "HELP HSD driver", **speed,

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -170,7 +171,7 @@ void build_schedules(help_gpu* h) {
     int32_t nphase = 0;
     for (int32_t s = 0; s < ns; ++s) {
         const int32_t nc = ncol(s), c = S.snrow[s];
-        smallf[s] = h->opts.front_path == 0 && is_small_front(c, nc);
+        smallf[s] = h->opts.front_path != 1 && is_small_front(c, nc);
         npan[s] = smallf[s] ? 1 : (nc + kTile - 1) / kTile;
         int32_t r = 0;
         for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
@@ -180,6 +181,67 @@ void build_schedules(help_gpu* h) {
         start[s] = r;
         done[s] = r + npan[s] - 1;
         nphase = std::max(nphase, done[s] + 1);
+    }
+    // Update-matrix placement.  Front s's update matrix is live from its
+    // first phase (its extend-add, or its small-front launch, zeroes it)
+    // through its parent's first phase (the parent's extend-add reads it),
+    // both inclusive; fronts whose lifetimes do not overlap share memory.
+    // First fit over an address-ordered free list, phase by phase: the blocks
+    // that died before the phase are freed, then its new ones are placed.
+    {
+        h->uplan.assign(ns + 1, 0);
+        std::vector<std::vector<int32_t>> born(nphase + 1), dies(nphase + 1);
+        for (int32_t s = 0; s < ns; ++s) {
+            const int64_t cu = S.snrow[s] - ncol(s);
+            if (cu <= 0) continue;
+            born[start[s]].push_back(s);
+            const int32_t last = S.sparent[s] >= 0 ? start[S.sparent[s]] : done[s];
+            dies[last + 1].push_back(s);
+        }
+        auto usize = [&](int32_t s) {
+            const int64_t cu = S.snrow[s] - ncol(s);
+            return (cu * cu + 63) / 64 * 64;   // 64-element aligned blocks
+        };
+        std::map<int64_t, int64_t> freel;   // offset -> length
+        int64_t top = 0, high = 0;
+        for (int32_t p = 0; p <= nphase; ++p) {
+            for (int32_t s : dies[p]) {
+                int64_t o = h->uplan[s], len = usize(s);
+                auto nx = freel.lower_bound(o);
+                if (nx != freel.end() && nx->first == o + len) { len += nx->second; nx = freel.erase(nx); }
+                if (nx != freel.begin()) {
+                    auto pv = std::prev(nx);
+                    if (pv->first + pv->second == o) { o = pv->first; len += pv->second; freel.erase(pv); }
+                }
+                if (o + len == top) top = o;
+                else freel[o] = len;
+            }
+            if (p == nphase) break;
+            for (int32_t s : born[p]) {
+                const int64_t len = usize(s);
+                auto it = freel.begin();
+                while (it != freel.end() && it->second < len) ++it;
+                if (it != freel.end()) {
+                    h->uplan[s] = it->first;
+                    if (it->second > len) freel[it->first + len] = it->second - len;
+                    freel.erase(it);
+                } else {
+                    // the free block just below the top, if any, grows into it
+                    int64_t o = top;
+                    if (!freel.empty()) {
+                        auto last = std::prev(freel.end());
+                        if (last->first + last->second == top) { o = last->first; freel.erase(last); }
+                    }
+                    h->uplan[s] = o;
+                    top = o + len;
+                    high = std::max(high, top);
+                }
+            }
+        }
+        h->uplan[ns] = high;
+        if (h->opts.verbose)
+            std::fprintf(stderr, "[help] update matrices: %lld elements placed (%lld without reuse)\n",
+                         (long long)high, (long long)S.uoff[ns]);
     }
     // fronts by phase: small ones, large ones starting, large ones active
     // (some panel in the phase), large ones finishing
@@ -204,7 +266,8 @@ void build_schedules(help_gpu* h) {
             };
             int32_t nwave = 0;
             for (int32_t s : at_small[p]) nwave += wave_ok(s) ? 1 : 0;
-            const bool use_wave = nwave >= 2048;
+            const bool force64 = h->opts.front_path == 2;
+            const bool use_wave = nwave >= 2048 || (force64 && nwave > 0);
             for (int pass = 0; pass < 2; ++pass) {
                 int32_t off = (int32_t)h->list_small.size();
                 double w = 0;
@@ -222,7 +285,7 @@ void build_schedules(help_gpu* h) {
                 if ((int32_t)h->list_small.size() > off) {
                     Op o{OP_SMALL, off, (int32_t)h->list_small.size() - off, w, lds};
                     if (pass == 0) {   // 32 lanes per front when every front has at most 32 rows
-                        o.wave = 32;
+                        o.wave = force64 ? 64 : 32;
                         for (int32_t t = off; t < (int32_t)h->list_small.size(); ++t)
                             if (S.snrow[h->list_small[t]] > 32) o.wave = 64;
                     }
@@ -231,14 +294,17 @@ void build_schedules(help_gpu* h) {
             }
         }
         if (at_active[p].empty()) continue;
-        // extend-add by kEaCols-column chunks of each starting front
+        // extend-add by kEaCols-column chunks of each starting front; the
+        // same tasks first zero the front's update matrix (a childless front
+        // gets the tasks of its update-matrix columns for that alone)
         int32_t off = (int32_t)h->ea.size();
         double w = 0;
         for (int32_t s : at_start[p]) {
-            const int32_t c = S.snrow[s];
-            if (S.childptr[s + 1] == S.childptr[s]) continue;
-            w += ea_bytes(s);
-            for (int32_t c0 = 0; c0 < c; c0 += kEaCols) {
+            const int32_t c = S.snrow[s], nc = ncol(s);
+            const bool leaf = S.childptr[s + 1] == S.childptr[s];
+            if (leaf && c == nc) continue;
+            w += ea_bytes(s) + 4.0 * (c - nc) * (c - nc + 1);
+            for (int32_t c0 = leaf ? nc / kEaCols * kEaCols : 0; c0 < c; c0 += kEaCols) {
                 const int32_t c1 = std::min(c, c0 + kEaCols);
                 h->ea.push_back({s, c0, c1, (int32_t)h->earange.size()});
                 // per child: the update columns that land in [c0, c1) (the
@@ -269,18 +335,30 @@ void build_schedules(help_gpu* h) {
                 h->panel.push_back({s, b, rb});
                 wt += (double)nb * nb * std::min(kTile, c - r0);
             }
-            for (int32_t c0 = b + nb; c0 < nc; c0 += kTile)
+            // Trailing updates by pairs of panels: the first panel of a pair
+            // updates only the second one's columns; the second updates every
+            // later column with both panels at once (an inner dimension of
+            // up to 128), so the trailing tiles are read and written once
+            // per two panels.  A front's last panel closes the pair it is in.
+            const int32_t pb = p - start[s], P = npan[s];
+            const bool pair_first = (pb % 2 == 0) && pb + 1 < P;
+            const int32_t kp = (pb % 2 == 1) ? b - kTile : b;   // first panel not yet applied
+            for (int32_t c0 = b + nb; c0 < nc; c0 += kTile) {
+                if (pair_first && c0 > b + nb) break;
+                const int32_t k0 = pair_first ? b : kp;
                 for (int32_t r0 = c0; r0 < c; r0 += kTile) {
-                    h->gemm.push_back({s, r0, c0, b, b + nb, nc});
-                    wg += 2.0 * nb * tile_lower_entries(r0, c0, c, nc);
+                    h->gemm.push_back({s, r0, c0, k0, b + nb, nc});
+                    wg += 2.0 * (b + nb - k0) * tile_lower_entries(r0, c0, c, nc);
                 }
-            if (folded(s, npan[s])) {
-                const int32_t pb = p - start[s], P = npan[s];
-                int32_t mode = kGemmNext, k0 = b, k1 = b + nb;
-                if (P == 1) { mode = kGemmDirect; k0 = 0; k1 = nc; }
-                else if (pb == 0) mode = kGemmFirst;
-                else if (pb == P - 1) mode = kGemmLast;
-                if (P > 1) h->use_uacc = true;
+            }
+            if (folded(s, npan[s]) && !pair_first) {
+                // update points: the second panel of each pair and the last
+                const int32_t q = pb / 2, Q = (P + 1) / 2;
+                int32_t mode = kGemmNext, k0 = kp, k1 = b + nb;
+                if (Q == 1) { mode = kGemmDirect; k0 = 0; k1 = nc; }
+                else if (q == 0) mode = kGemmFirst;
+                else if (q == Q - 1) mode = kGemmLast;
+                if (Q > 1) h->use_uacc = true;
                 for (int32_t c0 = nc; c0 < c; c0 += kTile)
                     for (int32_t r0 = c0; r0 < c; r0 += kTile) {
                         GemmTask g{s, r0, c0, k0, k1, c};
@@ -302,7 +380,7 @@ void build_schedules(help_gpu* h) {
         h->fops.push_back(op);
         if ((int32_t)h->trsm.size() > to) {
             Op ot{OP_TRSM, to, (int32_t)h->trsm.size() - to, wt};
-            ot.lds = fuse ? 1 : 0;
+            ot.fused = fuse;
             h->fops.push_back(ot);
         }
         if ((int32_t)h->gemm.size() > go) h->fops.push_back({OP_GEMM, go, (int32_t)h->gemm.size() - go, wg});
@@ -432,7 +510,7 @@ void build_schedules(help_gpu* h) {
         Op o{OP_FWD, off, (int32_t)h->list_fwd.size() - off, w, 0, ntiny};
         o.xoff = xo;
         o.xcount = (int32_t)h->fxs.size() - xo;
-        o.lds = tiny_lanes(h->list_fwd, off, o.count, ntiny, o.xcount);
+        o.wave = tiny_lanes(h->list_fwd, off, o.count, ntiny, o.xcount);
         h->sops_fwd.push_back(o);
         if ((int32_t)h->fgemv.size() > to) h->sops_fwd.push_back({OP_FWD_GEMV, to, (int32_t)h->fgemv.size() - to, wg});
     }
@@ -458,7 +536,7 @@ void build_schedules(help_gpu* h) {
         Op o{OP_BWD, off, (int32_t)h->list_bwd.size() - off, w, 0, ntiny};
         o.xoff = xo;
         o.xcount = (int32_t)h->bxs.size() - xo;
-        o.lds = tiny_lanes(h->list_bwd, off, o.count, ntiny, o.xcount);
+        o.wave = tiny_lanes(h->list_bwd, off, o.count, ntiny, o.xcount);
         h->sops_bwd.push_back(o);
     }
     // SpMV-class kernels: 12 bytes per nonzero (value + index) plus vectors
@@ -532,7 +610,7 @@ void setup_device_factor(help_gpu* h) {
     h->d_sfirst.upload(S.sfirst, st);
     h->d_snrow.upload(S.snrow, st);
     h->d_loff.upload(S.loff, st);
-    h->d_uoff.upload(S.uoff, st);
+    h->d_uoff.upload(h->uplan, st);
     h->d_rowoff.upload(S.rowoff, st);
     h->d_rowidx.upload(S.rowidx, st);
     h->d_reloff.upload(S.reloff, st);
@@ -561,15 +639,17 @@ void setup_device_factor(help_gpu* h) {
     h->w_len = std::max<int64_t>(1, S.reloff[S.nsuper]);
     h->d_y.alloc(2 * (size_t)h->m);
     h->d_W.alloc(2 * (size_t)h->w_len);
-    int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
+    int64_t nl = S.loff[S.nsuper];
     int64_t nd = h->dinvoff[S.nsuper];
+    // update matrices: shared by both precisions (at most one factorization
+    // runs at a time and nothing reads them after it)
+    h->d_Ubuf.alloc(std::max<int64_t>(1, h->uplan[S.nsuper]));
+    if (h->use_uacc) h->d_Uaccbuf.alloc(std::max<int64_t>(1, h->uplan[S.nsuper]));
     if (h->use_fp32) {
         h->d_slot_row.upload(h->slot_row, st);
         h->d_slot_col.upload(h->P.colind, st);
         h->d_scale.alloc(h->m);
         h->d_L32.alloc(std::max<int64_t>(1, nl));
-        h->d_U32.alloc(std::max<int64_t>(1, nu));
-        if (h->use_uacc) h->d_Uacc32.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv32.alloc(std::max<int64_t>(1, nd));
         if (h->use_xinv) {
             h->d_X32.alloc(std::max<int64_t>(1, h->xoff[S.nsuper]));
@@ -580,8 +660,6 @@ void setup_device_factor(help_gpu* h) {
         ensure_fp64_factor(h);
     } else {
         h->d_L64.alloc(std::max<int64_t>(1, nl));
-        h->d_U64.alloc(std::max<int64_t>(1, nu));
-        if (h->use_uacc) h->d_Uacc64.alloc(std::max<int64_t>(1, nu));
         h->d_Dinv64.alloc(std::max<int64_t>(1, nd));
         if (!h->pcopy.empty()) h->d_Lstage64.alloc(std::max<int64_t>(1, nd));
         if (h->use_xinv) {
@@ -614,8 +692,7 @@ T* xinv_store(help_gpu* h) {
 }
 template <typename T>
 T* uacc_store(help_gpu* h) {
-    if (std::is_same<T, float>::value) return (T*)h->d_Uacc32.p;
-    return (T*)h->d_Uacc64.p;
+    return (T*)h->d_Uaccbuf.p;
 }
 
 template <typename T>
@@ -641,7 +718,7 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
         }
         forked = side;
         const bool fused = std::is_same<T, double>::value &&
-                           ((o.kind == OP_POTRF && o.xcount > 0) || (o.kind == OP_TRSM && o.lds == 1));
+                           ((o.kind == OP_POTRF && o.xcount > 0) || (o.kind == OP_TRSM && o.fused));
         if (fused && o.kind == OP_TRSM) continue;   // done by the fused panel launch
         ProfScope ps(h, o.kind, fused ? o.work2 : o.work);
         switch (o.kind) {
@@ -698,7 +775,7 @@ void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x, int nrhs = 
         ProfScope ps(h, o.kind, o.work * nrhs);
         if (o.kind == OP_FWD)
             launch_fwd_level<T>(h->d_list_fwd.p + o.off, o.count, o.ntiny, h->d_fxs.p + o.xoff, o.xcount, h->sym,
-                                h->d_big.p, L, Dinv, X, x, y, h->d_W.p, nrhs, rs, st, o.lds);
+                                h->d_big.p, L, Dinv, X, x, y, h->d_W.p, nrhs, rs, st, o.wave);
         else
             launch_fwd_gemv<T>(h->d_fgemv.p + o.off, o.count, h->sym, L, y, h->d_W.p, nrhs, rs, st);
     }
@@ -706,7 +783,7 @@ void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x, int nrhs = 
         ProfScope ps(h, o.kind, o.work * nrhs);
         if (o.kind == OP_BWD)
             launch_bwd_level<T>(h->d_list_bwd.p + o.off, o.count, o.ntiny, h->d_bxs.p + o.xoff, o.xcount, h->sym,
-                                h->d_big.p, L, Dinv, X, y, h->d_t.p, x, nrhs, rs, st, o.lds);
+                                h->d_big.p, L, Dinv, X, y, h->d_t.p, x, nrhs, rs, st, o.wave);
         else
             launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, nrhs, rs, st);
     }
@@ -809,8 +886,6 @@ void ensure_fp64_factor(help_gpu* h) {
     if (h->d_L64.p) return;
     const Supernodal& S = h->S;
     h->d_L64.alloc(std::max<int64_t>(1, S.loff[S.nsuper]));
-    h->d_U64.alloc(std::max<int64_t>(1, S.uoff[S.nsuper]));
-    if (h->use_uacc) h->d_Uacc64.alloc(std::max<int64_t>(1, S.uoff[S.nsuper]));
     h->d_Dinv64.alloc(std::max<int64_t>(1, h->dinvoff[S.nsuper]));
     if (!h->pcopy.empty()) h->d_Lstage64.alloc(std::max<int64_t>(1, h->dinvoff[S.nsuper]));
     if (h->use_xinv) {
@@ -820,16 +895,17 @@ void ensure_fp64_factor(help_gpu* h) {
 }
 
 // ADA^T (+ beta on the diagonal) written into the zeroed factor storage of
-// the requested precision; d_dinv holds 1/d
+// the requested precision; d_dinv holds 1/d.  The update matrices are not
+// cleared here: the factorization's extend-add and small-front kernels zero
+// each front's update matrix before the first contribution to it.
 void assemble(help_gpu* h, bool f32) {
     hipStream_t st = h->st;
     const int32_t m = h->m;
     const Supernodal& S = h->S;
-    int64_t nl = S.loff[S.nsuper], nu = S.uoff[S.nsuper];
+    int64_t nl = S.loff[S.nsuper];
     int64_t nslot = (int64_t)h->P.colind.size();
     if (f32) {
         HIPCHK(hipMemsetAsync(h->d_L32.p, 0, sizeof(float) * nl, st));
-        HIPCHK(hipMemsetAsync(h->d_U32.p, 0, sizeof(float) * std::max<int64_t>(nu, 1), st));
         // factor S (A D^-1 A^T + beta I) S, S = diag(1/sqrt(diag)): unit
         // diagonal, entries bounded by 1, no FP32 overflow
         launch_csr_split(h->rows, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_dinv.p, h->d_t.p, true, st);
@@ -843,7 +919,6 @@ void assemble(help_gpu* h, bool f32) {
         launch_add_beta_scaled_f32(h->d_diagpos.p, m, h->opts.beta, h->d_scale.p, h->d_L32.p, st);
     } else {
         HIPCHK(hipMemsetAsync(h->d_L64.p, 0, sizeof(double) * nl, st));
-        HIPCHK(hipMemsetAsync(h->d_U64.p, 0, sizeof(double) * std::max<int64_t>(nu, 1), st));
         {
             ProfScope ps(h, HELP_K_ADAT, h->work_adat);
             launch_adat_assemble(h->d_pairptr.p, h->d_pair_p.p, h->d_pair_q.p, h->d_colof.p, h->d_val.p,
@@ -854,8 +929,8 @@ void assemble(help_gpu* h, bool f32) {
 }
 
 void factor(help_gpu* h, bool f32) {
-    if (f32) run_factor<float>(h, h->d_L32.p, h->d_U32.p, h->d_Dinv32.p);
-    else run_factor<double>(h, h->d_L64.p, h->d_U64.p, h->d_Dinv64.p);
+    if (f32) run_factor<float>(h, h->d_L32.p, reinterpret_cast<float*>(h->d_Ubuf.p), h->d_Dinv32.p);
+    else run_factor<double>(h, h->d_L64.p, h->d_Ubuf.p, h->d_Dinv64.p);
     h->factor_fp32 = f32;
 }
 

@@ -67,9 +67,12 @@ struct Op {
     int32_t ntiny = 0; // solves: leading supernodes of the list solved one per wave
     int32_t xoff = 0, xcount = 0;   // solves: the level's wide-supernode block tasks;
                                     // FP64 factor: the phase's fused panel tasks
-                                    // (POTRF; its TRSM op then has lds = 1)
+                                    // (POTRF; its TRSM op then has fused set)
     double work2 = 0;  // flops of the fused panel launch
-    int32_t wave = 0;  // small fronts: lanes per front of the one-wave kernel (64 or 32), 0 = workgroup
+    int32_t wave = 0;  // lanes per front or supernode of the one-wave forms (64 or 32):
+                       // small fronts (0 = workgroup kernel) and the tiny supernodes
+                       // of a solve level
+    bool fused = false;  // TRSM: the phase's panels ran in the fused POTRF launch
 };
 
 inline const char* const kKindName[HELP_K_COUNT] = {
@@ -122,6 +125,11 @@ struct help_gpu {
     bool use_xinv = false;
     bool use_uacc = false;   // some update matrix is built over several panels (Uacc)
     std::vector<int64_t> xoff;             // nsuper+1 offsets of the inverse blocks
+    // update-matrix placement: uplan[s] = element offset of front s's update
+    // matrix (and of its parked accumulator) in the shared update buffer,
+    // uplan[nsuper] = the buffer's length (fronts whose lifetimes do not
+    // overlap share memory)
+    std::vector<int64_t> uplan;
     std::vector<XTask> xtask;
     std::vector<SolveTask> fxs, bxs;       // row blocks (forward) / column blocks (backward)
     std::string err;
@@ -143,13 +151,16 @@ struct help_gpu {
     DBuf<double> d_val, d_csrval, d_d, d_dinv, d_b1, d_b2, d_x1, d_x2, d_rp, d_W, d_t, d_tn, d_r,
         d_c, d_z, d_p, d_q, d_part, d_scal, d_slotvals;
     DBuf<double> d_wnz;   // A_ij / d_j per nonzero (the assembly's first factor)
-    DBuf<double> d_L64, d_U64, d_Dinv64, d_Uacc64;
+    DBuf<double> d_L64, d_Dinv64;
+    // update matrices and parked accumulators: scratch of the factorization,
+    // one buffer each for both precisions (FP32 uses its first half)
+    DBuf<double> d_Ubuf, d_Uaccbuf;
     DBuf<double> d_Lstage64;   // L11 of the fused panels (Dinv layout) until the final copy
     // FP32 mode: symmetric diagonal scaling of the factored matrix
     std::vector<int32_t> slot_row;
     DBuf<int32_t> d_slot_row, d_slot_col;
     DBuf<double> d_scale;
-    DBuf<float> d_L32, d_U32, d_Dinv32, d_Uacc32;
+    DBuf<float> d_L32, d_Dinv32;
     DBuf<int8_t> d_big;
     DBuf<EaTask> d_ea;
     DBuf<int32_t> d_earange;

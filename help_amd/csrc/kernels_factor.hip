@@ -102,6 +102,12 @@ __global__ __launch_bounds__(256) void k_small_fronts(const int32_t* __restrict_
     const int tid = threadIdx.x, nt = blockDim.x;
     const int32_t cn = c * n;
     for (int32_t idx = tid; idx < cn; idx += nt) F[idx] = Lp[idx];
+    // the update matrix starts from zero (its lower part: the upper part is
+    // never used)
+    for (int32_t idx = tid; idx < cu * cu; idx += nt) {
+        const int32_t jj = idx / cu;
+        if (idx - jj * cu >= jj) Up[idx] = T(0);
+    }
     __syncthreads();
     // extend-add the children, one after the other
     for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
@@ -231,7 +237,11 @@ __global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ 
     T* Up = U + sym.uoff[s];
     const int32_t cn = c * n;
     for (int32_t idx = lane; idx < cn; idx += G) F[idx] = Lp[idx];
-    wave_sync();
+    for (int32_t idx = lane; idx < cu * cu; idx += G) {   // update matrix from zero, lower part
+        const int32_t jj = idx / cu;
+        if (idx - jj * cu >= jj) Up[idx] = T(0);
+    }
+    wave_fence_all();
     for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
         const int32_t ch = sym.child[t];
         const int32_t chn = sym.sfirst[ch + 1] - sym.sfirst[ch];
@@ -340,8 +350,9 @@ __global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ 
 }
 
 // ------------------------------------------------------------ extend-add
-// Task = kEaCols columns [c0, c1) of a front.  For each child in order, the
-// child columns that land in the range are spread over the 4 waves (two
+// Task = kEaCols columns [c0, c1) of a front.  The update-matrix columns of
+// the range are first set to zero (lower part), then for each child in order
+// the child columns that land in the range are spread over the 4 waves (two
 // columns per wave), the lanes take the rows, 4 rows per lane in flight.
 // Children with no column in the range are skipped without a barrier.
 template <typename T>
@@ -356,6 +367,13 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
     T* Lp = L + sym.loff[s];
     T* Up = U + sym.uoff[s];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (tk.c1 > n) {   // uniform over the workgroup
+        for (int32_t j = max(tk.c0, n); j < tk.c1; ++j) {
+            T* col = Up + (int64_t)(j - n) * cu - n;   // indexed by front row
+            for (int32_t i = j + (int32_t)threadIdx.x; i < c; i += 256) col[i] = T(0);
+        }
+        __syncthreads();
+    }
     for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
         const int32_t ch = sym.child[t];
         const int32_t chn = sym.sfirst[ch + 1] - sym.sfirst[ch];

@@ -90,7 +90,10 @@ def read_csc(path):
 
 
 def by_config(cfg, seed=0):
-    """The benchmark plan's configurations by index (0..3)."""
+    """The benchmark plan's configurations by index (0..4); 4 is a held-out
+    instance of the warm-start family (grids of side 60-150 with 2-8
+    commodities; the training instances use seeds 1000 and up, the held-out
+    ones 2000 and up): the smallest of its first eight, side 76 with K=2."""
     if cfg == 0:
         return random_lp(2000, 4000, 4, seed)
     if cfg == 1:
@@ -99,4 +102,6 @@ def by_config(cfg, seed=0):
         return staircase(200, 1000, 3000, 5, seed)
     if cfg == 3:
         return zipf(1_000_000, 3_000_000, 6, 1.1, seed)
+    if cfg == 4:
+        return mcf(76, 2, 20, 2002 + seed)
     raise ValueError(cfg)

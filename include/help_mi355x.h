@@ -67,7 +67,9 @@ typedef struct help_opts {
     int32_t fault_flags;     /* HELP_FAULT_*                                    */
     int32_t verbose;
     int32_t symbolic_only;   /* 1: analysis only, no device state (host tests)  */
-    int32_t front_path;      /* 0 auto; 1 every front through the tiled path   */
+    int32_t front_path;      /* 0 auto; 1 every front through the tiled path;
+                                2 (testing) every small front that fits one
+                                wave takes the 64-lane one-wave kernel        */
     int32_t profile;         /* 1: kernel-class profiling from creation on      */
     int32_t refine_fp64;     /* FP64 mode: refinement steps on the FP64 factor
                                 (0: one factor solve, as UseCholmod.cpp does)  */

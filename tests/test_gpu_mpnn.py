@@ -104,3 +104,26 @@ def test_learned_pcg_same_answer_and_fallback():
     for b in (plain, learned, forced):
         b.destruct()
     net.close()
+
+
+def test_committed_network_on_heldout_instance():
+    """The committed network (help_amd/data) in learned-PCG mode on a held-out
+    instance of its family at side >= 60 (side 76, K=2, seed 2002; training
+    used seeds 1000 and up): same answer as PCG from zero, no fallback."""
+    path = os.path.join(ROOT, "help_amd", "data", "warmstart_net_mcf.npz")
+    inst = instances.by_config(4)
+    _flush(f"held-out mcf side 76 K=2 m={inst.m} n={inst.n}")
+    tuples, res = harvest(inst, every=30, max_tuples=2, max_iter=40)
+    assert len(tuples) == 2
+    net = Mpnn.load(path)
+    plain = graph_handle(inst, pcg_k=50, pcg_tol=1e-10)
+    learned = graph_handle(inst, pcg_k=50, pcg_tol=1e-10, mode=L.MODE_PCG_LEARNED)
+    learned.set_warmstart(net)
+    for d, r, _ in tuples:
+        out = pcg_iterations(plain, learned, d, r)
+        _flush(f"  pcg iterations zero {out['it_zero']} net {out['it_net']} agree {out['agree']:.2e}")
+        assert out["fallback"] == 0
+        assert out["agree"] <= 1e-7
+    for b in (plain, learned):
+        b.destruct()
+    net.close()

@@ -384,8 +384,11 @@ def _first_bad_supernode(be, inst, d):
     return None
 
 
-@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("path", [0, 1, 2])
 def test_factor_paths_diagnostic(cfg0, path):
+    """Every supernode's columns against the oracle with the automatic front
+    schedule (0), every front through the tiled kernels (1), and every small
+    front that fits one wave through the 64-lane one-wave kernel (2)."""
     be = LinearEquation(front_path=path)
     be.construct(cfg0)
     rng = np.random.default_rng(4)
