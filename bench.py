@@ -300,6 +300,8 @@ def main():
     ap.add_argument("--inverse-solve", type=int, default=None, choices=[-1, 0, 1],
                     help="explicit L11 inverse for supernodes wider than 64 columns: "
                          "0 or 1 on (default), -1 off (block substitution)")
+    ap.add_argument("--factor-graph", type=int, default=None, choices=[0, 1],
+                    help="1: replay the factorization's launches as a captured HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         msg = ("bench.py measures one MI355X: this backend is single-GPU "
@@ -322,6 +324,8 @@ def main():
     extra = {}
     if args.inverse_solve is not None:
         extra["inverse_solve"] = args.inverse_solve
+    if args.factor_graph is not None:
+        extra["factor_graph"] = args.factor_graph
 
     mixed = {}
     if args.refine_tol is not None:

@@ -41,7 +41,8 @@ class Opts(C.Structure):
                 ("pcg_tol", C.c_double), ("pcg_fp32_precond", C.c_int32),
                 ("fault_flags", C.c_int32), ("verbose", C.c_int32), ("symbolic_only", C.c_int32),
                 ("front_path", C.c_int32), ("profile", C.c_int32), ("refine_fp64", C.c_int32),
-                ("inverse_solve", C.c_int32), ("ordering", C.c_int32), ("refine_switch", C.c_double)]
+                ("inverse_solve", C.c_int32), ("ordering", C.c_int32), ("factor_graph", C.c_int32),
+                ("refine_switch", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -129,6 +130,7 @@ SIGNATURES = {
     "help_gpu_factor_column": (C.c_int, [_P, C.c_int32, _I32, _D, _I32]),
     "help_gpu_supernodes": (C.c_int, [_P, _I32, _I32, _I32, _I32]),
     "help_selftest_mfma": (C.c_int, [C.c_int32, _D, _D, _D]),
+    "help_selftest_mfma_bf16": (C.c_int, [_D, _D, _D]),
     "help_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(LP)]),
     "help_lp_free": (None, [C.POINTER(LP)]),
     "help_write_mps": (C.c_int, [C.POINTER(LP), C.c_char_p, C.c_char_p]),

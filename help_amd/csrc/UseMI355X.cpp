@@ -11,6 +11,8 @@
 // warm-start network whose FP32 weights, help_mpnn_num_params() floats in
 // the library's parameter order, are read from the raw file named by
 // HELP_MI355X_NET; HELP_MI355X_NET_BF16=0 runs the network in FP32).
+// HELP_MI355X_ORDER = amd (default, CHOLMOD's setting) or nd (nested
+// dissection) picks the fill-reducing ordering of the direct modes.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -53,6 +55,9 @@ int LinearEquation_Construct() {
     else if (mode && !std::strcmp(mode, "pcg")) o.mode = HELP_MODE_PCG;
     else if (mode && !std::strcmp(mode, "pcg_learned")) o.mode = HELP_MODE_PCG_LEARNED;
     else if (mode && std::strcmp(mode, "fp64")) CheckError(1, (char*)"HELP_MI355X_MODE: fp64, fp32, pcg or pcg_learned");
+    const char* order = std::getenv("HELP_MI355X_ORDER");
+    if (order && !std::strcmp(order, "nd")) o.ordering = HELP_ORDER_ND;
+    else if (order && std::strcmp(order, "amd")) CheckError(1, (char*)"HELP_MI355X_ORDER: amd or nd");
     help_check(help_gpu_create(n_Row, n_Col, (int64_t)rowind.size(), colptr.data(), rowind.data(),
                                val.data(), &o, &g_help_handle),
                "construct");

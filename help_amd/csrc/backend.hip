@@ -928,9 +928,47 @@ void assemble(help_gpu* h, bool f32) {
     }
 }
 
+// The factorization's launches depend only on the symbolic schedule and the
+// handle's buffers, so with opts.factor_graph they are captured once per
+// precision (both streams: the side stream joins the capture through the
+// fork event and rejoins the main stream before it ends) and replayed on
+// every later renew.  The capture records kernels and events only (the
+// renew's memsets stay outside).  Not used while profiling (the profiling
+// events are per launch).
+template <typename T>
+void factor_launches(help_gpu* h) {
+    if constexpr (std::is_same<T, float>::value)
+        run_factor<float>(h, h->d_L32.p, reinterpret_cast<float*>(h->d_Ubuf.p), h->d_Dinv32.p);
+    else
+        run_factor<double>(h, h->d_L64.p, h->d_Ubuf.p, h->d_Dinv64.p);
+}
+
 void factor(help_gpu* h, bool f32) {
-    if (f32) run_factor<float>(h, h->d_L32.p, reinterpret_cast<float*>(h->d_Ubuf.p), h->d_Dinv32.p);
-    else run_factor<double>(h, h->d_L64.p, h->d_Ubuf.p, h->d_Dinv64.p);
+    if (h->opts.factor_graph && !h->prof) {
+        hipGraphExec_t& ge = h->fgraph[f32 ? 1 : 0];
+        if (!ge) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+            try {
+                if (f32) factor_launches<float>(h);
+                else factor_launches<double>(h);
+            } catch (...) {
+                hipGraph_t dead = nullptr;
+                (void)hipStreamEndCapture(h->st, &dead);
+                if (dead) (void)hipGraphDestroy(dead);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(h->st, &g));
+            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { ge = nullptr; HIPCHK(e); }
+        }
+        HIPCHK(hipGraphLaunch(ge, h->st));
+    } else if (f32) {
+        factor_launches<float>(h);
+    } else {
+        factor_launches<double>(h);
+    }
     h->factor_fp32 = f32;
 }
 

@@ -178,6 +178,9 @@ struct help_gpu {
     // (fork: main -> side, join: side -> main)
     hipStream_t st_side = nullptr;
     hipEvent_t ev_side[2] = {nullptr, nullptr};
+    // the factorization's launch sequence captured once per precision
+    // (opts.factor_graph): [0] FP64, [1] FP32
+    hipGraphExec_t fgraph[2] = {nullptr, nullptr};
     // pinned host words for results read after the solve's final
     // synchronization (the residual norms of an unrefined solve)
     double* h_pin = nullptr;
@@ -204,6 +207,8 @@ struct help_gpu {
     ~help_gpu() {
         if (st) { (void)hipStreamSynchronize(st); }
         if (st_side) { (void)hipStreamSynchronize(st_side); }
+        for (auto& g : fgraph)
+            if (g) (void)hipGraphExecDestroy(g);
         for (auto& e : ev_side)
             if (e) (void)hipEventDestroy(e);
         if (st_side) (void)hipStreamDestroy(st_side);

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "launch.h"
 #include "mpnn.h"
@@ -117,6 +118,82 @@ __global__ __launch_bounds__(256) void k_dense(int64_t N, const IT* __restrict__
         }
         __syncthreads();
     }
+}
+
+// The BF16 layers with K = k1 + k2 a multiple of 32 (the message layers and
+// the head) run on V_MFMA_F32_16X16X32_BF16 with FP32 accumulation: a
+// workgroup takes 64 nodes, each wave 16 of them across the 64 outputs (four
+// 16x16 tiles, the bias as the accumulators' start).  Lane layout (checked by
+// help_selftest_mfma_bf16): A[i][k] in lane i + 16 (k/8), element k%8;
+// B[k][j] in lane j + 16 (k/8), element k%8; D[i][j] in lane j + 16 (i/4),
+// element i%4.  A lane's 8 inputs are 16 contiguous bytes of one node's row;
+// the weights are staged in LDS transposed (row o holds W[.][o], padded by 8
+// elements against bank conflicts) as BF16, which is exact: the BF16 mode's
+// weights are BF16-rounded.
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
+typedef __attribute__((ext_vector_type(4))) float f32x4_t;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4_t;
+
+__device__ inline uint16_t bf16_bits(float x) {
+    Bf16 h;
+    stv(h, x);
+    return h.v;
+}
+
+__global__ __launch_bounds__(256) void k_dense_mfma(int64_t N, const Bf16* __restrict__ in1, int k1,
+                                                    const Bf16* __restrict__ in2, int k2,
+                                                    const float* __restrict__ W, const float* __restrict__ b,
+                                                    Bf16* __restrict__ out) {
+    constexpr int kLd = 2 * kNnH + 8;
+    __shared__ __attribute__((aligned(16))) uint16_t Wt[kNnH * kLd];
+    const int K = k1 + k2;
+    for (int t = threadIdx.x; t < K * kNnH; t += kB) {
+        const int k = t / kNnH, o = t - k * kNnH;
+        Wt[o * kLd + k] = bf16_bits(W[t]);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    float bias[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) bias[t] = b[16 * t + li];
+    __syncthreads();
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < N; base += (int64_t)gridDim.x * 64) {
+        const int64_t row = base + wave * 16 + li;
+        const int64_t rc = row < N ? row : N - 1;
+        f32x4_t acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = f32x4_t{bias[t], bias[t], bias[t], bias[t]};
+        for (int kk = 0; kk < K; kk += 32) {
+            const int k0 = kk + 8 * lk;
+            const Bf16* src = k0 < k1 ? in1 + rc * k1 + k0 : in2 + rc * k2 + (k0 - k1);
+            const bf16x8_t a = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(src));
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bf16x8_t bv =
+                    __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&Wt[(16 * t + li) * kLd + k0]));
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t node = base + wave * 16 + 4 * lk + r;
+                if (node < N) stv(out[node * kNnH + 16 * t + li], fmaxf(acc[t][r], 0.0f));
+            }
+    }
+}
+
+// lane-layout probe: lane l feeds a[8l .. 8l+7] and b[8l .. 8l+7] (BF16-exact
+// values) to one V_MFMA_F32_16X16X32_BF16 from zero; out[4l + r] = element r
+__global__ void k_mfma_bf16_probe(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out) {
+    const int l = threadIdx.x;
+    uint16_t ua[8], ub[8];
+    for (int e = 0; e < 8; ++e) { ua[e] = bf16_bits(a[8 * l + e]); ub[e] = bf16_bits(b[8 * l + e]); }
+    const bf16x8_t va = __builtin_bit_cast(bf16x8_t, ua);
+    const bf16x8_t vb = __builtin_bit_cast(bf16x8_t, ub);
+    const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+    for (int r = 0; r < 4; ++r) out[4 * l + r] = d[r];
 }
 
 // out[node] = sum_{p in adj(node)} val[p] src[idx[p]] / max(deg, 1)
@@ -346,7 +423,18 @@ void nn_logsum_partials(int64_t n, const double* x, int mode, double* part, int 
 template <typename AT, typename IT>
 void nn_dense(int64_t N, const IT* in1, int k1, const AT* in2, int k2, const float* W, const float* b, AT* out,
               hipStream_t st) {
-    if (N > 0) help::launch_kernel(k_dense<AT, IT>, grid_cap((N + 3) / 4, 2048), kB, 0, st, N, in1, k1, in2, k2, W, b, out);
+    if (N <= 0) return;
+    if constexpr (std::is_same<AT, Bf16>::value && std::is_same<IT, Bf16>::value) {
+        if ((k1 + k2) % 32 == 0 && k1 % 8 == 0 && k1 + k2 <= 2 * kNnH) {
+            help::launch_kernel(k_dense_mfma, grid_cap((N + 63) / 64, 2048), kB, 0, st, N, in1, k1, in2, k2, W, b, out);
+            return;
+        }
+    }
+    help::launch_kernel(k_dense<AT, IT>, grid_cap((N + 3) / 4, 2048), kB, 0, st, N, in1, k1, in2, k2, W, b, out);
+}
+
+void nn_mfma_bf16_probe(const float* a, const float* b, float* out, hipStream_t st) {
+    help::launch_kernel(k_mfma_bf16_probe, 1, 64, 0, st, a, b, out);
 }
 
 template <typename AT>

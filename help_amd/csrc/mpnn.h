@@ -17,7 +17,8 @@
 //
 // Activations are stored in BF16 (or FP32 for gradient checks); every
 // product accumulates in FP32; the weights used by the kernels are the FP32
-// master weights rounded to BF16 in the BF16 mode.
+// master weights rounded to BF16 in the BF16 mode, where the message layers
+// and the head (K = 128 and 64) run on BF16 MFMA.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -52,6 +53,8 @@ void nn_logsum_partials(int64_t n, const double* x, int mode, double* part, int 
 template <typename AT, typename IT>
 void nn_dense(int64_t N, const IT* in1, int k1, const AT* in2, int k2, const float* W, const float* b, AT* out,
               hipStream_t st);
+// one V_MFMA_F32_16X16X32_BF16 from zero: lane l feeds a[8l..8l+7], b[8l..8l+7]; out[4l+r]
+void nn_mfma_bf16_probe(const float* a, const float* b, float* out, hipStream_t st);
 template <typename AT>
 void nn_mean(int32_t N, const int64_t* ptr, const int32_t* idx, const double* val, const AT* src, AT* out,
              hipStream_t st);

@@ -218,6 +218,25 @@ extern "C" {
 
 int help_mpnn_num_params(void) { return NnLayout::total; }
 
+int help_selftest_mfma_bf16(const double* a, const double* b, double* out) {
+    if (!a || !b || !out) return HELP_ERR_ARG;
+    try {
+        if (help_device_count() == 0) return HELP_ERR_NOGPU;
+        std::vector<float> fa(a, a + 512), fb(b, b + 512), fo(256);
+        DBuf<float> da, db, dout;
+        da.alloc(512); db.alloc(512); dout.alloc(256);
+        HIPCHK(hipMemcpy(da.p, fa.data(), 512 * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(db.p, fb.data(), 512 * sizeof(float), hipMemcpyHostToDevice));
+        help::nn_mfma_bf16_probe(da.p, db.p, dout.p, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(fo.data(), dout.p, 256 * sizeof(float), hipMemcpyDeviceToHost));
+        for (int i = 0; i < 256; ++i) out[i] = fo[i];
+        return HELP_OK;
+    } catch (const std::exception& e) {
+        return fail_msg(nullptr, e);
+    }
+}
+
 int help_mpnn_new(int32_t bf16, uint64_t seed, help_mpnn** out) {
     if (!out) return HELP_ERR_ARG;
     *out = nullptr;

@@ -141,24 +141,36 @@ def train(net, data, epochs=10, lr=2e-3, log=None, seed=0):
     return hist
 
 
+def _pcg_solve(be, d, b1, b2):
+    """One PCG-mode solve; a solve that reaches the iteration cap
+    (HELP_ERR_NOCONV) still returns its iterate, flagged."""
+    x1 = np.empty(be.n)
+    x2 = np.empty(be.m)
+    t = time.time()
+    st = L.lib().help_gpu_solve(be.h, L.dptr(d), L.dptr(b1), L.dptr(b2), L.dptr(x1), L.dptr(x2),
+                                C.byref(be.solve_stats))
+    elapsed = time.time() - t
+    if st not in (L.HELP_OK, L.HELP_ERR_NOCONV):
+        be._check(st, "solve")
+    return x2, elapsed, st == L.HELP_ERR_NOCONV
+
+
 def pcg_iterations(be_plain, be_learned, d, r):
     """PCG iterations for A D^-1 A^T x = r from zero (be_plain, MODE_PCG) and
-    from the network's guess (be_learned, MODE_PCG_LEARNED), same target."""
+    from the network's guess (be_learned, MODE_PCG_LEARNED), same target; a
+    solve at the iteration cap counts the cap and is flagged."""
     n = be_plain.n
+    d = np.ascontiguousarray(d, np.float64)
     b1 = np.zeros(n)
-    b2 = -np.asarray(r, np.float64)
+    b2 = np.ascontiguousarray(-np.asarray(r, np.float64))
     be_plain.renew(d)
-    t = time.time()
-    _, x_zero = be_plain.solve(d, b1, b2)
-    t_zero = time.time() - t
+    x_zero, t_zero, cap_zero = _pcg_solve(be_plain, d, b1, b2)
     it_zero = be_plain.solve_stats.pcg_iters
     be_learned.renew(d)
-    t = time.time()
-    _, x_net = be_learned.solve(d, b1, b2)
-    t_net = time.time() - t
+    x_net, t_net, cap_net = _pcg_solve(be_learned, d, b1, b2)
     it_net = be_learned.solve_stats.pcg_iters
-    return dict(it_zero=it_zero, it_net=it_net, t_zero=t_zero, t_net=t_net,
-                fallback=be_learned.solve_stats.fallback,
+    return dict(it_zero=it_zero, it_net=it_net, t_zero=t_zero, t_net=t_net, capped_zero=cap_zero,
+                capped_net=cap_net, fallback=be_learned.solve_stats.fallback,
                 agree=float(np.linalg.norm(x_net - x_zero) / max(np.linalg.norm(x_zero), 1e-300)))
 
 
@@ -194,6 +206,8 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--pcg-k", type=int, default=50)
     ap.add_argument("--eval-per-instance", type=int, default=3)
+    ap.add_argument("--eval-cap", type=int, default=20000,
+                    help="PCG iteration cap of the held-out comparisons")
     ap.add_argument("--harvest-iter", type=int, default=0,
                     help="interior-point iteration cap of each harvesting run (0: the driver's own)")
     ap.add_argument("--e2e", type=int, default=1,
@@ -233,8 +247,9 @@ def main(argv=None):
         g = graph_handle(inst)
         losses = [net.loss_grad(g, *t) for t in tuples]
         g.destruct()
-        plain = graph_handle(inst, pcg_k=args.pcg_k, pcg_tol=1e-10)
-        learned = graph_handle(inst, pcg_k=args.pcg_k, pcg_tol=1e-10, mode=L.MODE_PCG_LEARNED)
+        plain = graph_handle(inst, pcg_k=args.pcg_k, pcg_tol=1e-10, pcg_max_iter=args.eval_cap)
+        learned = graph_handle(inst, pcg_k=args.pcg_k, pcg_tol=1e-10, pcg_max_iter=args.eval_cap,
+                               mode=L.MODE_PCG_LEARNED)
         learned.set_warmstart(net)
         pick = np.linspace(0, len(tuples) - 1, min(args.eval_per_instance, len(tuples))).astype(int)
         for i in pick:
@@ -242,7 +257,8 @@ def main(argv=None):
             out.update(instance=key, tuple=int(i), loss=float(losses[i]), m=inst.m, n=inst.n)
             rows.append(out)
             log(f"test {key} tuple {i}: loss {losses[i]:.3f} pcg iterations zero {out['it_zero']} "
-                f"net {out['it_net']} ({out['t_zero']:.3f}s / {out['t_net']:.3f}s) agree {out['agree']:.1e}")
+                f"net {out['it_net']} ({out['t_zero']:.3f}s / {out['t_net']:.3f}s) agree {out['agree']:.1e}"
+                + (" (cap reached)" if out["capped_zero"] or out["capped_net"] else ""))
         plain.destruct()
         learned.destruct()
     it0 = np.array([r["it_zero"] for r in rows], float)
@@ -271,6 +287,8 @@ def main(argv=None):
         n_train=args.train, n_test=args.test, training_tuples=len(data), epochs=args.epochs,
         params=net.nparams, precision="bf16 activations and weights, fp32 accumulation",
         loss_history=hist, harvest_seconds=t_harvest, train_seconds=t_train, pcg_k=args.pcg_k, pcg_tol=1e-10,
+        eval_pcg_cap=args.eval_cap,
+        capped_solves=int(sum(1 for r in rows if r["capped_zero"] or r["capped_net"])),
         pcg_iterations_zero_mean=float(it0.mean()), pcg_iterations_net_mean=float(it1.mean()),
         iteration_ratio_mean=float((it1 / np.maximum(it0, 1)).mean()),
         time_zero_mean=float(tz.mean()), time_net_mean=float(tn.mean()),

@@ -84,6 +84,9 @@ typedef struct help_opts {
                                 dissection, the family of the METIS ordering
                                 HELP's PARDISO backend asks for,
                                 UsePardiso.cpp:101)                            */
+    int32_t factor_graph;    /* 1: capture the factorization's launches once per
+                                precision as a HIP graph and replay it on every
+                                renew (0: launch them one by one)             */
     double  refine_switch;   /* FP32 mode: when the solves since the last renew
                                 took more than this many refinement steps on
                                 average, the next renews factor in FP64 (d's
@@ -288,6 +291,11 @@ int help_gpu_supernodes(const help_gpu* h, int32_t* sfirst, int32_t* snrow,
  * V_MFMA_F64_16X16X4_F64, else V_MFMA_F32_16X16X4_F32): lane l feeds
  * a[l] and b[l]; out[4*l + r] receives element r of lane l's result. */
 int help_selftest_mfma(int32_t fp64, const double* a, const double* b, double* out);
+
+/* Lane-layout probe of V_MFMA_F32_16X16X32_BF16 (the warm-start network's
+ * dense layers): lane l feeds a[8l..8l+7] and b[8l..8l+7] (values exact in
+ * BF16); out[4*l + r] receives element r of lane l's result. */
+int help_selftest_mfma_bf16(const double* a, const double* b, double* out);
 
 /* ------------------------------------------------------------------ */
 /* Instance generators (host; deterministic splitmix64/xoshiro256**).  */

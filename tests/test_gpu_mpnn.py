@@ -30,6 +30,24 @@ def _case(seed=0, side=6):
     return inst, d, r, dy
 
 
+def test_mfma_bf16_lane_layout():
+    """In-bounds probe of V_MFMA_F32_16X16X32_BF16, the BF16 dense layers'
+    instruction: A[i][k] in lane i+16(k/8) element k%8, B[k][j] in lane
+    j+16(k/8) element k%8, D[i][j] in lane j+16(i/4) element i%4 (small
+    integers: every product and sum is exact)."""
+    rng = np.random.default_rng(12)
+    A = rng.integers(-4, 5, (16, 32)).astype(np.float64)
+    B = rng.integers(-4, 5, (32, 16)).astype(np.float64)
+    a = np.array([A[l % 16, 8 * (l // 16) + e] for l in range(64) for e in range(8)])
+    b = np.array([B[8 * (l // 16) + e, l % 16] for l in range(64) for e in range(8)])
+    out = np.zeros(256)
+    assert L.lib().help_selftest_mfma_bf16(L.dptr(a), L.dptr(b), L.dptr(out)) == 0
+    D = A @ B
+    exp = np.array([D[4 * (l // 16) + r, l % 16] for l in range(64) for r in range(4)])
+    _flush(f"mfma bf16 16x16x32 max|diff|={np.abs(out - exp).max()}")
+    assert np.array_equal(out, exp)
+
+
 def test_gradients_fp32_match_autograd():
     import mpnn_oracle as MO
     inst, d, r, dy = _case()

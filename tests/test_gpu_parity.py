@@ -400,3 +400,32 @@ def test_factor_paths_diagnostic(cfg0, path):
            f"(s, level, ncol, nrow, tiled, err, children) = {bad}")
     be.destruct()
     assert bad is None
+
+
+@pytest.mark.parametrize("mode", [L.MODE_DIRECT_FP64, L.MODE_DIRECT_FP32])
+def test_factor_graph_replays_match_eager(small_mcf, mode):
+    """The factorization captured once as a HIP graph and replayed on later
+    renews gives the eager launches' factor bit for bit, replay after replay,
+    with other eager work and synchronizations between the replays."""
+    inst = small_mcf
+    eager = LinearEquation(mode=mode)
+    graph = LinearEquation(mode=mode, factor_graph=1)
+    eager.construct(inst)
+    graph.construct(inst)
+    rng = np.random.default_rng(21)
+    cols = rng.choice(inst.m, 40, replace=False)
+    for rep in range(4):
+        d = log_uniform(rng, inst.n, -2, 2)
+        _flush(f"factor graph mode={mode} renew {rep}")
+        eager.renew(d)
+        graph.renew(d)
+        for j in cols:
+            re_, ve = eager.factor_column(int(j))
+            rg, vg = graph.factor_column(int(j))
+            assert np.array_equal(re_, rg) and np.array_equal(ve, vg)
+        b1, b2 = rng.standard_normal(inst.n), rng.standard_normal(inst.m)
+        xe = eager.solve(d, b1, b2)
+        xg = graph.solve(d, b1, b2)
+        assert np.array_equal(xe[0], xg[0]) and np.array_equal(xe[1], xg[1])
+    eager.destruct()
+    graph.destruct()

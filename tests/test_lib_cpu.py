@@ -33,6 +33,23 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_options_struct_matches_the_header():
+    """help_opts_default through the Python mirror of help_opts: every field
+    lands where the header puts it (a field added on one side only shifts
+    the later ones)."""
+    o = L.Opts()
+    L.lib().help_opts_default(C.byref(o))
+    assert o.mode == L.MODE_DIRECT_FP64 and o.beta == 1e-5
+    assert o.refine_max == 20 and o.refine_tol == 1e-12
+    assert o.pcg_k == 500 and o.pcg_max_iter == 5000 and o.pcg_tol == 1e-10 and o.pcg_fp32_precond == 1
+    assert o.ordering == L.ORDER_AMD and o.factor_graph == 0 and o.inverse_solve == 0
+    assert o.refine_switch == 2.5
+    fields = re.search(r"typedef struct help_opts \{(.*?)\} help_opts;", open(HEADER).read(), re.S).group(1)
+    fields = re.sub(r"/\*.*?\*/", "", fields, flags=re.S)
+    declared = re.findall(r"\b(?:int32_t|double)\s+(\w+)\s*;", fields)
+    assert declared == [f[0] for f in L.Opts._fields_]
+
+
 def test_no_gpu_is_an_error_not_a_fallback():
     if L.lib().help_device_count() > 0:
         pytest.skip("a GPU is visible")
