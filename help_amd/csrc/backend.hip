@@ -306,16 +306,19 @@ void build_schedules(help_gpu* h) {
             w += ea_bytes(s) + 4.0 * (c - nc) * (c - nc + 1);
             for (int32_t c0 = leaf ? nc / kEaCols * kEaCols : 0; c0 < c; c0 += kEaCols) {
                 const int32_t c1 = std::min(c, c0 + kEaCols);
-                h->ea.push_back({s, c0, c1, (int32_t)h->earange.size()});
-                // per child: the update columns that land in [c0, c1) (the
-                // relative map of a child is ascending)
+                EaTask tk{s, c0, c1, (int32_t)h->earef.size(), 0, nc, c, S.loff[s], h->uplan[s]};
+                // the children whose update columns land in [c0, c1) (the
+                // relative map of a child is ascending), in child order
                 for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
                     const int32_t ch = S.child[t];
                     const int32_t chc = S.snrow[ch] - ncol(ch);
                     const int32_t* rel = S.relmap.data() + S.reloff[ch];
-                    h->earange.push_back((int32_t)(std::lower_bound(rel, rel + chc, c0) - rel));
-                    h->earange.push_back((int32_t)(std::lower_bound(rel, rel + chc, c1) - rel));
+                    const int32_t lo = (int32_t)(std::lower_bound(rel, rel + chc, c0) - rel);
+                    const int32_t hi = (int32_t)(std::lower_bound(rel, rel + chc, c1) - rel);
+                    if (lo < hi) h->earef.push_back({h->uplan[ch], S.reloff[ch], chc, lo, hi});
                 }
+                tk.nref = (int32_t)h->earef.size() - tk.r0;
+                h->ea.push_back(tk);
             }
         }
         if ((int32_t)h->ea.size() > off) h->fops.push_back({OP_EA, off, (int32_t)h->ea.size() - off, w});
@@ -448,6 +451,12 @@ void build_schedules(help_gpu* h) {
             if ((int32_t)h->xtask.size() > xo)
                 h->fops.push_back({OP_XINV, xo, (int32_t)h->xtask.size() - xo, wx});
         }
+    }
+    for (GemmTask& g : h->gemm) {
+        g.n = ncol(g.s);
+        g.c = S.snrow[g.s];
+        g.loff = S.loff[g.s];
+        g.uoff = h->uplan[g.s];
     }
     // triangular solves
     // algorithmic bytes of the solves: each stored factor entry is read once
@@ -623,7 +632,7 @@ void setup_device_factor(help_gpu* h) {
     h->d_list_bwd.upload(h->list_bwd, st);
     h->d_big.upload(h->sbig, st);
     h->d_ea.upload(h->ea, st);
-    h->d_earange.upload(h->earange, st);
+    h->d_earef.upload(h->earef, st);
     h->d_potrf.upload(h->potrf, st);
     h->d_panel.upload(h->panel, st);
     h->d_pcopy.upload(h->pcopy, st);
@@ -729,7 +738,7 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
                 else
                     launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st);
                 break;
-            case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->d_earange.p, h->sym, L, U, st); break;
+            case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->d_earef.p, h->sym, L, U, st); break;
             case OP_POTRF:
                 if constexpr (std::is_same<T, double>::value) {
                     if (fused) {

@@ -49,6 +49,11 @@ struct GemmTask {
     // C -= Uacc + A B^T.  The MFMA accumulation runs in the same k order as
     // one product over the whole panel, so the result is the same.
     int32_t mode = 0;
+    // the front's sizes and storage offsets, copied from the symbolic
+    // structure when the schedule is built: the task's first load carries
+    // everything its addresses depend on
+    int32_t n = 0, c = 0;
+    int64_t loff = 0, uoff = 0;
 };
 enum : int32_t { kGemmDirect = 0, kGemmFirst = 1, kGemmNext = 2, kGemmLast = 3 };
 
@@ -68,7 +73,19 @@ constexpr int kEaCols = 8;
 struct EaTask {
     int32_t s;
     int32_t c0, c1;
-    int32_t r0;   // ranges[r0 + 2t, r0 + 2t + 1]: the columns of child t of s that land in [c0, c1)
+    int32_t r0, nref;     // refs[r0 .. r0+nref): the children with columns in [c0, c1), in child order
+    int32_t n, c;         // the front's columns and rows
+    int64_t loff, uoff;   // the front's storage offsets
+};
+// One child of an extend-add task: where its update matrix and relative map
+// live, its update rows, and its columns [lo, hi) that land in the task's
+// range (the relative map is ascending).
+struct EaRef {
+    int64_t uoff;
+    int64_t reloff;
+    int32_t chc;
+    int32_t lo, hi;
+    int32_t pad = 0;
 };
 
 constexpr int kTile = 64;      // GEMM tile and panel width
@@ -137,7 +154,7 @@ void launch_small_wave(const int32_t* list, int32_t count, DevSym sym, T* L, T* 
                        int32_t* err, int32_t lds_elems, int32_t lanes, hipStream_t st);
 constexpr int kSmallWaveLds = 2048;   // LDS elements per front of the one-wave form
 template <typename T>
-void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges, DevSym sym, T* L, T* U,
+void launch_extend_add(const EaTask* tasks, int32_t count, const EaRef* refs, DevSym sym, T* L, T* U,
                        hipStream_t st);
 // factor the 64x64 diagonal block at column b and store its inverse
 template <typename T>

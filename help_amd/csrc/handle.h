@@ -116,7 +116,7 @@ struct help_gpu {
     std::vector<Op> fops, sops_fwd, sops_bwd;
     std::vector<int32_t> list_small, list_fwd, list_bwd;
     std::vector<EaTask> ea;
-    std::vector<int32_t> earange;
+    std::vector<help::EaRef> earef;
     std::vector<PanelTask> potrf, trsm, panel;
     std::vector<PanelTask> pcopy;   // diagonal blocks factored by fused panel launches
     std::vector<GemmTask> gemm;
@@ -163,7 +163,7 @@ struct help_gpu {
     DBuf<float> d_L32, d_Dinv32;
     DBuf<int8_t> d_big;
     DBuf<EaTask> d_ea;
-    DBuf<int32_t> d_earange;
+    DBuf<help::EaRef> d_earef;
     DBuf<PanelTask> d_potrf, d_trsm, d_panel, d_pcopy;
     DBuf<GemmTask> d_gemm;
     DBuf<SolveTask> d_fgemv, d_bgemv, d_fxs, d_bxs;

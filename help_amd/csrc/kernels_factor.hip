@@ -357,15 +357,14 @@ __global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ 
 // Children with no column in the range are skipped without a barrier.
 template <typename T>
 __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks,
-                                                    const int32_t* __restrict__ ranges, DevSym sym,
+                                                    const EaRef* __restrict__ refs, DevSym sym,
                                                     T* __restrict__ L, T* __restrict__ U) {
     const EaTask tk = tasks[blockIdx.x];
-    const int32_t s = tk.s;
-    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
-    const int32_t c = sym.snrow[s];
+    const int32_t n = tk.n;
+    const int32_t c = tk.c;
     const int32_t cu = c - n;
-    T* Lp = L + sym.loff[s];
-    T* Up = U + sym.uoff[s];
+    T* Lp = L + tk.loff;
+    T* Up = U + tk.uoff;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (tk.c1 > n) {   // uniform over the workgroup
         for (int32_t j = max(tk.c0, n); j < tk.c1; ++j) {
@@ -374,16 +373,12 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
         }
         __syncthreads();
     }
-    for (int32_t t = sym.childptr[s]; t < sym.childptr[s + 1]; ++t) {
-        const int32_t ch = sym.child[t];
-        const int32_t chn = sym.sfirst[ch + 1] - sym.sfirst[ch];
-        const int32_t chc = sym.snrow[ch] - chn;
-        const int32_t* rel = sym.relmap + sym.reloff[ch];
-        const T* Uc = U + sym.uoff[ch];
-        // the child columns that land in [c0, c1), precomputed on the host
-        const int32_t lo = ranges[tk.r0 + 2 * (t - sym.childptr[s])];
-        const int32_t hi2 = ranges[tk.r0 + 2 * (t - sym.childptr[s]) + 1];
-        if (lo >= hi2) continue;   // uniform over the workgroup
+    for (int32_t t = 0; t < tk.nref; ++t) {
+        const EaRef rf = refs[tk.r0 + t];
+        const int32_t chc = rf.chc;
+        const int32_t* rel = sym.relmap + rf.reloff;
+        const T* Uc = U + rf.uoff;
+        const int32_t lo = rf.lo, hi2 = rf.hi;
         // each wave takes two child columns at a time, 4 rows of each per lane
         // in flight; all loads are issued before the read-modify-writes
         for (int32_t j0 = lo + wave; j0 < hi2; j0 += 8) {
@@ -936,13 +931,12 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
     __shared__ T As[2][kKc * kTile];
     __shared__ T Bs[2][kKc * kTile];
     const GemmTask tk = tasks[blockIdx.x];
-    const int32_t s = tk.s;
-    const int32_t n = sym.sfirst[s + 1] - sym.sfirst[s];
-    const int32_t c = sym.snrow[s];
+    const int32_t n = tk.n;
+    const int32_t c = tk.c;
     const int32_t cu = c - n;
-    const T* Lp = L + sym.loff[s];
-    T* Lw = L + sym.loff[s];
-    T* Up = U + sym.uoff[s];
+    const T* Lp = L + tk.loff;
+    T* Lw = L + tk.loff;
+    T* Up = U + tk.uoff;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
@@ -951,7 +945,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
         for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
     // update-matrix tile continued from an earlier panel: reload the parked
     // accumulator (entries outside the tile's lower part read element 0)
-    T* Ua = Uacc ? Uacc + sym.uoff[s] : nullptr;
+    T* Ua = Uacc ? Uacc + tk.uoff : nullptr;
     if (tk.mode == kGemmNext || tk.mode == kGemmLast) {
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -1007,24 +1001,29 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
                 }
         return;
     }
-    // read-modify-write of C: every load is issued before the first store
-    // (entries outside the lower part or the front read the front's first
-    // element and are not written back)
+    // read-modify-write of C: every load is issued before the first store.
+    // A task's tile lies in L's columns (col_end <= n) or in the update
+    // matrix (col0 >= n), so one tile origin (uniform over the workgroup)
+    // and 32-bit element offsets from it address every entry; entries
+    // outside the lower part or the front read the origin and are not
+    // written back.
+    const bool inU = tk.col0 >= n;
+    const int32_t ld = inU ? cu : c;
+    T* Ct = inU ? Up + (int64_t)(tk.col0 - n) * cu + (tk.row0 - n) : Lw + (int64_t)tk.col0 * c + tk.row0;
     T cv[2][2][4];
-    T* cp[2][2][4];
+    int32_t co[2][2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int32_t i = tk.row0 + wr + a * 16 + Mfma<T>::row(lane, r);
-                const int32_t j = tk.col0 + wc + b * 16 + (lane & 15);
+                const int32_t ti = wr + a * 16 + Mfma<T>::row(lane, r);
+                const int32_t tj = wc + b * 16 + (lane & 15);
+                const int32_t i = tk.row0 + ti, j = tk.col0 + tj;
                 const bool ok = i < c && j < tk.col_end && i >= j;
-                T* p = (j < n) ? Lw + (int64_t)j * c + i : Up + (int64_t)(j - n) * cu + (i - n);
-                p = ok ? p : nullptr;
-                cp[a][b][r] = p;
-                cv[a][b][r] = *(ok ? p : Lw);
+                co[a][b][r] = ok ? tj * ld + ti : -1;
+                cv[a][b][r] = Ct[ok ? tj * ld + ti : 0];
             }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -1032,7 +1031,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                if (cp[a][b][r]) *cp[a][b][r] = cv[a][b][r] - acc[a][b][r];
+                if (co[a][b][r] >= 0) Ct[co[a][b][r]] = cv[a][b][r] - acc[a][b][r];
 }
 
 // X := X L_bb^-T for the 64 rows of block rb-1 below the diagonal block,
@@ -1258,14 +1257,15 @@ void launch_small_wave(const int32_t* list, int32_t count, DevSym sym, T* L, T* 
                             list, count, lds_elems, sym, L, U, Dinv, err);
 }
 template <typename T>
-void launch_extend_add(const EaTask* tasks, int32_t count, const int32_t* ranges, DevSym sym, T* L, T* U,
+void launch_extend_add(const EaTask* tasks, int32_t count, const EaRef* refs, DevSym sym, T* L, T* U,
                        hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_extend_add<T>, count, 256, 0, st, tasks, ranges, sym, L, U);
+    if (count > 0) help::launch_kernel(k_extend_add<T>, count, 256, 0, st, tasks, refs, sym, L, U);
 }
 template <typename T>
 void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, T* Uacc, hipStream_t st) {
     if (count > 0) help::launch_kernel(k_gemm<T>, count, 256, 0, st, tasks, sym, L, U, Uacc);
 }
+
 template <typename T>
 void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv, int32_t* err,
                        hipStream_t st) {
@@ -1301,7 +1301,7 @@ void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, cons
                                          int32_t, hipStream_t);                                     \
     template void launch_small_wave<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,        \
                                        int32_t, int32_t, hipStream_t);                              \
-    template void launch_extend_add<T>(const EaTask*, int32_t, const int32_t*, DevSym, T*, T*, hipStream_t); \
+    template void launch_extend_add<T>(const EaTask*, int32_t, const EaRef*, DevSym, T*, T*, hipStream_t); \
     template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, T*, hipStream_t);           \
     template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, T*, int32_t*,        \
                                        hipStream_t);                                                \
