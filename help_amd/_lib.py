@@ -145,6 +145,8 @@ SIGNATURES = {
     "help_gpu_backend_handle": (_P, [C.POINTER(Backend)]),
     "help_gpu_backend_kernel_stats": (C.c_int, [C.POINTER(Backend), C.c_int32, _D,
                                                 C.POINTER(C.c_int64), _D]),
+    "help_gpu_backend_kernel_stats_prec": (C.c_int, [C.POINTER(Backend), C.c_int32, C.c_int32, _D,
+                                                     C.POINTER(C.c_int64), _D]),
     "help_gpu_backend_solve_stats": (C.c_int, [C.POINTER(Backend), C.POINTER(C.c_int64),
                                                C.POINTER(C.c_int64), _D]),
     "help_gpu_backend_solve2": (C.c_int, [C.POINTER(Backend), _D, _D, _D, _D, _D, _D, _D, _D, _D]),

@@ -58,7 +58,7 @@ struct ProfScope {
         hipEvent_t a = h->evpool[used];
         b = h->evpool[used + 1];
         idx = h->pending.size();
-        h->pending.push_back({kind, work, a, b});
+        h->pending.push_back({kind, work, a, b, h->prof_fp32});
         // the launches of the region take a and b as launch events
         help::g_region_events = {a, b, false};
     }
@@ -82,9 +82,12 @@ void collect_prof(help_gpu* h) {
     for (auto& p : h->pending) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-            h->kstat[p.kind].ms += ms;
-            h->kstat[p.kind].work += p.work;
-            h->kstat[p.kind].launches += 1;
+            for (help_gpu::KStat* k : {&h->kstat[p.kind], p.fp32 ? &h->kstat32[p.kind] : nullptr}) {
+                if (!k) continue;
+                k->ms += ms;
+                k->work += p.work;
+                k->launches += 1;
+            }
         }
     }
     h->pending.clear();
@@ -713,6 +716,7 @@ T* xinv_partial(help_gpu* h) {
 template <typename T>
 void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
     hipStream_t st = h->st;
+    h->prof_fp32 = std::is_same<T, float>::value;
     // The inverse blocks of a level need only that level's finished L11 and
     // Dinv blocks, and nothing on the main stream reads them before the
     // solves: each run of XINV launches goes to the side stream behind a
@@ -765,6 +769,7 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
         HIPCHK(hipEventRecord(h->ev_side[1], h->st_side));
         HIPCHK(hipStreamWaitEvent(st, h->ev_side[1], 0));
     }
+    h->prof_fp32 = false;
     HIPCHK(hipGetLastError());
 }
 
@@ -775,6 +780,7 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
 template <typename T>
 void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x, int nrhs = 1) {
     hipStream_t st = h->st;
+    h->prof_fp32 = std::is_same<T, float>::value;
     const T* X = xinv_store<T>(h);
     double* y = h->d_y.p;
     RhsStride rs;
@@ -796,6 +802,7 @@ void run_trisolve(help_gpu* h, const T* L, const T* Dinv, double* x, int nrhs = 
         else
             launch_bwd_gemv<T>(h->d_bgemv.p + o.off, o.count, h->sym, L, x, h->d_t.p, nrhs, rs, st);
     }
+    h->prof_fp32 = false;
     HIPCHK(hipGetLastError());
 }
 
@@ -1541,6 +1548,7 @@ int help_gpu_profile(help_gpu* h, int32_t enable) {
     if (!h) return HELP_ERR_ARG;
     h->prof = enable != 0;
     for (auto& k : h->kstat) k = help_gpu::KStat();
+    for (auto& k : h->kstat32) k = help_gpu::KStat();
     h->pending.clear();
     return HELP_OK;
 }
@@ -1609,7 +1617,7 @@ struct GpuBackendCtx {
     help_opts opts;
     help_gpu* h = nullptr;
     int32_t last_status = 0;
-    help_gpu::KStat kstat[HELP_K_COUNT];
+    help_gpu::KStat kstat[HELP_K_COUNT], kstat32[HELP_K_COUNT];
     int64_t n_solve = 0, refine_steps = 0;
     double max_rel_residual = 0;
     int64_t pcg_iters = 0;
@@ -1622,6 +1630,9 @@ static void gb_release(GpuBackendCtx* c) {
         c->kstat[k].ms += c->h->kstat[k].ms;
         c->kstat[k].work += c->h->kstat[k].work;
         c->kstat[k].launches += c->h->kstat[k].launches;
+        c->kstat32[k].ms += c->h->kstat32[k].ms;
+        c->kstat32[k].work += c->h->kstat32[k].work;
+        c->kstat32[k].launches += c->h->kstat32[k].launches;
     }
     help_gpu_destroy(c->h);
     c->h = nullptr;
@@ -1724,11 +1735,20 @@ int help_gpu_backend_solve_stats(const help_backend* be, int64_t* n_solve, int64
 
 int help_gpu_backend_kernel_stats(const help_backend* be, int32_t kind, double* ms,
                                   int64_t* launches, double* work) {
+    return help_gpu_backend_kernel_stats_prec(be, kind, 0, ms, launches, work);
+}
+
+int help_gpu_backend_kernel_stats_prec(const help_backend* be, int32_t kind, int32_t fp32_only, double* ms,
+                                       int64_t* launches, double* work) {
     if (!be || !be->ctx || kind < 0 || kind >= HELP_K_COUNT) return HELP_ERR_ARG;
     auto* c = (GpuBackendCtx*)be->ctx;
-    double t = c->kstat[kind].ms, w = c->kstat[kind].work;
-    int64_t n = c->kstat[kind].launches;
-    if (c->h) { t += c->h->kstat[kind].ms; w += c->h->kstat[kind].work; n += c->h->kstat[kind].launches; }
+    const help_gpu::KStat* own = fp32_only ? c->kstat32 : c->kstat;
+    double t = own[kind].ms, w = own[kind].work;
+    int64_t n = own[kind].launches;
+    if (c->h) {
+        const help_gpu::KStat* live = fp32_only ? c->h->kstat32 : c->h->kstat;
+        t += live[kind].ms; w += live[kind].work; n += live[kind].launches;
+    }
     if (ms) *ms = t;
     if (launches) *launches = n;
     if (work) *work = w;

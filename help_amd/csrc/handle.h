@@ -188,7 +188,9 @@ struct help_gpu {
     // profiling
     bool prof = false;
     struct KStat { double ms = 0, work = 0; int64_t launches = 0; } kstat[HELP_K_COUNT];
-    struct Pending { int32_t kind; double work; hipEvent_t a, b; };
+    KStat kstat32[HELP_K_COUNT];   // the part of kstat run on the FP32 factor
+    bool prof_fp32 = false;        // the launches being recorded use the FP32 factor
+    struct Pending { int32_t kind; double work; hipEvent_t a, b; bool fp32; };
     std::vector<hipEvent_t> evpool;
     std::vector<Pending> pending;
     double work_adat = 0, work_rhs = 0, work_backsub = 0;

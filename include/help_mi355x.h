@@ -422,6 +422,10 @@ help_gpu* help_gpu_backend_handle(const help_backend* be);
  * opts.profile = 1), plus its live handle. */
 int help_gpu_backend_kernel_stats(const help_backend* be, int32_t kind, double* ms,
                                   int64_t* launches, double* work);
+/* The same with fp32_only != 0 restricted to the launches that ran on the
+ * FP32 factor (factorization and triangular solves of the FP32 mode). */
+int help_gpu_backend_kernel_stats_prec(const help_backend* be, int32_t kind, int32_t fp32_only,
+                                       double* ms, int64_t* launches, double* work);
 /* Solve accounting over the backend's lifetime: solves, FP64 refinement
  * (factor-preconditioned CG) iterations, and the largest final relative
  * residual ||r - (A D^-1 A^T + beta I) x2|| / ||r||. */
