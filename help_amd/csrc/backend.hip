@@ -14,6 +14,8 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <exception>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -93,13 +95,17 @@ void collect_prof(help_gpu* h) {
     h->pending.clear();
 }
 
-// lower-triangle entries (i >= j) of a rows x cols tile at (r0, c0) below row/col limits
+// lower-triangle entries (i >= j) of a 64 x 64 tile at (r0, c0) below
+// row/col limits: sum over its columns j of the rows max(r0, j) .. hi-1
 double tile_lower_entries(int32_t r0, int32_t c0, int32_t rlim, int32_t clim) {
+    const int64_t hi = std::min(r0 + kTile, rlim), jend = std::min(c0 + kTile, clim);
     double cnt = 0;
-    for (int32_t j = c0; j < std::min(c0 + kTile, clim); ++j) {
-        int32_t lo = std::max(r0, j), hi = std::min(r0 + kTile, rlim);
-        if (hi > lo) cnt += hi - lo;
-    }
+    // columns left of r0 see the whole row range
+    const int64_t jl = std::min<int64_t>(jend, r0);
+    if (jl > c0 && hi > r0) cnt += (double)(jl - c0) * (double)(hi - r0);
+    // columns j >= r0 see hi - j rows while j < hi
+    const int64_t ja = std::max<int64_t>(c0, r0), jb = std::min<int64_t>(jend, hi);
+    if (jb > ja) cnt += (double)(jb - ja) * (double)(2 * hi - ja - jb + 1) / 2.0;
     return cnt;
 }
 
@@ -302,26 +308,48 @@ void build_schedules(help_gpu* h) {
         // gets the tasks of its update-matrix columns for that alone)
         int32_t off = (int32_t)h->ea.size();
         double w = 0;
+        std::vector<int32_t> chunk_cnt, chunk_fill;
         for (int32_t s : at_start[p]) {
             const int32_t c = S.snrow[s], nc = ncol(s);
             const bool leaf = S.childptr[s + 1] == S.childptr[s];
             if (leaf && c == nc) continue;
             w += ea_bytes(s) + 4.0 * (c - nc) * (c - nc + 1);
-            for (int32_t c0 = leaf ? nc / kEaCols * kEaCols : 0; c0 < c; c0 += kEaCols) {
-                const int32_t c1 = std::min(c, c0 + kEaCols);
-                EaTask tk{s, c0, c1, (int32_t)h->earef.size(), 0, nc, c, S.loff[s], h->uplan[s]};
-                // the children whose update columns land in [c0, c1) (the
-                // relative map of a child is ascending), in child order
-                for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
-                    const int32_t ch = S.child[t];
-                    const int32_t chc = S.snrow[ch] - ncol(ch);
-                    const int32_t* rel = S.relmap.data() + S.reloff[ch];
-                    const int32_t lo = (int32_t)(std::lower_bound(rel, rel + chc, c0) - rel);
-                    const int32_t hi = (int32_t)(std::lower_bound(rel, rel + chc, c1) - rel);
-                    if (lo < hi) h->earef.push_back({h->uplan[ch], S.reloff[ch], chc, lo, hi});
+            // each child's relative map is ascending: one walk over it cuts it
+            // into the kEaCols-column chunks it touches, and the chunks' child
+            // lists are filled child by child (child order within a chunk)
+            const int32_t nchunk = (c + kEaCols - 1) / kEaCols;
+            chunk_cnt.assign(nchunk + 1, 0);
+            for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
+                const int32_t ch = S.child[t];
+                const int32_t chc = S.snrow[ch] - ncol(ch);
+                const int32_t* rel = S.relmap.data() + S.reloff[ch];
+                for (int32_t j = 0; j < chc;) {
+                    const int32_t k = rel[j] / kEaCols;
+                    while (j < chc && rel[j] / kEaCols == k) ++j;
+                    chunk_cnt[k + 1]++;
                 }
-                tk.nref = (int32_t)h->earef.size() - tk.r0;
-                h->ea.push_back(tk);
+            }
+            for (int32_t k = 0; k < nchunk; ++k) chunk_cnt[k + 1] += chunk_cnt[k];
+            const int32_t base = (int32_t)h->earef.size();
+            h->earef.resize(base + chunk_cnt[nchunk]);
+            chunk_fill.assign(chunk_cnt.begin(), chunk_cnt.end() - 1);
+            for (int32_t t = S.childptr[s]; t < S.childptr[s + 1]; ++t) {
+                const int32_t ch = S.child[t];
+                const int32_t chc = S.snrow[ch] - ncol(ch);
+                const int32_t* rel = S.relmap.data() + S.reloff[ch];
+                for (int32_t j = 0; j < chc;) {
+                    const int32_t k = rel[j] / kEaCols, lo = j;
+                    while (j < chc && rel[j] / kEaCols == k) ++j;
+                    h->earef[base + chunk_fill[k]++] = {h->uplan[ch], S.reloff[ch], chc, lo, j};
+                }
+            }
+            // a task per chunk with children, and per update-matrix chunk
+            // (its columns are zeroed there)
+            for (int32_t k = leaf ? nc / kEaCols : 0; k < nchunk; ++k) {
+                const int32_t c0 = k * kEaCols, c1 = std::min(c, c0 + kEaCols);
+                const int32_t nref = chunk_cnt[k + 1] - chunk_cnt[k];
+                if (nref == 0 && c1 <= nc) continue;
+                h->ea.push_back({s, c0, c1, base + chunk_cnt[k], nref, nc, c, S.loff[s], h->uplan[s]});
             }
         }
         if ((int32_t)h->ea.size() > off) h->fops.push_back({OP_EA, off, (int32_t)h->ea.size() - off, w});
@@ -1079,8 +1107,19 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
             h->work_rhs = 12.0 * nnz_d + 16.0 * nd + 16.0 * md;
             h->work_backsub = 12.0 * nnz_d + 8.0 * (nd + 1) + 8.0 * md + 24.0 * nd;
         } else {
-            h->P = adat_pattern(h->A, h->R, true);
-            phase("adat pattern");
+            // The ADA^T pattern with its contribution pairs is needed only by
+            // the assembly maps: it is built on a thread of its own while the
+            // ordering and the symbolic analysis run (the ordering reads the
+            // adjacency, which comes from A directly).  The schedules, which
+            // need only the analysis, are built beside the assembly maps.
+            std::exception_ptr side_err;
+            std::thread side([&] {
+                try { h->P = adat_pattern(h->A, h->R, true); } catch (...) { side_err = std::current_exception(); }
+            });
+            struct Joiner {
+                std::thread& t;
+                ~Joiner() { if (t.joinable()) t.join(); }
+            } join_side{side};
             std::vector<int64_t> adjptr;
             std::vector<int32_t> adj;
             adat_adjacency(h->A, h->R, adjptr, adj);
@@ -1092,10 +1131,18 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
             phase("ordering (ND)");
             h->S = analyze(m, adjptr, adj, perm, rp);
             phase("analyze");
+            side.join();
+            if (side_err) std::rethrow_exception(side_err);
+            phase("adat pattern (rest)");
+            std::thread sched([&] {
+                try { build_schedules(h); } catch (...) { side_err = std::current_exception(); }
+            });
+            struct Joiner join_sched{sched};
             build_assembly_maps(h);
             phase("assembly maps");
-            build_schedules(h);
-            phase("schedules");
+            sched.join();
+            if (side_err) std::rethrow_exception(side_err);
+            phase("schedules (rest)");
             if (h->use_fp32) {
                 h->slot_row.resize(h->P.colind.size());
                 for (int32_t i = 0; i < m; ++i)

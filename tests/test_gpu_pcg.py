@@ -187,3 +187,33 @@ def test_pcg_cap_reports_noconv():
         _flush(f"  hsd device={device}: status {res.status} stall {res.stall_reason} "
                f"backend_error {res.backend_error} iterations {res.iterations}")
         assert res.status == 1 and res.stall_reason == 3 and res.backend_error == L.HELP_ERR_NOCONV
+
+
+def test_pcg_full_size_zipf_residuals():
+    """The Zipf LP at its full size (m=10^6, n=4*10^6, nnz=1.9*10^7) in PCG
+    mode with K=500: the backend's solution of A D^-1 A^T x2 = r (the PCG
+    mode's operator has no diagonal shift, as PCG.cpp's), r = A D^-1 b1 - b2,
+    checked by an independent CPU product (scipy) against the stop tolerance
+    (the recursive CG residual and the true one differ by rounding), at d = 1
+    and at a log-uniform d over +-1 decade, plus x1 = D^-1 (b1 - A^T x2).  The
+    oracle's partial Cholesky takes too long at this size; its parity is
+    pinned on the smaller instances above."""
+    inst = instances.by_config(3)
+    A = inst.to_scipy().tocsr()
+    At = A.T.tocsr()
+    rng = np.random.default_rng(5)
+    be = LinearEquation(mode=L.MODE_PCG, pcg_k=500, pcg_tol=1e-10, pcg_max_iter=5000)
+    be.construct(inst)
+    for name, d in (("d=1", np.ones(inst.n)), ("d log-uniform +-1", np.exp(rng.uniform(-1, 1, inst.n) * np.log(10.0)))):
+        b1, b2 = rng.standard_normal(inst.n), rng.standard_normal(inst.m)
+        be.renew(d)
+        x1, x2 = be.solve(d, b1, b2)
+        r = A @ (b1 / d) - b2
+        res = r - A @ ((At @ x2) / d)
+        rel = np.linalg.norm(res) / np.linalg.norm(r)
+        bs = np.linalg.norm(x1 - (b1 - At @ x2) / d) / np.linalg.norm(x1)
+        _flush(f"zipf m={inst.m} {name}: {be.solve_stats.pcg_iters} CG iterations, relative residual {rel:.2e} "
+               f"(library {be.solve_stats.rel_residual:.2e}), x1 back-substitution {bs:.1e}")
+        assert rel <= 1e-9
+        assert bs <= 1e-12
+    be.destruct()
