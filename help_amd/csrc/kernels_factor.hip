@@ -928,8 +928,13 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
                                               T* __restrict__ L, T* __restrict__ U, T* __restrict__ Uacc) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int kPer = kKc / 4;
-    __shared__ T As[2][kKc * kTile];
-    __shared__ T Bs[2][kKc * kTile];
+    // the double-buffered k chunks, and after the product the 64x64
+    // accumulator tile (column-major, leading dimension kTile + 1)
+    constexpr int kCLd = kTile + 1;
+    constexpr int kSmem = 4 * kKc * kTile > kTile * kCLd ? 4 * kKc * kTile : kTile * kCLd;
+    __shared__ T smem[kSmem];
+    T (*As)[kKc * kTile] = reinterpret_cast<T (*)[kKc * kTile]>(smem);
+    T (*Bs)[kKc * kTile] = reinterpret_cast<T (*)[kKc * kTile]>(smem + 2 * kKc * kTile);
     const GemmTask tk = tasks[blockIdx.x];
     const int32_t n = tk.n;
     const int32_t c = tk.c;
@@ -1001,17 +1006,16 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
                 }
         return;
     }
-    // read-modify-write of C: every load is issued before the first store.
-    // A task's tile lies in L's columns (col_end <= n) or in the update
-    // matrix (col0 >= n), so one tile origin (uniform over the workgroup)
-    // and 32-bit element offsets from it address every entry; entries
+    // read-modify-write of C.  The accumulator tile goes through LDS so
+    // that the lanes run along the rows of one column per access (64
+    // consecutive entries per load or store); each thread takes one row of
+    // 16 columns and issues its 16 loads before its stores.  A task's tile
+    // lies in L's columns (col_end <= n) or in the update matrix
+    // (col0 >= n), so one tile origin and 32-bit offsets address it; entries
     // outside the lower part or the front read the origin and are not
     // written back.
-    const bool inU = tk.col0 >= n;
-    const int32_t ld = inU ? cu : c;
-    T* Ct = inU ? Up + (int64_t)(tk.col0 - n) * cu + (tk.row0 - n) : Lw + (int64_t)tk.col0 * c + tk.row0;
-    T cv[2][2][4];
-    int32_t co[2][2][4];
+    __syncthreads();   // every wave is done with the k chunks
+    T* Cs = smem;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -1020,18 +1024,25 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
             for (int r = 0; r < 4; ++r) {
                 const int32_t ti = wr + a * 16 + Mfma<T>::row(lane, r);
                 const int32_t tj = wc + b * 16 + (lane & 15);
-                const int32_t i = tk.row0 + ti, j = tk.col0 + tj;
-                const bool ok = i < c && j < tk.col_end && i >= j;
-                co[a][b][r] = ok ? tj * ld + ti : -1;
-                cv[a][b][r] = Ct[ok ? tj * ld + ti : 0];
+                Cs[tj * kCLd + ti] = acc[a][b][r];
             }
+    __syncthreads();
+    const bool inU = tk.col0 >= n;
+    const int32_t ld = inU ? cu : c;
+    T* Ct = inU ? Up + (int64_t)(tk.col0 - n) * cu + (tk.row0 - n) : Lw + (int64_t)tk.col0 * c + tk.row0;
+    const int32_t ti = tid & 63, tj0 = (tid >> 6) * 16;
+    const int32_t i = tk.row0 + ti;
+    T cv[16];
+    bool okv[16];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int q = 0; q < 16; ++q) {
+        const int32_t tj = tj0 + q, j = tk.col0 + tj;
+        okv[q] = i < c && j < tk.col_end && i >= j;
+        cv[q] = Ct[okv[q] ? tj * ld + ti : 0];
+    }
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (co[a][b][r] >= 0) Ct[co[a][b][r]] = cv[a][b][r] - acc[a][b][r];
+    for (int q = 0; q < 16; ++q)
+        if (okv[q]) Ct[(tj0 + q) * ld + ti] = cv[q] - Cs[(tj0 + q) * kCLd + ti];
 }
 
 // X := X L_bb^-T for the 64 rows of block rb-1 below the diagonal block,
