@@ -932,7 +932,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
     // accumulator tile (column-major, leading dimension kTile + 1)
     constexpr int kCLd = kTile + 1;
     constexpr int kSmem = 4 * kKc * kTile > kTile * kCLd ? 4 * kKc * kTile : kTile * kCLd;
-    __shared__ T smem[kSmem];
+    __shared__ __attribute__((aligned(16))) T smem[kSmem];
     T (*As)[kKc * kTile] = reinterpret_cast<T (*)[kKc * kTile]>(smem);
     T (*Bs)[kKc * kTile] = reinterpret_cast<T (*)[kKc * kTile]>(smem + 2 * kKc * kTile);
     const GemmTask tk = tasks[blockIdx.x];
@@ -964,33 +964,74 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
                     acc[a][b][r] = Ua[ok ? (int64_t)(j - n) * cu + (i - n) : 0];
                 }
     }
-    const int li = tid & 63, lk = tid >> 6;
-    const int32_t ia = tk.row0 + li, ib = tk.col0 + li;
-    const bool ain = ia < c, bin = ib < c;
-    const T* pa = Lp + min(ia, c - 1);
-    const T* pb = Lp + min(ib, c - 1);
-    T ra[kPer], rb[kPer];
-    auto load = [&](int32_t kb) {
+    if constexpr (sizeof(T) == 4) {
+        // FP32: each thread moves 4 consecutive rows of one column of the
+        // chunk with one 16-byte load (4-byte aligned: the column stride c is
+        // arbitrary, and dword alignment is what multi-dword global loads
+        // need); rows past the front read zeros through clamped loads.  The
+        // LDS chunks hold the same values as the scalar form below.
+        typedef float vload_t __attribute__((ext_vector_type(4), aligned(4)));
+        const int vr = (tid & 15) * 4, vk = tid >> 4;   // 16 columns, 64 rows
+        const int32_t ia = tk.row0 + vr, ib = tk.col0 + vr;
+        const bool afull = ia + 4 <= c, bfull = ib + 4 <= c;
+        float4_t ra, rb;
+        auto rows4 = [&](const T* col, int32_t i0, bool full) {
+            if (full) {
+                const vload_t v = *reinterpret_cast<const vload_t*>(col + i0);
+                return float4_t{v[0], v[1], v[2], v[3]};
+            }
+            float4_t v;
 #pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const int32_t kg = min(kb + lk + 4 * r, tk.k1 - 1);
-            ra[r] = pa[(int64_t)kg * c];
-            rb[r] = pb[(int64_t)kg * c];
+            for (int e = 0; e < 4; ++e) {
+                const T x = col[min(i0 + e, c - 1)];
+                v[e] = (i0 + e < c) ? x : T(0);
+            }
+            return v;
+        };
+        auto load = [&](int32_t kb) {
+            const T* col = Lp + (int64_t)min(kb + vk, tk.k1 - 1) * c;
+            ra = rows4(col, ia, afull);
+            rb = rows4(col, ib, bfull);
+        };
+        load(tk.k0);
+        int buf = 0;
+        for (int32_t kb = tk.k0; kb < tk.k1; kb += kKc, buf ^= 1) {
+            const bool kin = kb + vk < tk.k1;
+            *reinterpret_cast<float4_t*>(&As[buf][vk * kTile + vr]) = kin ? ra : float4_t{0, 0, 0, 0};
+            *reinterpret_cast<float4_t*>(&Bs[buf][vk * kTile + vr]) = kin ? rb : float4_t{0, 0, 0, 0};
+            __syncthreads();
+            if (kb + kKc < tk.k1) load(kb + kKc);
+            mfma_tile<T>(As[buf], Bs[buf], kKc, lane, wr, wc, acc);
         }
-    };
-    load(tk.k0);
-    int buf = 0;
-    for (int32_t kb = tk.k0; kb < tk.k1; kb += kKc, buf ^= 1) {
+    } else {
+        const int li = tid & 63, lk = tid >> 6;
+        const int32_t ia = tk.row0 + li, ib = tk.col0 + li;
+        const bool ain = ia < c, bin = ib < c;
+        const T* pa = Lp + min(ia, c - 1);
+        const T* pb = Lp + min(ib, c - 1);
+        T ra[kPer], rb[kPer];
+        auto load = [&](int32_t kb) {
 #pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const int k = lk + 4 * r;
-            const bool kin = kb + k < tk.k1;
-            As[buf][k * kTile + li] = (kin && ain) ? ra[r] : T(0);
-            Bs[buf][k * kTile + li] = (kin && bin) ? rb[r] : T(0);
+            for (int r = 0; r < kPer; ++r) {
+                const int32_t kg = min(kb + lk + 4 * r, tk.k1 - 1);
+                ra[r] = pa[(int64_t)kg * c];
+                rb[r] = pb[(int64_t)kg * c];
+            }
+        };
+        load(tk.k0);
+        int buf = 0;
+        for (int32_t kb = tk.k0; kb < tk.k1; kb += kKc, buf ^= 1) {
+#pragma unroll
+            for (int r = 0; r < kPer; ++r) {
+                const int k = lk + 4 * r;
+                const bool kin = kb + k < tk.k1;
+                As[buf][k * kTile + li] = (kin && ain) ? ra[r] : T(0);
+                Bs[buf][k * kTile + li] = (kin && bin) ? rb[r] : T(0);
+            }
+            __syncthreads();
+            if (kb + kKc < tk.k1) load(kb + kKc);
+            mfma_tile<T>(As[buf], Bs[buf], kKc, lane, wr, wc, acc);
         }
-        __syncthreads();
-        if (kb + kKc < tk.k1) load(kb + kKc);
-        mfma_tile<T>(As[buf], Bs[buf], kKc, lane, wr, wc, acc);
     }
     if (tk.mode == kGemmFirst || tk.mode == kGemmNext) {
         // park the accumulator for the next panel
