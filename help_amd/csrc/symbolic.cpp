@@ -159,15 +159,30 @@ void adat_adjacency(const CscMatrix& A, const CsrMatrix& R, std::vector<int64_t>
 
 void permute_graph(int32_t m, const std::vector<int64_t>& adjptr, const std::vector<int32_t>& adj,
                    const std::vector<int32_t>& perm, const std::vector<int32_t>& iperm,
-                   std::vector<int64_t>& pp, std::vector<int32_t>& pj) {
+                   std::vector<int64_t>& pp, std::vector<int32_t>& pj, int half) {
     pp.assign(m + 1, 0);
-    for (int32_t k = 0; k < m; ++k) pp[k + 1] = pp[k] + (adjptr[perm[k] + 1] - adjptr[perm[k]]);
+    auto keep = [&](int32_t i, int32_t k) { return half == 0 || (half < 0 ? i < k : i > k); };
+    if (half == 0) {
+        for (int32_t k = 0; k < m; ++k) pp[k + 1] = pp[k] + (adjptr[perm[k] + 1] - adjptr[perm[k]]);
+    } else {
+#pragma omp parallel for schedule(dynamic, 512)
+        for (int32_t k = 0; k < m; ++k) {
+            const int32_t old = perm[k];
+            int64_t cnt = 0;
+            for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) cnt += keep(iperm[adj[t]], k) ? 1 : 0;
+            pp[k + 1] = cnt;
+        }
+        for (int32_t k = 0; k < m; ++k) pp[k + 1] += pp[k];
+    }
     pj.resize(pp[m]);
 #pragma omp parallel for schedule(dynamic, 512)
     for (int32_t k = 0; k < m; ++k) {
         const int32_t old = perm[k];
         int64_t o = pp[k];
-        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) pj[o++] = iperm[adj[t]];
+        for (int64_t t = adjptr[old]; t < adjptr[old + 1]; ++t) {
+            const int32_t i = iperm[adj[t]];
+            if (keep(i, k)) pj[o++] = i;
+        }
     }
 }
 
@@ -737,7 +752,7 @@ Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vec
     // then read it in order)
     std::vector<int64_t> pp;
     std::vector<int32_t> pj;
-    permute_graph(m, adjptr, adj, perm, iperm, pp, pj);
+    permute_graph(m, adjptr, adj, perm, iperm, pp, pj, -1);
     std::vector<int32_t> par = etree(m, pp, pj);
     std::vector<int32_t> post = postorder(par);
     std::vector<int32_t> perm2(m), ipost(m);
@@ -747,7 +762,7 @@ Supernodal analyze(int32_t m, const std::vector<int64_t>& adjptr, const std::vec
     std::vector<int32_t> par2(m);
     for (int32_t k = 0; k < m; ++k) par2[k] = par[post[k]] >= 0 ? ipost[par[post[k]]] : -1;
     par.swap(par2);
-    permute_graph(m, adjptr, adj, perm, iperm, pp, pj);
+    permute_graph(m, adjptr, adj, perm, iperm, pp, pj, 1);
     std::vector<int32_t> cc = colcounts(m, pp, pj, par);
     S.perm = perm; S.iperm = iperm; S.parent = par; S.colcount = cc;
     for (int32_t j = 0; j < m; ++j) {

@@ -83,10 +83,12 @@ void adat_adjacency(const CscMatrix& A, const CsrMatrix& R, std::vector<int64_t>
                     std::vector<int32_t>& adj);
 
 // The adjacency relabelled by perm (new -> old): row k holds iperm[] of the
-// neighbours of perm[k], in their order there.
+// neighbours of perm[k], in their order there; half < 0 keeps only the
+// neighbours numbered below k (what the elimination tree reads), half > 0
+// only those above k (what the column counts read).
 void permute_graph(int32_t m, const std::vector<int64_t>& adjptr, const std::vector<int32_t>& adj,
                    const std::vector<int32_t>& perm, const std::vector<int32_t>& iperm,
-                   std::vector<int64_t>& pp, std::vector<int32_t>& pj);
+                   std::vector<int64_t>& pp, std::vector<int32_t>& pj, int half = 0);
 
 // Elimination tree of P M P^T (Liu's algorithm with path compression), from
 // the permuted adjacency (pp, pj).
