@@ -495,9 +495,12 @@ struct NdWork {
         if ((int32_t)set.size() <= leaf) { if (!set.empty()) out.push_back(std::move(set)); return; }
         const int32_t tag = new_tag();
         for (int32_t v : set) st(v, tag);
-        // connected components, each dissected on its own
+        // the first level structure, from set[0]; when it misses vertices the
+        // subgraph is disconnected and its components are dissected on their
+        // own (the first component is this BFS's vertex set, in this order)
         std::vector<int32_t> visit;
-        {
+        int32_t depth = bfs(set[0], tag, visit);
+        if (visit.size() < set.size()) {
             NdJobs comps;
             size_t covered = 0;
             const int32_t seen = new_tag();
@@ -515,15 +518,12 @@ struct NdWork {
                 comps.push_back(visit);
                 if (covered == set.size()) break;
             }
-            if (comps.size() > 1) {
-                for (auto& c : comps) dissect(std::move(c), out);
-                return;
-            }
-            for (int32_t v : set) st(v, tag);
+            for (auto& c : comps) dissect(std::move(c), out);
+            return;
         }
         // pseudo-peripheral root: repeat BFS from a minimum-degree vertex of
         // the last level while the depth grows (at most 4 sweeps)
-        int32_t depth = bfs(set[0], tag, visit);
+        std::vector<int32_t> lsave;
         for (int it = 0; it < 4; ++it) {
             int32_t best = -1;
             int64_t bdeg = INT64_MAX;
@@ -532,10 +532,12 @@ struct NdWork {
                     const int64_t dg = ptr[v + 1] - ptr[v];
                     if (dg < bdeg) { bdeg = dg; best = v; }
                 }
+            lsave.resize(visit.size());
+            for (size_t q = 0; q < visit.size(); ++q) lsave[q] = level[visit[q]];
             std::vector<int32_t> vis2;
             const int32_t d2 = bfs(best, tag, vis2);
             if (d2 < depth) {   // the previous level structure stays
-                bfs(visit[0], tag, visit);
+                for (size_t q = 0; q < visit.size(); ++q) level[visit[q]] = lsave[q];
                 break;
             }
             const bool grew = d2 > depth;
