@@ -352,7 +352,11 @@ void build_schedules(help_gpu* h) {
                 h->ea.push_back({s, c0, c1, base + chunk_cnt[k], nref, nc, c, S.loff[s], h->uplan[s]});
             }
         }
-        if ((int32_t)h->ea.size() > off) h->fops.push_back({OP_EA, off, (int32_t)h->ea.size() - off, w});
+        if ((int32_t)h->ea.size() > off) {
+            Op oe{OP_EA, off, (int32_t)h->ea.size() - off, w};
+            for (int32_t s : at_start[p]) if (S.snrow[s] >= kEaDeepRows) oe.depth = 8;
+            h->fops.push_back(oe);
+        }
         // the panel of every active front
         int32_t po = (int32_t)h->potrf.size(), to = (int32_t)h->trsm.size(), go = (int32_t)h->gemm.size();
         const int32_t pa = (int32_t)h->panel.size();
@@ -770,7 +774,7 @@ void run_factor(help_gpu* h, T* L, T* U, T* Dinv) {
                 else
                     launch_small_fronts<T>(h->d_list_small.p + o.off, o.count, h->sym, L, U, Dinv, h->d_err.p, o.lds, st);
                 break;
-            case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->d_earef.p, h->sym, L, U, st); break;
+            case OP_EA: launch_extend_add<T>(h->d_ea.p + o.off, o.count, h->d_earef.p, h->sym, L, U, o.depth, st); break;
             case OP_POTRF:
                 if constexpr (std::is_same<T, double>::value) {
                     if (fused) {

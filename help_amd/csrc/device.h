@@ -155,7 +155,8 @@ void launch_small_wave(const int32_t* list, int32_t count, DevSym sym, T* L, T* 
 constexpr int kSmallWaveLds = 2048;   // LDS elements per front of the one-wave form
 template <typename T>
 void launch_extend_add(const EaTask* tasks, int32_t count, const EaRef* refs, DevSym sym, T* L, T* U,
-                       hipStream_t st);
+                       int32_t depth, hipStream_t st);
+constexpr int kEaDeepRows = 2048;   // launches with a front this tall take 8 rows per lane in flight
 // factor the 64x64 diagonal block at column b and store its inverse
 template <typename T>
 void launch_potrf_diag(const PanelTask* tasks, int32_t count, DevSym sym, T* L, T* Dinv,

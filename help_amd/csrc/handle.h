@@ -73,6 +73,7 @@ struct Op {
                        // small fronts (0 = workgroup kernel) and the tiny supernodes
                        // of a solve level
     bool fused = false;  // TRSM: the phase's panels ran in the fused POTRF launch
+    int32_t depth = 4;   // extend-add: child rows per lane in flight (4, or 8 for large fronts)
 };
 
 inline const char* const kKindName[HELP_K_COUNT] = {

@@ -353,9 +353,10 @@ __global__ __launch_bounds__(256) void k_small_wave(const int32_t* __restrict__ 
 // Task = kEaCols columns [c0, c1) of a front.  The update-matrix columns of
 // the range are first set to zero (lower part), then for each child in order
 // the child columns that land in the range are spread over the 4 waves (two
-// columns per wave), the lanes take the rows, 4 rows per lane in flight.
+// columns per wave), the lanes take the rows, kDepth rows per lane in flight
+// (8 in the launches of large fronts, whose children have long columns).
 // Children with no column in the range are skipped without a barrier.
-template <typename T>
+template <typename T, int kDepth>
 __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ tasks,
                                                     const EaRef* __restrict__ refs, DevSym sym,
                                                     T* __restrict__ L, T* __restrict__ U) {
@@ -390,11 +391,11 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
             T* dstA = (ccA < n) ? Lp + (int64_t)ccA * c : Up + (int64_t)(ccA - n) * cu - n;
             T* dstB = (ccB < n) ? Lp + (int64_t)ccB * c : Up + (int64_t)(ccB - n) * cu - n;
             const int32_t endA = chc, endB = hasB ? chc : jB;   // B empty when absent
-            for (int32_t i0 = lane; jA + i0 < endA || jB + i0 < endB; i0 += 256) {
-                int32_t ra[4], rb[4];
-                T va[4], vb[4];
+            for (int32_t i0 = lane; jA + i0 < endA || jB + i0 < endB; i0 += 64 * kDepth) {
+                int32_t ra[kDepth], rb[kDepth];
+                T va[kDepth], vb[kDepth];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kDepth; ++u) {
                     const int32_t ia = jA + i0 + 64 * u, ib = jB + i0 + 64 * u;
                     const bool oka = ia < endA, okb = ib < endB;
                     const int32_t ca = oka ? ia : jA, cb = okb ? ib : jB;
@@ -405,14 +406,14 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask* __restrict__ t
                     va[u] = x1;
                     vb[u] = x2;
                 }
-                T da[4], db[4];
+                T da[kDepth], db[kDepth];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kDepth; ++u) {
                     da[u] = dstA[ra[u] >= 0 ? ra[u] : ccA];
                     db[u] = dstB[rb[u] >= 0 ? rb[u] : ccB];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kDepth; ++u) {
                     if (ra[u] >= 0) dstA[ra[u]] = da[u] + va[u];
                     if (rb[u] >= 0) dstB[rb[u]] = db[u] + vb[u];
                 }
@@ -1310,8 +1311,10 @@ void launch_small_wave(const int32_t* list, int32_t count, DevSym sym, T* L, T* 
 }
 template <typename T>
 void launch_extend_add(const EaTask* tasks, int32_t count, const EaRef* refs, DevSym sym, T* L, T* U,
-                       hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_extend_add<T>, count, 256, 0, st, tasks, refs, sym, L, U);
+                       int32_t depth, hipStream_t st) {
+    if (count <= 0) return;
+    if (depth == 8) help::launch_kernel(k_extend_add<T, 8>, count, 256, 0, st, tasks, refs, sym, L, U);
+    else help::launch_kernel(k_extend_add<T, 4>, count, 256, 0, st, tasks, refs, sym, L, U);
 }
 template <typename T>
 void launch_gemm(const GemmTask* tasks, int32_t count, DevSym sym, T* L, T* U, T* Uacc, hipStream_t st) {
@@ -1353,7 +1356,7 @@ void launch_xinv(const XTask* tasks, int32_t count, DevSym sym, const T* L, cons
                                          int32_t, hipStream_t);                                     \
     template void launch_small_wave<T>(const int32_t*, int32_t, DevSym, T*, T*, T*, int32_t*,        \
                                        int32_t, int32_t, hipStream_t);                              \
-    template void launch_extend_add<T>(const EaTask*, int32_t, const EaRef*, DevSym, T*, T*, hipStream_t); \
+    template void launch_extend_add<T>(const EaTask*, int32_t, const EaRef*, DevSym, T*, T*, int32_t, hipStream_t); \
     template void launch_gemm<T>(const GemmTask*, int32_t, DevSym, T*, T*, T*, hipStream_t);           \
     template void launch_potrf_diag<T>(const PanelTask*, int32_t, DevSym, T*, T*, int32_t*,        \
                                        hipStream_t);                                                \
