@@ -380,13 +380,20 @@ __global__ __launch_bounds__(256) void k_fwd_tiny(const int32_t* __restrict__ li
     if (idx < count) fwd_tiny<T, G>(list[idx], sym, L, Dinv, b, y, W, xw[g]);
 }
 
+// W(rows of L21) += L21 y_s for a 64-row block of a big supernode's L21.
+// Every row sums its columns as four parts (columns j = part mod 4), each
+// part as four interleaved chains a_q (columns part + 4q + 16t while the
+// group start part + 16t has its 16 columns; the group tail's columns go to
+// a_0 in order), a part as (a_0 + a_1) + (a_2 + a_3) and the row as
+// (part 0 + part 1) + (part 2 + part 3).  One wave per chain (16 waves, the
+// lanes on the rows), so a block keeps 16 chains' loads in flight.
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_gemv(const SolveTask* __restrict__ tasks, DevSym sym,
-                                                  const T* __restrict__ L,
-                                                  const double* __restrict__ y,
-                                                  double* __restrict__ W, RhsStride rs) {
+__global__ __launch_bounds__(1024) void k_fwd_gemv(const SolveTask* __restrict__ tasks, DevSym sym,
+                                                   const T* __restrict__ L,
+                                                   const double* __restrict__ y,
+                                                   double* __restrict__ W, RhsStride rs) {
     __shared__ double xs[kMaxLdsCols];
-    __shared__ double red[4][64];
+    __shared__ double red[16][64];
     y += blockIdx.y * rs.v; W += blockIdx.y * rs.w;
     const SolveTask tk = tasks[blockIdx.x];
     const int32_t s = tk.s;
@@ -398,25 +405,29 @@ __global__ __launch_bounds__(256) void k_fwd_gemv(const SolveTask* __restrict__ 
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int32_t t = tid; t < n; t += nt) xs[t] = y[f + t];
     __syncthreads();
-    // 64 rows, 4 threads per row splitting the columns, fixed combine order
-    const int32_t r = tid & 63, part = tid >> 6;
+    const int32_t r = tid & 63, ch = tid >> 6;   // chain ch = part + 4 q
+    const int32_t part = ch & 3, q = ch >> 2;
     const int32_t i = n + tk.blk * 64 + r;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    double a = 0.0;
     if (i < c) {
         const T* lp = Lp + i;
-        int32_t j = part;
-#pragma unroll 2
-        for (; j + 12 < n; j += 16) {
-            a0 += (double)lp[(int64_t)j * c] * xs[j];
-            a1 += (double)lp[(int64_t)(j + 4) * c] * xs[j + 4];
-            a2 += (double)lp[(int64_t)(j + 8) * c] * xs[j + 8];
-            a3 += (double)lp[(int64_t)(j + 12) * c] * xs[j + 12];
+        int32_t jb = part;
+#pragma unroll 8
+        for (; jb + 12 < n; jb += 16) {
+            const int32_t j = jb + 4 * q;
+            a += (double)lp[(int64_t)j * c] * xs[j];
         }
-        for (; j < n; j += 4) a0 += (double)lp[(int64_t)j * c] * xs[j];
+        if (q == 0)
+            for (; jb < n; jb += 4) a += (double)lp[(int64_t)jb * c] * xs[jb];
     }
-    red[part][r] = (a0 + a1) + (a2 + a3);
+    red[ch][r] = a;
     __syncthreads();
-    if (part == 0 && i < c) Ws[i - n] += (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+    if (ch == 0 && i < c) {
+        double pv[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) pv[p] = (red[p][r] + red[p + 4][r]) + (red[p + 8][r] + red[p + 12][r]);
+        Ws[i - n] += (pv[0] + pv[1]) + (pv[2] + pv[3]);
+    }
 }
 
 // t[f+j] = sum_{i>=n} L21(i,j) x[row_i] for 16 columns j of a big supernode
@@ -713,7 +724,7 @@ void launch_bwd_level(const int32_t* list, int32_t count, int32_t ntiny, const S
 template <typename T>
 void launch_fwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* y,
                      double* W, int nrhs, RhsStride rs, hipStream_t st) {
-    if (count > 0) help::launch_kernel(k_fwd_gemv<T>, dim3(count, nrhs), 256, 0, st, tasks, sym, L, y, W, rs);
+    if (count > 0) help::launch_kernel(k_fwd_gemv<T>, dim3(count, nrhs), 1024, 0, st, tasks, sym, L, y, W, rs);
 }
 template <typename T>
 void launch_bwd_gemv(const SolveTask* tasks, int32_t count, DevSym sym, const T* L, const double* x,
