@@ -42,7 +42,7 @@ class Opts(C.Structure):
                 ("fault_flags", C.c_int32), ("verbose", C.c_int32), ("symbolic_only", C.c_int32),
                 ("front_path", C.c_int32), ("profile", C.c_int32), ("refine_fp64", C.c_int32),
                 ("inverse_solve", C.c_int32), ("ordering", C.c_int32), ("factor_graph", C.c_int32),
-                ("refine_switch", C.c_double)]
+                ("refine_switch", C.c_double), ("ea_depth", C.c_int32)]
 
 
 class SolveStats(C.Structure):

@@ -355,6 +355,7 @@ void build_schedules(help_gpu* h) {
         if ((int32_t)h->ea.size() > off) {
             Op oe{OP_EA, off, (int32_t)h->ea.size() - off, w};
             for (int32_t s : at_start[p]) if (S.snrow[s] >= kEaDeepRows) oe.depth = 8;
+            if (h->opts.ea_depth == 4 || h->opts.ea_depth == 8) oe.depth = h->opts.ea_depth;
             h->fops.push_back(oe);
         }
         // the panel of every active front

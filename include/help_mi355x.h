@@ -93,6 +93,10 @@ typedef struct help_opts {
                                 spread only grows along an IPM run, and each
                                 step costs a triangular-solve pair); 0: only a
                                 refinement failure switches                    */
+    int32_t ea_depth;        /* (testing) extend-add child rows per lane in
+                                flight: 0 auto (8 in the launches that start a
+                                front of at least 2048 rows, else 4); 4 or 8
+                                for every launch                               */
 } help_opts;
 #define HELP_ORDER_AMD 0
 #define HELP_ORDER_ND  1

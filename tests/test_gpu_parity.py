@@ -384,16 +384,18 @@ def _first_bad_supernode(be, inst, d):
     return None
 
 
-@pytest.mark.parametrize("path", [0, 1, 2])
-def test_factor_paths_diagnostic(cfg0, path):
+@pytest.mark.parametrize("path,depth", [(0, 0), (1, 0), (2, 0), (1, 4), (1, 8)])
+def test_factor_paths_diagnostic(cfg0, path, depth):
     """Every supernode's columns against the oracle with the automatic front
     schedule (0), every front through the tiled kernels (1), and every small
-    front that fits one wave through the 64-lane one-wave kernel (2)."""
-    be = LinearEquation(front_path=path)
+    front that fits one wave through the 64-lane one-wave kernel (2); with
+    every front tiled, the extend-add also with 4 and with 8 child rows per
+    lane in flight in every launch (depth 0: by front size)."""
+    be = LinearEquation(front_path=path, ea_depth=depth)
     be.construct(cfg0)
     rng = np.random.default_rng(4)
     d = log_uniform(rng, cfg0.n, -2, 2)
-    _flush(f"renew cfg0 front_path={path}")
+    _flush(f"renew cfg0 front_path={path} ea_depth={depth}")
     be.renew(d)
     bad = _first_bad_supernode(be, cfg0, d)
     _flush(f"  tiny pivots {be.renew_stats.n_tiny_pivots}; first bad supernode "

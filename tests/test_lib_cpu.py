@@ -43,7 +43,7 @@ def test_options_struct_matches_the_header():
     assert o.refine_max == 20 and o.refine_tol == 1e-12
     assert o.pcg_k == 500 and o.pcg_max_iter == 5000 and o.pcg_tol == 1e-10 and o.pcg_fp32_precond == 1
     assert o.ordering == L.ORDER_AMD and o.factor_graph == 0 and o.inverse_solve == 0
-    assert o.refine_switch == 2.5
+    assert o.refine_switch == 2.5 and o.ea_depth == 0
     fields = re.search(r"typedef struct help_opts \{(.*?)\} help_opts;", open(HEADER).read(), re.S).group(1)
     fields = re.sub(r"/\*.*?\*/", "", fields, flags=re.S)
     declared = re.findall(r"\b(?:int32_t|double)\s+(\w+)\s*;", fields)
