@@ -650,6 +650,8 @@ void k_potrf_diag(const PanelTask* __restrict__ tasks, DevSym sym, T* __restrict
 // j + 16(i/4), element i%4 for V_MFMA_F32_16X16X4_F32.
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;   // LDS and global address spaces
+typedef __attribute__((address_space(1))) void glb_void;   // (GLOBAL_LOAD_LDS operands)
 
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
@@ -925,7 +927,7 @@ constexpr int kKc = 16;   // inner-dimension chunk staged in LDS
 // and the next chunk's global loads are in flight while the MFMAs consume
 // the current one (one barrier per chunk).
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks, DevSym sym,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void k_gemm(const GemmTask* __restrict__ tasks, DevSym sym,
                                               T* __restrict__ L, T* __restrict__ U, T* __restrict__ Uacc) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int kPer = kKc / 4;
@@ -966,6 +968,36 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
                 }
     }
     if constexpr (sizeof(T) == 4) {
+      if (tk.row0 + kTile <= c && tk.col0 + kTile <= c) {   // uniform over the workgroup
+        // FP32 interior tile: the chunks go from HBM straight into LDS
+        // (GLOBAL_LOAD_LDS_DWORDX4: lane l of a wave lands 16 bytes at the
+        // wave's LDS base + 16 l).  Wave w moves columns 4w .. 4w+3 of the
+        // chunk, 16 lanes per column with 4 rows each, which is the k-major
+        // chunk layout.  Columns past the panel read its last column and are
+        // zeroed in LDS once they have landed (last chunk only).  The chunks
+        // hold the same values as the register-staged forms.
+        const int kc = 4 * wave + (lane >> 4), rr = (lane & 15) * 4;
+        const T* pa = Lp + tk.row0 + rr;
+        const T* pb = Lp + tk.col0 + rr;
+        auto issue = [&](int32_t kb, int b) {
+            const int64_t o = (int64_t)min(kb + kc, tk.k1 - 1) * c;
+            __builtin_amdgcn_global_load_lds((const glb_void*)(pa + o), (lds_void*)&As[b][4 * wave * kTile], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const glb_void*)(pb + o), (lds_void*)&Bs[b][4 * wave * kTile], 16, 0, 0);
+        };
+        issue(tk.k0, 0);
+        int buf = 0;
+        for (int32_t kb = tk.k0; kb < tk.k1; kb += kKc, buf ^= 1) {
+            __syncthreads();   // this chunk has landed; the other buffer's readers are done
+            if (kb + kKc < tk.k1) {
+                issue(kb + kKc, buf ^ 1);
+            } else if (tk.k1 - kb < kKc) {
+                for (int idx = tid; idx < kKc * kTile; idx += 256)
+                    if (kb + idx / kTile >= tk.k1) { As[buf][idx] = T(0); Bs[buf][idx] = T(0); }
+                __syncthreads();
+            }
+            mfma_tile<T>(As[buf], Bs[buf], kKc, lane, wr, wc, acc);
+        }
+      } else {
         // FP32: each thread moves 4 consecutive rows of one column of the
         // chunk with one 16-byte load (4-byte aligned: the column stride c is
         // arbitrary, and dword alignment is what multi-dword global loads
@@ -1004,6 +1036,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* __restrict__ tasks
             if (kb + kKc < tk.k1) load(kb + kKc);
             mfma_tile<T>(As[buf], Bs[buf], kKc, lane, wr, wc, acc);
         }
+      }
     } else {
         const int li = tid & 63, lk = tid >> 6;
         const int32_t ia = tk.row0 + li, ib = tk.col0 + li;
