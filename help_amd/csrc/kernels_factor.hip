@@ -925,9 +925,11 @@ constexpr int kKc = 16;   // inner-dimension chunk staged in LDS
 // C[row0:+64, col0:+64] -= P[rows, k0:k1] * P[cols, k0:k1]^T, lower part only,
 // with P the L panel of the front.  The k chunks are double-buffered in LDS
 // and the next chunk's global loads are in flight while the MFMAs consume
-// the current one (one barrier per chunk).
+// the current one (one barrier per chunk).  At least 4 waves per SIMD are
+// asked of the register allocation (the interior tiles' loads into LDS need
+// no staging registers).
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void k_gemm(const GemmTask* __restrict__ tasks, DevSym sym,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_gemm(const GemmTask* __restrict__ tasks, DevSym sym,
                                               T* __restrict__ L, T* __restrict__ U, T* __restrict__ Uacc) {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int kPer = kKc / 4;
@@ -967,22 +969,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
                     acc[a][b][r] = Ua[ok ? (int64_t)(j - n) * cu + (i - n) : 0];
                 }
     }
-    if constexpr (sizeof(T) == 4) {
-      if (tk.row0 + kTile <= c && tk.col0 + kTile <= c) {   // uniform over the workgroup
-        // FP32 interior tile: the chunks go from HBM straight into LDS
+    if (tk.row0 + kTile <= c && tk.col0 + kTile <= c) {   // uniform over the workgroup
+        // Interior tile: the chunks go from HBM straight into LDS
         // (GLOBAL_LOAD_LDS_DWORDX4: lane l of a wave lands 16 bytes at the
         // wave's LDS base + 16 l).  Wave w moves columns 4w .. 4w+3 of the
-        // chunk, 16 lanes per column with 4 rows each, which is the k-major
-        // chunk layout.  Columns past the panel read its last column and are
+        // chunk, 64 / kV lanes per column with kV rows each (kV = 4 FP32, 2
+        // FP64: one or two loads per operand), which is the k-major chunk
+        // layout.  Columns past the panel read its last column and are
         // zeroed in LDS once they have landed (last chunk only).  The chunks
         // hold the same values as the register-staged forms.
-        const int kc = 4 * wave + (lane >> 4), rr = (lane & 15) * 4;
+        constexpr int kV = 16 / (int)sizeof(T);
+        constexpr int kLc = kTile / kV;        // lanes per column
+        constexpr int kCpi = 64 / kLc;         // columns per load
+        const int rr = (lane % kLc) * kV;
         const T* pa = Lp + tk.row0 + rr;
         const T* pb = Lp + tk.col0 + rr;
         auto issue = [&](int32_t kb, int b) {
-            const int64_t o = (int64_t)min(kb + kc, tk.k1 - 1) * c;
-            __builtin_amdgcn_global_load_lds((const glb_void*)(pa + o), (lds_void*)&As[b][4 * wave * kTile], 16, 0, 0);
-            __builtin_amdgcn_global_load_lds((const glb_void*)(pb + o), (lds_void*)&Bs[b][4 * wave * kTile], 16, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4 / kCpi; ++q) {
+                const int k = 4 * wave + q * kCpi;
+                const int64_t o = (int64_t)min(kb + k + lane / kLc, tk.k1 - 1) * c;
+                __builtin_amdgcn_global_load_lds((const glb_void*)(pa + o), (lds_void*)&As[b][k * kTile], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds((const glb_void*)(pb + o), (lds_void*)&Bs[b][k * kTile], 16, 0, 0);
+            }
         };
         issue(tk.k0, 0);
         int buf = 0;
@@ -997,7 +1006,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
             }
             mfma_tile<T>(As[buf], Bs[buf], kKc, lane, wr, wc, acc);
         }
-      } else {
+    } else if constexpr (sizeof(T) == 4) {
+      {
         // FP32: each thread moves 4 consecutive rows of one column of the
         // chunk with one 16-byte load (4-byte aligned: the column stride c is
         // arbitrary, and dword alignment is what multi-dword global loads
