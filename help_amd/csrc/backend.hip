@@ -1139,8 +1139,17 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
             side.join();
             if (side_err) std::rethrow_exception(side_err);
             phase("adat pattern (rest)");
+            // (the FP32 mode's row of every ADA^T slot, for the scaled
+            // assembly, is filled on the same thread as the schedules)
             std::thread sched([&] {
-                try { build_schedules(h); } catch (...) { side_err = std::current_exception(); }
+                try {
+                    build_schedules(h);
+                    if (h->use_fp32) {
+                        h->slot_row.resize(h->P.colind.size());
+                        for (int32_t i = 0; i < m; ++i)
+                            for (int64_t t = h->P.rowptr[i]; t < h->P.rowptr[i + 1]; ++t) h->slot_row[t] = i;
+                    }
+                } catch (...) { side_err = std::current_exception(); }
             });
             struct Joiner join_sched{sched};
             build_assembly_maps(h);
@@ -1148,11 +1157,6 @@ int help_gpu_create(int32_t m, int32_t n, int64_t nnz, const int64_t* colptr,
             sched.join();
             if (side_err) std::rethrow_exception(side_err);
             phase("schedules (rest)");
-            if (h->use_fp32) {
-                h->slot_row.resize(h->P.colind.size());
-                for (int32_t i = 0; i < m; ++i)
-                    for (int64_t t = h->P.rowptr[i]; t < h->P.rowptr[i + 1]; ++t) h->slot_row[t] = i;
-            }
         }
         if (!h->opts.symbolic_only) setup_device(h);
         phase("device setup");
