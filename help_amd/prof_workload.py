@@ -28,7 +28,8 @@ def _solve(args):
     from .backend import LinearEquation
     inst = instances.by_config(args.config)
     mode = {"fp64": L.MODE_DIRECT_FP64, "fp32": L.MODE_DIRECT_FP32, "pcg": L.MODE_PCG}[args.mode]
-    extra = {"pcg_max_iter": args.pcg_max_iter} if args.mode == "pcg" else {"ordering": args.ordering}
+    extra = ({"pcg_max_iter": args.pcg_max_iter} if args.mode == "pcg"
+             else {"ordering": args.ordering, "ea_depth": args.ea_depth})
     be = LinearEquation(mode=mode, **extra)
     be.construct(inst)
     rng = np.random.default_rng(args.seed)
@@ -101,6 +102,8 @@ def main(argv=None):
     s.add_argument("--solves", type=int, default=3)
     s.add_argument("--decades", type=float, default=2.0)
     s.add_argument("--seed", type=int, default=0)
+    s.add_argument("--ea-depth", type=int, default=0, choices=[0, 4, 8],
+                   help="direct modes: extend-add rows per lane in flight (0: by front size)")
     t = sub.add_parser("trace")
     t.add_argument("--config", type=int, default=2)
     t.add_argument("--mode", default="fp32", choices=["fp64", "fp32", "pcg"])
