@@ -613,8 +613,8 @@ void setup_device_common(help_gpu* h) {
     h->d_d.alloc(n); h->d_dinv.alloc(n); h->d_b1.alloc(2 * (size_t)n); h->d_x1.alloc(2 * (size_t)n);
     h->d_tn.alloc(n);
     h->d_b2.alloc(2 * (size_t)m); h->d_x2.alloc(2 * (size_t)m); h->d_rp.alloc(2 * (size_t)m);
-    h->d_t.alloc(2 * (size_t)m); h->d_r.alloc(m); h->d_c.alloc(2 * (size_t)m);
-    h->d_z.alloc(m); h->d_p.alloc(m); h->d_q.alloc(m);
+    h->d_t.alloc(2 * (size_t)m); h->d_r.alloc(2 * (size_t)m); h->d_c.alloc(2 * (size_t)m);
+    h->d_z.alloc(2 * (size_t)m); h->d_p.alloc(2 * (size_t)m); h->d_q.alloc(2 * (size_t)m);
     h->d_part.alloc(kDotPartials); h->d_scal.alloc(8);
     h->d_err.alloc(1);
     std::vector<help::RowChunk> chunks;
@@ -927,6 +927,106 @@ int pcg_factor(help_gpu* h, double& rel, int max_it) {
         launch_xpby(m, z, beta, p, st);
     }
     return it;
+}
+
+// pcg_factor for two right-hand sides at once (d_rp slots 0 and 1 on entry,
+// d_x2 slots 0 and 1 on exit): the two refinements run in lockstep and share
+// the factor's triangular solves (one batched solve per step while both are
+// running); every vector operation, dot and scalar test of a system is the
+// one pcg_factor makes, so each solution is the same bit for bit.
+template <typename T>
+void pcg_factor2(help_gpu* h, double rel[2], int it[2], int max_it) {
+    hipStream_t st = h->st;
+    const int32_t m = h->m, n = h->n;
+    const T* Lf = std::is_same<T, float>::value ? (const T*)h->d_L32.p : (const T*)h->d_L64.p;
+    const T* Df = std::is_same<T, float>::value ? (const T*)h->d_Dinv32.p : (const T*)h->d_Dinv64.p;
+    const bool f32 = std::is_same<T, float>::value;
+    double* rp = h->d_rp.p;
+    auto slot = [&](double* base, int r) { return base + (size_t)r * m; };
+    for (int r = 0; r < 2; ++r) {
+        launch_gather_perm(m, slot(rp, r), h->d_iperm.p, slot(h->d_c.p, r), st);
+        if (f32) launch_scale_perm(m, slot(rp, r), h->d_scale.p, h->d_iperm.p, st);
+    }
+    run_trisolve<T>(h, Lf, Df, rp, 2);
+    for (int r = 0; r < 2; ++r) {
+        if (f32) launch_scale_perm(m, slot(rp, r), h->d_scale.p, h->d_iperm.p, st);
+        launch_gather_perm(m, slot(rp, r), h->d_iperm.p, slot(h->d_x2.p, r), st);
+    }
+    launch_dinv(h->d_d.p, h->d_dinv.p, n, st);
+    auto matvec = [&](const double* v, double* out) {
+        launch_normal_matvec(m, n, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_colptr.p,
+                             h->d_rowind.p, h->d_val.p, h->d_dinv.p, h->opts.beta, v, h->d_tn.p, out, st);
+    };
+    // z_r = P (L L^T)^-1 P^T r_r for the systems in `act` (slots of d_rp in
+    // that order), one batched triangular solve
+    auto apply2 = [&](const int* act, int na) {
+        for (int a = 0; a < na; ++a) {
+            launch_scatter_perm(m, slot(h->d_r.p, act[a]), h->d_iperm.p, slot(rp, a), st);
+            if (f32) launch_scale_perm(m, slot(rp, a), h->d_scale.p, h->d_iperm.p, st);
+        }
+        run_trisolve<T>(h, Lf, Df, rp, na);
+        for (int a = 0; a < na; ++a) {
+            if (f32) launch_scale_perm(m, slot(rp, a), h->d_scale.p, h->d_iperm.p, st);
+            launch_gather_perm(m, slot(rp, a), h->d_iperm.p, slot(h->d_z.p, act[a]), st);
+        }
+    };
+    double rr0[2], rz[2];
+    bool run[2];
+    for (int r = 0; r < 2; ++r) {
+        matvec(slot(h->d_x2.p, r), slot(h->d_q.p, r));
+        launch_sub(m, slot(h->d_c.p, r), slot(h->d_q.p, r), slot(h->d_r.p, r), st);
+    }
+    for (int r = 0; r < 2; ++r) {
+        it[r] = 0;
+        rel[r] = 0;
+        run[r] = false;
+        rr0[r] = dot_dev(h, slot(h->d_c.p, r), slot(h->d_c.p, r), m);
+        if (rr0[r] == 0.0) continue;
+        const double rr = dot_dev(h, slot(h->d_r.p, r), slot(h->d_r.p, r), m);
+        rel[r] = std::sqrt(rr / rr0[r]);
+        run[r] = !(rel[r] <= h->opts.refine_tol || max_it <= 0);
+    }
+    int act[2], na = 0;
+    for (int r = 0; r < 2; ++r) if (run[r]) act[na++] = r;
+    if (na == 0) return;
+    apply2(act, na);
+    for (int a = 0; a < na; ++a) {
+        const int r = act[a];
+        HIPCHK(hipMemcpyAsync(slot(h->d_p.p, r), slot(h->d_z.p, r), sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+        rz[r] = dot_dev(h, slot(h->d_r.p, r), slot(h->d_z.p, r), m);
+    }
+    while (true) {
+        na = 0;
+        for (int r = 0; r < 2; ++r) {
+            if (!run[r]) continue;
+            if (!(it[r] < max_it && rz[r] > 0)) { run[r] = false; continue; }
+            ++it[r];
+            double* p = slot(h->d_p.p, r);
+            double* q = slot(h->d_q.p, r);
+            double* rv = slot(h->d_r.p, r);
+            matvec(p, q);
+            const double pq = dot_dev(h, p, q, m);
+            if (!(pq > 0)) { run[r] = false; continue; }
+            const double alpha = rz[r] / pq;
+            launch_axpy(m, alpha, p, slot(h->d_x2.p, r), st);
+            launch_axpy(m, -alpha, q, rv, st);
+            const double prev = rel[r];
+            const double rr = dot_dev(h, rv, rv, m);
+            rel[r] = std::sqrt(rr / rr0[r]);
+            if (rel[r] <= h->opts.refine_tol) { run[r] = false; continue; }
+            if (std::is_same<T, double>::value && rel[r] > 0.5 * prev) { run[r] = false; continue; }
+            act[na++] = r;
+        }
+        if (na == 0) break;
+        apply2(act, na);
+        for (int a = 0; a < na; ++a) {
+            const int r = act[a];
+            const double rz_new = dot_dev(h, slot(h->d_r.p, r), slot(h->d_z.p, r), m);
+            const double beta = rz_new / rz[r];
+            rz[r] = rz_new;
+            launch_xpby(m, slot(h->d_z.p, r), beta, slot(h->d_p.p, r), st);
+        }
+    }
 }
 
 // FP64 factor storage, allocated on first use in the FP32 mode (after a
@@ -1356,6 +1456,73 @@ int help_gpu_solve2(help_gpu* h, const double* d, const double* b1a, const doubl
     if (!h || !d || !b1a || !b2a || !x1a || !x2a || !b1b || !b2b || !x1b || !x2b) return HELP_ERR_ARG;
     if (!h->device || !h->factored) { h->err = "solve2: call renew first"; return HELP_ERR_STATE; }
     const int max_it = h->fp32_off ? h->opts.refine_max : h->opts.refine_fp64;
+    const bool lockstep = !h->use_pcg && h->factor_fp32 && h->opts.refine_max > 0 &&
+                          !(h->opts.fault_flags & (HELP_FAULT_FP32_BREAKDOWN | HELP_FAULT_REFINE_NOCONV));
+    if (lockstep) {
+        // FP32 factor: both refinements in lockstep over shared triangular
+        // solves (pcg_factor2).  When both reach refine_tol the results are
+        // those of two single solves bit for bit; otherwise the two single
+        // solves run after all (the first failure switches the run to the
+        // FP64 factor), so a failure behaves exactly as before.
+        try {
+            hipStream_t st = h->st;
+            const int32_t m = h->m, n = h->n;
+            const double* b1[2] = {b1a, b1b};
+            const double* b2[2] = {b2a, b2b};
+            double* x1[2] = {x1a, x1b};
+            double* x2[2] = {x2a, x2b};
+            HIPCHK(hipMemcpyAsync(h->d_d.p, d, sizeof(double) * n, hipMemcpyDefault, st));
+            for (int r = 0; r < 2; ++r) {
+                HIPCHK(hipMemcpyAsync(h->d_b1.p + (size_t)r * n, b1[r], sizeof(double) * n, hipMemcpyDefault, st));
+                HIPCHK(hipMemcpyAsync(h->d_b2.p + (size_t)r * m, b2[r], sizeof(double) * m, hipMemcpyDefault, st));
+            }
+            HIPCHK(hipEventRecord(h->ev[0], st));
+            {
+                ProfScope ps(h, HELP_K_RHS, 2 * h->work_rhs);
+                for (int r = 0; r < 2; ++r)
+                    launch_rhs(m, n, h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, h->d_b1.p + (size_t)r * n,
+                               h->d_d.p, h->d_b2.p + (size_t)r * m, h->d_iperm.p, h->d_tn.p,
+                               h->d_rp.p + (size_t)r * m, st);
+            }
+            HIPCHK(hipEventRecord(h->ev[1], st));
+            double rel[2];
+            int it[2];
+            pcg_factor2<float>(h, rel, it, h->opts.refine_max);
+            if (rel[0] <= h->opts.refine_tol && rel[1] <= h->opts.refine_tol) {
+                h->cyc_solves += 2;
+                h->cyc_steps += it[0] + it[1];
+                HIPCHK(hipEventRecord(h->ev[2], st));
+                {
+                    ProfScope ps(h, HELP_K_BACKSUB, 2 * h->work_backsub);
+                    for (int r = 0; r < 2; ++r)
+                        launch_backsub(n, h->d_colptr.p, h->d_rowind.p, h->d_val.p, h->d_b1.p + (size_t)r * n,
+                                       h->d_d.p, h->d_x2.p + (size_t)r * m, h->d_x1.p + (size_t)r * n, st);
+                }
+                HIPCHK(hipEventRecord(h->ev[3], st));
+                HIPCHK(hipGetLastError());
+                for (int r = 0; r < 2; ++r) {
+                    HIPCHK(hipMemcpyAsync(x1[r], h->d_x1.p + (size_t)r * n, sizeof(double) * n, hipMemcpyDefault, st));
+                    HIPCHK(hipMemcpyAsync(x2[r], h->d_x2.p + (size_t)r * m, sizeof(double) * m, hipMemcpyDefault, st));
+                }
+                HIPCHK(hipStreamSynchronize(st));
+                if (h->prof) collect_prof(h);
+                help_solve_stats loc{};
+                loc.ms_rhs = elapsed_ms(h->ev[0], h->ev[1]);
+                loc.ms_solve = elapsed_ms(h->ev[1], h->ev[2]);
+                loc.ms_back = elapsed_ms(h->ev[2], h->ev[3]);
+                help_solve_stats* ss[2] = {ssa, ssb};
+                for (int r = 0; r < 2; ++r) {
+                    if (!ss[r]) continue;
+                    *ss[r] = loc;
+                    ss[r]->refine_steps = it[r];
+                    ss[r]->rel_residual = rel[r];
+                }
+                return HELP_OK;
+            }
+        } catch (const std::exception& e) {
+            return fail(h, e);
+        }
+    }
     if (h->use_pcg || h->factor_fp32 || max_it > 0) {
         // iterative paths decide per right-hand side: two single solves
         // (a PCG solve stopped at its cap still leaves an iterate: the

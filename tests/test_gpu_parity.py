@@ -227,38 +227,44 @@ def test_solve_with_explicit_inverse(mode, inverse):
     be.destruct()
 
 
-@pytest.mark.parametrize("mode,inverse", [(L.MODE_DIRECT_FP64, 1), (L.MODE_DIRECT_FP64, -1),
-                                          (L.MODE_DIRECT_FP32, 1)])
-def test_solve2_matches_two_solves(mode, inverse):
+@pytest.mark.parametrize("mode,inverse,dec", [(L.MODE_DIRECT_FP64, 1, 2), (L.MODE_DIRECT_FP64, -1, 2),
+                                              (L.MODE_DIRECT_FP32, 1, 2), (L.MODE_DIRECT_FP32, 1, 6)])
+def test_solve2_matches_two_solves(mode, inverse, dec):
     """help_gpu_solve2 (two right-hand sides through each triangular-solve
-    launch in the FP64 mode, two single solves otherwise) returns exactly the
-    two single solves' results, on an instance with wide supernodes (explicit
-    inverse and block substitution) and many tiny ones; the first solution
-    also within 1e-10 of the oracle."""
+    launch: in the FP64 mode one factor solve each, in the FP32 mode the two
+    refinements in lockstep, with d over 6 decades for many steps) returns
+    exactly the two single solves' results and refinement counts, on an instance with wide supernodes (explicit inverse
+    and block substitution) and many tiny ones; the first solution also
+    within 1e-10 of the oracle."""
     inst = instances.mcf(side=60, commodities=4, pairs=10, seed=7)
     be = LinearEquation(mode=mode, inverse_solve=inverse)
     be.construct(inst)
     rng = np.random.default_rng(21)
-    d = log_uniform(rng, inst.n, -2, 2)
+    d = log_uniform(rng, inst.n, -dec, dec)
     b1a, b1b = rng.standard_normal(inst.n), rng.standard_normal(inst.n)
     b2a, b2b = rng.standard_normal(inst.m), rng.standard_normal(inst.m)
     be.renew(d)
     for rep in range(2):   # the second pass reuses the update buffers left by the first
         (x1a, x2a), (x1b, x2b) = be.solve2(d, b1a, b2a, b1b, b2b)
+        steps2 = be.solve_stats.refine_steps
         y1a, y2a = be.solve(d, b1a, b2a)
+        steps1 = be.solve_stats.refine_steps
         y1b, y2b = be.solve(d, b1b, b2b)
-        _flush(f"solve2 mode={mode} inverse={inverse} pass {rep}: max |diff| "
+        _flush(f"solve2 mode={mode} inverse={inverse} decades={dec} pass {rep}: refinement steps "
+               f"{steps2} / {steps1} (then {be.solve_stats.refine_steps}), max |diff| "
                f"{max(np.abs(x2a - y2a).max(), np.abs(x2b - y2b).max()):.3e}")
+        assert steps2 == steps1
         for u, v in ((x1a, y1a), (x2a, y2a), (x1b, y1b), (x2b, y2b)):
             assert np.array_equal(u, v)
         b1a, b1b = b1b, b1a   # swap the roles for the second pass
         b2a, b2b = b2b, b2a
-    perm, _, _ = be.symbolic_arrays()
-    Lp, Li, Lx = _oracle_factor(inst, perm, d)
-    o1, o2 = O.solve_kkt(inst, perm, Lp, Li, Lx, d, b1a, b2a)
-    e2 = np.linalg.norm(x2b - o2) / np.linalg.norm(o2)
-    _flush(f"  dy err {e2:.3e}")
-    assert e2 <= 1e-10
+    if dec == 2:   # the forward error bound holds for the moderately conditioned systems
+        perm, _, _ = be.symbolic_arrays()
+        Lp, Li, Lx = _oracle_factor(inst, perm, d)
+        o1, o2 = O.solve_kkt(inst, perm, Lp, Li, Lx, d, b1a, b2a)
+        e2 = np.linalg.norm(x2b - o2) / np.linalg.norm(o2)
+        _flush(f"  dy err {e2:.3e}")
+        assert e2 <= 1e-10
     be.destruct()
 
 
