@@ -301,6 +301,9 @@ def main():
                     help="mixed paths: FP64 refinement stops at this relative residual (library default 1e-12)")
     ap.add_argument("--refine-max", type=int, default=None,
                     help="mixed paths: refinement iterations per solve (library default 20)")
+    ap.add_argument("--refine-switch", type=float, default=None,
+                    help="mixed paths: average refinement steps per solve above which the next "
+                         "renews factor in FP64 (library default 2.5)")
     ap.add_argument("--inverse-solve", type=int, default=None, choices=[-1, 0, 1],
                     help="explicit L11 inverse for supernodes wider than 64 columns: "
                          "0 or 1 on (default), -1 off (block substitution)")
@@ -336,6 +339,8 @@ def main():
         mixed["refine_tol"] = args.refine_tol
     if args.refine_max is not None:
         mixed["refine_max"] = args.refine_max
+    if args.refine_switch is not None:
+        mixed["refine_switch"] = args.refine_switch
     results = {}
     for p in paths:
         results[p] = run_path(inst, p, args.steps, args.warmup, args.driver, path_opts(p, extra, mixed), L,
