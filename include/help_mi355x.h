@@ -154,8 +154,12 @@ int help_gpu_solve(help_gpu* h, const double* d, const double* b1,
 /* Two SolveLinearEquation calls (UseCholmod.cpp:157-238) with the same d
  * in one pass: (b1a, b2a) -> (x1a, x2a) and (b1b, b2b) -> (x1b, x2b).  In
  * the plain FP64 direct mode (refine_fp64 = 0) both right-hand sides go
- * through each triangular-solve launch together, with results identical to
- * two help_gpu_solve calls; the other modes make two help_gpu_solve calls.
+ * through each triangular-solve launch together; with the FP32 factor the
+ * two refinements run in lockstep over shared triangular solves (when either
+ * misses refine_tol the two single solves run instead, so a refinement
+ * failure switches to the FP64 factor as in help_gpu_solve).  The results
+ * are identical to two help_gpu_solve calls; the PCG modes make two
+ * help_gpu_solve calls.
  * HELP's driver calls the backend once per solve, so the reference has no
  * counterpart: this serves drivers that know two right-hand sides at once
  * (help_hsd_solve_gpu's bottom and predictor solves). */
