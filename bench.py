@@ -114,8 +114,9 @@ def roofline(rows, mode, traffic_file, workload_key):
 
 
 def host_cores():
-    """Threads the CPU baseline uses: OMP_NUM_THREADS when set (the GPU box
-    sets it to the host-core share of its GPU), else the affinity mask."""
+    """Threads the CPU baseline uses: OMP_NUM_THREADS when set (a machine
+    shared between GPUs sets it to one GPU's share of the host cores), else
+    the affinity mask."""
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
         return int(env)
